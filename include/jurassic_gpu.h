@@ -1,0 +1,274 @@
+/*
+  jurassic_gpu.h -- C-ABI of the MI355X forward model (libjurassic_gpu.so).
+
+  The three structs below are the caller-side data model of JURASSIC's
+  forward model, laid out field for field as in the reference header
+  (src/jurassic.h:488-538 atm_t, :541-675 ctl_t, :734-775 obs_t) with the
+  same compile-time capacities (src/jurassic.h:346-444).  A C caller that
+  already fills these structs for formod() can pass the same objects to
+  formod_gpu().  The capacities must agree between caller and library;
+  jur_gpu_abi_check() reports the sizes the library was built with.
+
+  Emissivity tables and filter functions are read from the files named by
+  ctl->tblbase exactly as the reference does (src/jurassic.c:4949-5110,
+  4907-4946); they are kept resident in HBM in a device context that is
+  keyed on the ctl content (no process-wide statics bound to the first ctl).
+*/
+#ifndef JURASSIC_GPU_H
+#define JURASSIC_GPU_H
+
+#include <stddef.h>
+
+/* ------------------------------------------------------------------
+   Capacities (same names and defaults as src/jurassic.h:346-444).
+   ------------------------------------------------------------------ */
+#ifndef NCL
+#define NCL 8
+#endif
+#ifndef ND
+#define ND 128
+#endif
+#ifndef NG
+#define NG 8
+#endif
+#ifndef NP
+#define NP 256
+#endif
+#ifndef NR
+#define NR 256
+#endif
+#ifndef NSF
+#define NSF 8
+#endif
+#ifndef NW
+#define NW 4
+#endif
+#ifndef LEN
+#define LEN 10000
+#endif
+#ifndef NLOS
+#define NLOS 4096
+#endif
+#ifndef NSHAPE
+#define NSHAPE 20000
+#endif
+#ifndef NFOV
+#define NFOV 5
+#endif
+#ifndef TBLNP
+#define TBLNP 41
+#endif
+#ifndef TBLNT
+#define TBLNT 30
+#endif
+#ifndef TBLNU
+#define TBLNU 320
+#endif
+#ifndef TBLNS
+#define TBLNS 1200
+#endif
+
+/* ------------------------------------------------------------------
+   Data model (field order and types as src/jurassic.h:488-775).
+   ------------------------------------------------------------------ */
+
+/* Atmospheric data (src/jurassic.h:488-538). */
+typedef struct {
+  int np;                 /* number of levels */
+  double time[NP];        /* s since 2000-01-01T00:00Z */
+  double z[NP];           /* altitude [km] */
+  double lon[NP];         /* [deg] */
+  double lat[NP];         /* [deg] */
+  double p[NP];           /* pressure [hPa] */
+  double t[NP];           /* temperature [K] */
+  double q[NG][NP];       /* volume mixing ratio [ppv] */
+  double k[NW][NP];       /* extinction [km^-1] */
+  double clz;             /* cloud layer height [km] */
+  double cldz;            /* cloud layer depth [km] */
+  double clk[NCL];        /* cloud layer extinction [km^-1] */
+  double sfz;             /* surface height [km] */
+  double sfp;             /* surface pressure [hPa] */
+  double sft;             /* surface temperature [K] */
+  double sfeps[NSF];      /* surface emissivity */
+} atm_t;
+
+/* Forward model control parameters (src/jurassic.h:541-675). */
+typedef struct {
+  int ng;
+  char emitter[NG][LEN];
+  int nd;
+  double nu[ND];
+  int nw;
+  int window[ND];
+  int ncl;
+  double clnu[NCL];
+  int nsf;
+  double sfnu[NSF];
+  int sftype;             /* 0 none, 1 emission, 2 downward, 3 solar */
+  double sfsza;
+  char tblbase[LEN];
+  int tblfmt;             /* 1 ASCII, 2 binary */
+  double hydz;
+  int ctm_co2;
+  int ctm_h2o;
+  int ctm_n2;
+  int ctm_o2;
+  int refrac;
+  double rayds;
+  double raydz;
+  char fov[LEN];
+  double retp_zmin;
+  double retp_zmax;
+  double rett_zmin;
+  double rett_zmax;
+  double retq_zmin[NG];
+  double retq_zmax[NG];
+  double retk_zmin[NW];
+  double retk_zmax[NW];
+  int ret_clz;
+  int ret_cldz;
+  int ret_clk;
+  int ret_sfz;
+  int ret_sfp;
+  int ret_sft;
+  int ret_sfeps;
+  int write_bbt;
+  int write_matrix;
+  int formod;             /* 0 CGA, 1 EGA, 2 RFM */
+  char rfmbin[LEN];
+  char rfmhit[LEN];
+  char rfmxsc[NG][LEN];
+} ctl_t;
+
+/* Observation geometry and radiance data (src/jurassic.h:734-775). */
+typedef struct {
+  int nr;
+  double time[NR];
+  double obsz[NR];
+  double obslon[NR];
+  double obslat[NR];
+  double vpz[NR];
+  double vplon[NR];
+  double vplat[NR];
+  double tpz[NR];
+  double tplon[NR];
+  double tplat[NR];
+  double tau[ND][NR];
+  double rad[ND][NR];
+} obs_t;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------
+   Status codes and options.
+   ------------------------------------------------------------------ */
+#define JUR_OK 0
+#define JUR_ERR_ARG 1         /* invalid argument or capacity exceeded */
+#define JUR_ERR_IO 2          /* a required input file cannot be read */
+#define JUR_ERR_LOS 3         /* "Too many LOS points!" */
+#define JUR_ERR_OBSERVER 4    /* "Observer below surface!" */
+#define JUR_ERR_HIP 5         /* HIP runtime failure */
+#define JUR_ERR_UNSUPPORTED 6 /* e.g. FORMOD = 2 (RFM) */
+#define JUR_ERR_MEM 7         /* device allocation failed */
+#define JUR_ERR_FOV 8         /* "Cannot apply FOV convolution!" */
+
+#define JUR_FP64 0            /* all arithmetic in FP64, reference order */
+#define JUR_FP32 1            /* FP32 lookups, FP64 geometry */
+
+#define JUR_JAC_FD 0          /* one-sided finite differences (kernel()) */
+#define JUR_JAC_ADJOINT 1     /* reverse sweep, path held fixed */
+
+/* Text of the last error of the calling thread ("" if none). */
+const char *jur_gpu_last_error(void);
+
+/* Size of each ABI struct as compiled into the library, for a caller-side
+   check that both sides were built with the same capacities.
+   Returns 0 when all three sizes match the arguments. */
+int jur_gpu_abi_check(size_t sizeof_ctl, size_t sizeof_atm, size_t sizeof_obs);
+
+/* ------------------------------------------------------------------
+   Drop-in entry points.
+   ------------------------------------------------------------------ */
+
+/* Replaces formod() (src/jurassic.h:890-893, src/jurassic.c:3026-3071).
+   Same inputs, outputs and error behaviour: on error it prints
+   "Error (file, func, line): msg" and calls exit(EXIT_FAILURE), as ERRMSG
+   does (src/jurassic.h:237-241).  Arithmetic is FP64. */
+void formod_gpu(const ctl_t *ctl, atm_t *atm, obs_t *obs);
+
+/* Status-returning form of formod_gpu with a precision selector
+   (JUR_FP64 or JUR_FP32).  Returns JUR_OK or an error code; on error obs
+   is left unchanged. */
+int jur_gpu_formod(const ctl_t *ctl, atm_t *atm, obs_t *obs, int precision);
+
+/* n independent (atm[i], obs[i]) packages with one ctl in one device pass,
+   the shape of the external GPU library's
+   jur_unified_formod_multiple_packages (src/formod.c:72-73). */
+int jur_gpu_formod_batch(const ctl_t *ctl, atm_t *atm, obs_t *obs, int n,
+                         int precision);
+
+/* Replaces kernel() (src/jurassic.h:1008-1012, src/jurassic.c:4004-4099):
+   k is the m x n Jacobian, row-major with row stride n (a gsl_matrix with
+   tda == size2).  m and n must equal the sizes of obs2y/atm2x.
+   Finite differences use the reference step sizes and one formod per
+   column, all columns in one device pass. */
+int jur_gpu_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
+                   size_t m, size_t n, int method, int precision);
+
+/* ERRMSG-style wrapper of jur_gpu_kernel(..., JUR_JAC_FD, JUR_FP64). */
+void kernel_gpu(ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m,
+                size_t n);
+
+/* ------------------------------------------------------------------
+   Device-resident batch interface (inputs and outputs in HBM).
+   ------------------------------------------------------------------ */
+typedef struct jur_ctx jur_ctx_t;
+
+/* Load tables and filter functions for ctl and build the device context. */
+int jur_ctx_create(const ctl_t *ctl, jur_ctx_t **ctx);
+
+/* As jur_ctx_create, with a channel list of any length (nd may exceed ND).
+   window[] may be NULL (all channels in window 0). */
+int jur_ctx_create_wide(const ctl_t *ctl, int nd, const double *nu,
+                        const int *window, jur_ctx_t **ctx);
+
+void jur_ctx_destroy(jur_ctx_t *ctx);
+
+/* Number of channels of the context. */
+int jur_ctx_nd(const jur_ctx_t *ctx);
+
+/* One forward pass over device-resident rays.
+   Rays are processed in blocks of 64 consecutive rays; ordering rays so
+   that each block holds similar geometries (e.g. one view angle or tangent
+   height over many profiles) lets the lanes of a wavefront share table
+   cache lines. */
+typedef struct {
+  int nprof;                  /* number of atmospheres */
+  atm_t *atm;                 /* device: nprof records (p updated if HYDZ>=0) */
+  int nray;                   /* number of rays */
+  const int *ray_prof;        /* device: atmosphere index of each ray */
+  const double *ray_geom;     /* device: [nray][7] time obsz obslon obslat vpz vplon vplat */
+  const unsigned char *mask;  /* device or NULL: [nray][nd], nonzero -> NaN radiance */
+  double *rad;                /* device out: [nray][nd] radiance or BT */
+  double *tau;                /* device out: [nray][nd] transmittance */
+  double *tp;                 /* device out or NULL: [nray][3] tpz tplon tplat */
+} jur_batch_t;
+
+int jur_ctx_formod_device(jur_ctx_t *ctx, const jur_batch_t *batch,
+                          int precision, void *hip_stream);
+
+/* Kernel timing of the last jur_ctx_formod_device call on this context, in
+   milliseconds, measured with HIP events on the stream the kernels ran on:
+   [0] raytrace, [1] radiance (table lookups + accumulation). */
+int jur_ctx_last_timing(const jur_ctx_t *ctx, double *ms, int n);
+
+/* Mean number of LOS points per ray of the last pass. */
+double jur_ctx_last_mean_np(const jur_ctx_t *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

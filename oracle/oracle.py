@@ -1,0 +1,182 @@
+"""ctypes bindings of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py as the checker.  The product library never
+uses it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    """Compile liboracle.so (gcc, a few seconds)."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        import sys
+        sys.path.insert(0, os.path.dirname(_HERE))
+        from jurassic_amd import abi  # noqa: F401  (types for argtypes)
+        L = C.CDLL(path)
+        P = C.c_void_p
+        D = C.c_double
+        I = C.c_int
+        S = C.c_size_t
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.orc_formod.argtypes = [P, P, P]
+        L.orc_formod.restype = I
+        L.orc_formod_batch.argtypes = [P, P, P, I, I]
+        L.orc_formod_batch.restype = I
+        L.orc_kernel.argtypes = [P, P, P, dp, S, S, I]
+        L.orc_kernel.restype = I
+        L.orc_kernel_central.argtypes = [P, P, P, dp, S, S, I]
+        L.orc_kernel_central.restype = I
+        L.orc_atm2x.argtypes = [P, P, dp, ip, ip]
+        L.orc_atm2x.restype = S
+        L.orc_x2atm.argtypes = [P, dp, P]
+        L.orc_obs2y.argtypes = [P, P, dp, ip, ip]
+        L.orc_obs2y.restype = S
+        L.orc_locate_irr.argtypes = [dp, I, D]
+        L.orc_locate_irr.restype = I
+        L.orc_locate_reg.argtypes = [dp, I, D]
+        L.orc_locate_reg.restype = I
+        L.orc_locate_tbl.argtypes = [C.POINTER(C.c_float), I, D]
+        L.orc_locate_tbl.restype = I
+        for f in ("orc_intpol_col_eps", "orc_intpol_col_u"):
+            getattr(L, f).argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), I, D]
+            getattr(L, f).restype = D
+        L.orc_ctmco2.argtypes = [D, D, D, D]
+        L.orc_ctmco2.restype = D
+        L.orc_ctmh2o.argtypes = [D, D, D, D, D]
+        L.orc_ctmh2o.restype = D
+        L.orc_ctmn2.argtypes = [D, D, D]
+        L.orc_ctmn2.restype = D
+        L.orc_ctmo2.argtypes = [D, D, D]
+        L.orc_ctmo2.restype = D
+        L.orc_planck.argtypes = [D, D]
+        L.orc_planck.restype = D
+        L.orc_bright.argtypes = [D, D]
+        L.orc_bright.restype = D
+        L.orc_sza.argtypes = [D, D, D]
+        L.orc_sza.restype = D
+        L.orc_srcfunc_table.argtypes = [P, dp, dp]
+        L.orc_srcfunc_table.restype = I
+        L.orc_trace_ray.argtypes = [P, P, P, I, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_trace_ray.restype = I
+        L.orc_hydrostatic.argtypes = [P, P]
+        L.orc_reset.argtypes = []
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_optimal_estimation.argtypes = [P, P, P, P, P, P, dp, ip, P, P]
+        L.orc_optimal_estimation.restype = I
+        L.orc_set_cov_apr.argtypes = [P, P, P, ip, ip, S, dp]
+        L.orc_set_cov_apr.restype = I
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def formod(ctl, atm, obs):
+    rc = lib().orc_formod(C.byref(ctl), C.byref(atm), C.byref(obs))
+    if rc:
+        raise RuntimeError("orc_formod: %s" % lib().orc_last_error().decode())
+
+
+def formod_batch(ctl, atms, obss, nthreads=0):
+    n = len(atms)
+    rc = lib().orc_formod_batch(C.byref(ctl), C.cast(atms, C.c_void_p),
+                                C.cast(obss, C.c_void_p), n, nthreads)
+    if rc:
+        raise RuntimeError("orc_formod_batch: %s" % lib().orc_last_error().decode())
+
+
+def sizes(ctl, atm, obs):
+    n = lib().orc_atm2x(C.byref(ctl), C.byref(atm), None, None, None)
+    m = lib().orc_obs2y(C.byref(ctl), C.byref(obs), None, None, None)
+    return m, n
+
+
+def kernel(ctl, atm, obs, central=False, nthreads=0):
+    m, n = sizes(ctl, atm, obs)
+    k = np.zeros((m, n))
+    f = lib().orc_kernel_central if central else lib().orc_kernel
+    rc = f(C.byref(ctl), C.byref(atm), C.byref(obs), _ptr(k), m, n, nthreads)
+    if rc:
+        raise RuntimeError("orc_kernel: %s" % lib().orc_last_error().decode())
+    return k
+
+
+def trace_ray(ctl, atm, obs, ir):
+    from jurassic_amd import abi
+    z, p, t, ds, lon, lat = (np.zeros(abi.NLOS) for _ in range(6))
+    u = np.zeros(abi.NLOS * max(ctl.ng, 1))
+    sft = C.c_double()
+    np_ = lib().orc_trace_ray(C.byref(ctl), C.byref(atm), C.byref(obs), ir,
+                             _ptr(z), _ptr(p), _ptr(t), _ptr(ds), _ptr(u),
+                             _ptr(lon), _ptr(lat), C.byref(sft))
+    if np_ < 0:
+        raise RuntimeError("orc_trace_ray: %s" % lib().orc_last_error().decode())
+    g = max(ctl.ng, 1)
+    return dict(z=z[:np_], p=p[:np_], t=t[:np_], ds=ds[:np_],
+                u=u[:np_ * g].reshape(np_, g), lon=lon[:np_], lat=lat[:np_],
+                sft=sft.value, np=np_)
+
+
+def srcfunc_table(ctl):
+    from jurassic_amd import abi
+    st = np.zeros(abi.TBLNS)
+    sr = np.zeros(abi.TBLNS * ctl.nd)
+    rc = lib().orc_srcfunc_table(C.byref(ctl), _ptr(st), _ptr(sr))
+    if rc:
+        raise RuntimeError("orc_srcfunc_table: %s" % lib().orc_last_error().decode())
+    return st, sr.reshape(abi.TBLNS, ctl.nd)
+
+
+class orc_ret_t(C.Structure):
+    from jurassic_amd.abi import ND as _ND, NG as _NG, NW as _NW, NCL as _NCL, NSF as _NSF
+    _fields_ = [
+        ("kernel_recomp", C.c_int), ("conv_itmax", C.c_int), ("conv_dmin", C.c_double),
+        ("err_formod", C.c_double * _ND), ("err_noise", C.c_double * _ND),
+        ("err_press", C.c_double), ("err_press_cz", C.c_double), ("err_press_ch", C.c_double),
+        ("err_temp", C.c_double), ("err_temp_cz", C.c_double), ("err_temp_ch", C.c_double),
+        ("err_q", C.c_double * _NG), ("err_q_cz", C.c_double * _NG), ("err_q_ch", C.c_double * _NG),
+        ("err_k", C.c_double * _NW), ("err_k_cz", C.c_double * _NW), ("err_k_ch", C.c_double * _NW),
+        ("err_clz", C.c_double), ("err_cldz", C.c_double), ("err_clk", C.c_double * _NCL),
+        ("err_sfz", C.c_double), ("err_sfp", C.c_double), ("err_sft", C.c_double),
+        ("err_sfeps", C.c_double * _NSF),
+    ]
+
+
+FWD_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                     C.POINTER(C.c_double))
+
+
+def optimal_estimation(ret, ctl, obs_meas, atm_apr, fwd=None):
+    """Run the oracle LM retrieval; returns (atm_i, obs_i, chisq_hist, niter)."""
+    from jurassic_amd import abi
+    obs_i = abi.obs_t()
+    atm_i = abi.atm_t()
+    hist = np.zeros(ret.conv_itmax + 2)
+    nit = C.c_int()
+    cb = FWD_FN(fwd) if fwd is not None else None
+    rc = lib().orc_optimal_estimation(C.byref(ret), C.byref(ctl), C.byref(obs_meas),
+                                      C.byref(obs_i), C.byref(atm_apr), C.byref(atm_i),
+                                      _ptr(hist), C.byref(nit),
+                                      C.cast(cb, C.c_void_p) if cb else None, None)
+    if rc:
+        raise RuntimeError("orc_optimal_estimation: %s" % lib().orc_last_error().decode())
+    return atm_i, obs_i, hist[:nit.value + 1], nit.value
