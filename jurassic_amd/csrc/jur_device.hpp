@@ -1,0 +1,301 @@
+// Device helpers of the FP64 path.  Every function keeps the expression
+// tree and the float/double promotions of the reference function it names,
+// with multiply-add contraction disabled, so that the FP64 kernels agree with
+// the reference to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "jur_internal.hpp"
+
+namespace jur {
+
+#define JUR_FP_EXACT _Pragma("clang fp contract(off)")
+
+// LIN (src/jurassic.h:164-165) on doubles.
+__device__ __forceinline__ double lin(double x0, double y0, double x1, double y1, double x) {
+  JUR_FP_EXACT
+  return y0 + (y1 - y0) / (x1 - x0) * (x - x0);
+}
+
+// LIN with float table entries: the slope is formed in float arithmetic, then
+// multiplied by the double (x - x0), as C promotion does in the reference.
+__device__ __forceinline__ double lin_f(float x0, float y0, float x1, float y1, double x) {
+  JUR_FP_EXACT
+  const float s = (y1 - y0) / (x1 - x0);
+  return (double)y0 + (double)s * (x - (double)x0);
+}
+
+// LOGY (src/jurassic.h:174-177).
+__device__ __forceinline__ double logy(double x0, double y0, double x1, double y1, double x) {
+  JUR_FP_EXACT
+  return (y1 / y0 > 0) ? (y0 * exp(log(y1 / y0) / (x1 - x0) * (x - x0))) : lin(x0, y0, x1, y1, x);
+}
+
+// LOGX (src/jurassic.h:168-171).
+__device__ __forceinline__ double logx(double x0, double y0, double x1, double y1, double x) {
+  JUR_FP_EXACT
+  return (x / x0 > 0 && x1 / x0 > 0) ? (y0 + (y1 - y0) * log(x / x0) / log(x1 / x0))
+                                      : lin(x0, y0, x1, y1, x);
+}
+
+// geo2cart (src/jurassic.c:3500-3516).
+__device__ __forceinline__ void geo2cart(double z, double lon, double lat, double *x) {
+  JUR_FP_EXACT
+  const double radius = z + JUR_RE;
+  const double latrad = lat / 180. * JUR_PI;
+  const double lonrad = lon / 180. * JUR_PI;
+  const double coslat = cos(latrad);
+  x[0] = radius * coslat * cos(lonrad);
+  x[1] = radius * coslat * sin(lonrad);
+  x[2] = radius * sin(latrad);
+}
+
+// cart2geo (src/jurassic.c:108-119).
+__device__ __forceinline__ void cart2geo(const double *x, double *z, double *lon, double *lat) {
+  JUR_FP_EXACT
+  const double radius = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  *lat = asin(x[2] / radius) * (180.0 / JUR_PI);
+  *lon = atan2(x[1], x[0]) * (180.0 / JUR_PI);
+  *z = radius - JUR_RE;
+}
+
+// locate_irr (src/jurassic.c:4103-4129): bisection, either grid order.
+template <typename T>
+__device__ __forceinline__ int locate_irr(const T *xx, int n, double x) {
+  int ilo = 0, ihi = n - 1, i = (ihi + ilo) >> 1;
+  if ((double)xx[i] < (double)xx[i + 1])
+    while (ihi > ilo + 1) {
+      i = (ihi + ilo) >> 1;
+      if ((double)xx[i] > x) ihi = i; else ilo = i;
+    }
+  else
+    while (ihi > ilo + 1) {
+      i = (ihi + ilo) >> 1;
+      if ((double)xx[i] <= x) ihi = i; else ilo = i;
+    }
+  return ilo;
+}
+
+// locate_reg (src/jurassic.c:4133-4148).
+__device__ __forceinline__ int locate_reg(const double *xx, int n, double x) {
+  JUR_FP_EXACT
+  const int i = (int)((x - xx[0]) / (xx[1] - xx[0]));
+  return i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+}
+
+// ------------------------------------------------------------------
+// Column search on a table column of n >= 2 records.
+// C selects the searched component: 0 = u (rec.x / rec.z), 1 = eps (rec.y / rec.w).
+// Returns i in [0, n-2] with (i == 0 or v_i <= x) and (i == n-2 or v_{i+1} > x),
+// which for a non-decreasing column is the index the reference bisection
+// locate_tbl (src/jurassic.c:4152-4170) returns.  The walk starts at the
+// previous segment's index `g`, which is usually already right.
+// ------------------------------------------------------------------
+template <int C>
+__device__ __forceinline__ float lo_of(const float4 &r) { return C == 0 ? r.x : r.y; }
+template <int C>
+__device__ __forceinline__ float hi_of(const float4 &r) { return C == 0 ? r.z : r.w; }
+
+template <int C>
+__device__ __forceinline__ int col_walk(const float4 *col, int n, double x, int g, float4 &r) {
+  g = g < 0 ? 0 : (g > n - 2 ? n - 2 : g);
+  r = col[g];
+  if (g > 0 && (double)lo_of<C>(r) > x) {
+    // answer < g: gallop left to a lower bound, then bisect
+    int hi = g, step = 1, lo;
+    float4 rl;
+    for (;;) {
+      int j = hi - step;
+      if (j <= 0) { lo = 0; rl = col[0]; break; }
+      float4 rj = col[j];
+      if ((double)lo_of<C>(rj) <= x) { lo = j; rl = rj; break; }
+      hi = j;
+      step <<= 1;
+    }
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      const float4 rm = col[mid];
+      if ((double)lo_of<C>(rm) > x) hi = mid; else { lo = mid; rl = rm; }
+    }
+    r = rl;
+    return lo;
+  }
+  if (g < n - 2 && (double)hi_of<C>(r) <= x) {
+    // answer > g: gallop right to an upper bound, then bisect
+    int lo = g, step = 1, hi;
+    float4 rh;
+    for (;;) {
+      int j = lo + step;
+      if (j >= n - 2) { hi = n - 2; rh = col[n - 2]; break; }
+      float4 rj = col[j];
+      if ((double)hi_of<C>(rj) > x) { hi = j; rh = rj; break; }
+      lo = j;
+      step <<= 1;
+    }
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      const float4 rm = col[mid];
+      if ((double)hi_of<C>(rm) > x) { hi = mid; rh = rm; } else lo = mid;
+    }
+    r = rh;
+    return hi;
+  }
+  return g;
+}
+
+// The reference bisection itself (for columns that are not monotone).
+template <int C>
+__device__ __forceinline__ int col_bisect(const float4 *col, int n, double x, float4 &r) {
+  int ilo = 0, ihi = n - 1, i;
+  while (ihi > ilo + 1) {
+    i = (ihi + ilo) >> 1;
+    if ((double)lo_of<C>(col[i]) > x) ihi = i; else ilo = i;
+  }
+  r = col[ilo];
+  return ilo;
+}
+
+// intpol_tbl_eps on one column (src/jurassic.c:3895-3929).
+__device__ __forceinline__ double col_eps(const float4 *col, int n, double x, int mono, int &g) {
+  JUR_FP_EXACT
+  const float4 r0 = col[0];
+  if (x < (double)r0.x) return lin_f(0.f, 0.f, r0.x, r0.y, x);
+  const float4 rl = col[n - 1];
+  if (x > (double)rl.x) {
+    const double a = log((double)(1.f - rl.y)) / (double)rl.x;
+    return 1 - exp(a * x);
+  }
+  float4 r;
+  g = mono ? col_walk<0>(col, n, x, g, r) : col_bisect<0>(col, n, x, r);
+  return lin_f(r.x, r.y, r.z, r.w, x);
+}
+
+// intpol_tbl_u on one column (src/jurassic.c:3933-3967).
+__device__ __forceinline__ double col_u(const float4 *col, int n, double x, int mono, int &g) {
+  JUR_FP_EXACT
+  const float4 r0 = col[0];
+  if (x < (double)r0.y) return lin_f(0.f, 0.f, r0.y, r0.x, x);
+  const float4 rl = col[n - 1];
+  if (x > (double)rl.y) {
+    const double a = log((double)(1.f - rl.y)) / (double)rl.x;
+    return log(1 - x) / a;
+  }
+  float4 r;
+  g = mono ? col_walk<1>(col, n, x, g, r) : col_bisect<1>(col, n, x, r);
+  return lin_f(r.y, r.x, r.w, r.z, x);
+}
+
+// ------------------------------------------------------------------
+// Continua (src/jurassic.c:886-2918), FP64, reference expression trees.
+// ------------------------------------------------------------------
+__device__ __forceinline__ double ctmco2(const CtmView &c, double nu, double p, double t, double u) {
+  JUR_FP_EXACT
+  const double xw = nu / 2 + 1;
+  if (xw >= 1 && xw < 2001) {
+    const int iw = (int)xw;
+    const double dw = xw - iw;
+    const double ew = 1 - dw;
+    const double cw296 = ew * c.co2_296[iw - 1] + dw * c.co2_296[iw];
+    const double cw260 = ew * c.co2_260[iw - 1] + dw * c.co2_260[iw];
+    const double cw230 = ew * c.co2_230[iw - 1] + dw * c.co2_230[iw];
+    const double dt230 = t - 230;
+    const double dt260 = t - 260;
+    const double dt296 = t - 296;
+    const double ctw = dt260 * 5.050505e-4 * dt296 * cw230 - dt230 * 9.259259e-4 * dt296 * cw260 +
+                       dt230 * 4.208754e-4 * dt260 * cw296;
+    return u / JUR_NA / 1000 * p / JUR_P0 * ctw;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ double ctmh2o(const CtmView &c, double nu, double p, double t, double q,
+                                         double u) {
+  JUR_FP_EXACT
+  double sfac;
+  const double xw = nu / 10 + 1;
+  if (xw >= 1 && xw < 2001) {
+    const int iw = (int)xw;
+    const double dw = xw - iw;
+    const double ew = 1 - dw;
+    const double cw296 = ew * c.h2o_296[iw - 1] + dw * c.h2o_296[iw];
+    const double cw260 = ew * c.h2o_260[iw - 1] + dw * c.h2o_260[iw];
+    const double cwfrn = ew * c.h2o_frn[iw - 1] + dw * c.h2o_frn[iw];
+    if (nu <= 820 || nu >= 960) {
+      sfac = 1;
+    } else {
+      const double xx = (nu - 820) / 10;
+      const int ix = (int)xx;
+      const double dx = xx - ix;
+      sfac = (1 - dx) * c.h2o_xfcrev[ix] + dx * c.h2o_xfcrev[ix + 1];
+    }
+    const double ctwslf = sfac * cw296 * pow(cw260 / cw296, (296 - t) / (296 - 260));
+    const double vf2 = (nu - 370) * (nu - 370);
+    const double vf6 = vf2 * vf2 * vf2;
+    const double fscal = 36100 / (vf2 + vf6 * 1e-8 + 36100) * -.25 + 1;
+    const double ctwfrn = cwfrn * fscal;
+    const double a1 = nu * u * tanh(.7193876 / t * nu);
+    const double a2 = 296 / t;
+    const double a3 = p / JUR_P0 * (q * ctwslf + (1 - q) * ctwfrn) * 1e-20;
+    return a1 * a2 * a3;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ double ctmn2(const CtmView &c, double nu, double p, double t) {
+  JUR_FP_EXACT
+  const double q_n2 = 0.79, t0 = 273.0, tr = 296.0;
+  if (nu < c.n2_nu[0] || nu > c.n2_nu[97]) return 0;
+  const int idx = locate_reg(c.n2_nu, 98, nu);
+  const double b = lin(c.n2_nu[idx], c.n2_b[idx], c.n2_nu[idx + 1], c.n2_b[idx + 1], nu);
+  const double beta = lin(c.n2_nu[idx], c.n2_beta[idx], c.n2_nu[idx + 1], c.n2_beta[idx + 1], nu);
+  const double r = p / JUR_P0 * t0 / t;
+  return 0.1 * (r * r) * exp(beta * (1 / tr - 1 / t)) * q_n2 * b *
+         (q_n2 + (1 - q_n2) * (1.294 - 0.4545 * t / tr));
+}
+
+__device__ __forceinline__ double ctmo2(const CtmView &c, double nu, double p, double t) {
+  JUR_FP_EXACT
+  const double q_o2 = 0.21, t0 = 273, tr = 296;
+  if (nu < c.o2_nu[0] || nu > c.o2_nu[89]) return 0;
+  const int idx = locate_reg(c.o2_nu, 90, nu);
+  const double b = lin(c.o2_nu[idx], c.o2_b[idx], c.o2_nu[idx + 1], c.o2_b[idx + 1], nu);
+  const double beta = lin(c.o2_nu[idx], c.o2_beta[idx], c.o2_nu[idx + 1], c.o2_beta[idx + 1], nu);
+  const double r = p / JUR_P0 * t0 / t;
+  return 0.1 * (r * r) * exp(beta * (1 / tr - 1 / t)) * q_o2 * b;
+}
+
+// formod_srcfunc (src/jurassic.c:3483-3496) for one channel column of the
+// source-function table (sr_col[TBLNS]).
+__device__ __forceinline__ double srcfunc(const double *st, const double *sr_col, double t) {
+  JUR_FP_EXACT
+  const int it = locate_reg(st, TBLNS, t);
+  return lin(st[it], sr_col[it], st[it + 1], sr_col[it + 1], t);
+}
+
+// PLANCK (src/jurassic.h:183-184).
+__device__ __forceinline__ double planck(double t, double nu) {
+  JUR_FP_EXACT
+  return JUR_C1 * (nu * nu * nu) / expm1(JUR_C2 * nu / t);
+}
+
+// BRIGHT (src/jurassic.h:126-127).
+__device__ __forceinline__ double bright(double rad, double nu) {
+  JUR_FP_EXACT
+  return JUR_C2 * nu / log1p(JUR_C1 * (nu * nu * nu) / rad);
+}
+
+// XCD-aware work mapping: consecutive hardware workgroup ids go to the 8
+// XCDs round-robin; remap so that each XCD gets a contiguous range of
+// logical indices (neighbouring channels of one ray block share an L2).
+__device__ __forceinline__ int xcd_remap(int wg, int nwg) {
+  constexpr int NX = 8;
+  const int per = nwg / NX, rem = nwg % NX;
+  const int x = wg % NX, k = wg / NX;
+  // XCD x owns logical range [x*per + min(x, rem), ...) of length per + (x < rem)
+  if (k < per) return x * per + (x < rem ? x : rem) + k;
+  // tail ids (k == per) exist only for x < rem
+  return x * per + x + k;
+}
+
+}  // namespace jur

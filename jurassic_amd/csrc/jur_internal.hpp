@@ -1,0 +1,128 @@
+// Internal data layout shared by the host code and the HIP kernels of
+// libjurassic_gpu.so.  Plain structs only; device pointers are raw.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/jurassic_gpu.h"
+
+namespace jur {
+
+// Physical constants (same values as src/jurassic.h:252-340).
+#define JUR_C1 1.19104259e-8
+#define JUR_C2 1.43877506
+#define JUR_KB 1.3806504e-23
+#define JUR_NA 6.02214199e23
+#define JUR_RE 6367.421
+#define JUR_P0 1013.25
+#define JUR_TMIN 100.
+#define JUR_TMAX 400.
+#define JUR_TSUN 5780.
+#define JUR_PI 3.14159265358979323846
+
+// Rays are processed in blocks of WAVE consecutive rays: one lane per ray.
+constexpr int WAVE = 64;
+
+// ------------------------------------------------------------------
+// Emissivity tables in HBM, laid out [gas][channel][p][T][u].
+// Each (p, T) column of nu knots is stored as nu records
+//   rec[i] = (u_i, eps_i, u_{i+1}, eps_{i+1})   (last: (u_n-1, e_n-1, same))
+// so that one aligned 16-byte load holds a whole interpolation bracket.
+// ------------------------------------------------------------------
+struct TabView {
+  int nd, ng;
+  int TP, TT, TU;          // padded capacities of p, T, u
+  const int *np;           // [ng][nd]           (-1: missing table)
+  const int *nt;           // [ng][nd][TP]
+  const int *nu;           // [ng][nd][TP][TT]
+  const double *p;         // [ng][nd][TP]
+  const double *t;         // [ng][nd][TP][TT]
+  const float4 *rec;       // [ng][nd][TP][TT][TU]
+  const int *mono;         // [ng][nd]: 1 if every column is non-decreasing
+                           // in u and eps (search by local walk), 0 otherwise
+                           // (search by the reference bisection)
+};
+
+// ------------------------------------------------------------------
+// Line-of-sight buffer written by the ray tracer, read by the radiance
+// kernels.  Per block of WAVE rays: field-major, point-major, lane fastest:
+//   los[((blk * nfield + f) * cap + ip) * WAVE + lane]
+// so that the 64 lanes of a wavefront read one field of one point with a
+// single coalesced 512-byte access.
+// ------------------------------------------------------------------
+enum {
+  F_Z = 0,
+  F_LON = 1,
+  F_LAT = 2,
+  F_P = 3,
+  F_T = 4,
+  F_DS = 5,   // raw step length before the trapezoid rule
+  F_Q0 = 6    // then q[ng], k[nw], cloud factor (if clouds)
+};
+
+struct LosView {
+  double *los;
+  int cap;      // points per ray
+  int nfield;
+  int f_k;      // first window-extinction field
+  int f_cl;     // cloud factor field, -1 if none
+};
+
+// Per-ray results of the ray tracer.
+struct RayInfo {
+  int np;          // LOS points
+  int status;      // JUR_OK, JUR_ERR_LOS, JUR_ERR_OBSERVER
+  double sft;      // surface temperature if the LOS hit the ground, else -999
+  double tpz, tplon, tplat;
+  double rcos;     // solar incidence factor (SFTYPE 3), 0 if sun below horizon
+  double pad;
+};
+
+// Per-channel constants.
+struct ChanInfo {
+  double nu;
+  int window;
+  int icl;         // cloud spectral index (locate_irr on clnu), 0 if none
+  int isf;         // surface spectral index (locate_irr on sfnu), 0 if none
+  int pad;
+  double planck_sun;   // PLANCK(TSUN, nu)
+};
+
+// Continuum coefficient tables on the device (see ctm_coeffs.h).
+struct CtmView {
+  const double *co2_296, *co2_260, *co2_230;
+  const double *h2o_296, *h2o_260, *h2o_frn, *h2o_xfcrev;
+  const double *n2_b, *n2_beta, *n2_nu;
+  const double *o2_b, *o2_beta, *o2_nu;
+};
+
+// Forward-model settings that the kernels read.
+struct ModelView {
+  int nd, ng, nw, ncl, nsf;
+  int sftype;
+  double sfsza;
+  int formod;          // 0 CGA, 1 EGA
+  int refrac;
+  double rayds, raydz;
+  int ig_co2, ig_h2o;  // emitter indices for the continua, -1 if off/absent
+  int ctm_n2, ctm_o2;
+  int write_bbt;
+  const ChanInfo *chan;   // [nd]
+  const double *clnu;     // [NCL]
+  const double *sfnu;     // [NSF]
+  const double *st;       // [TBLNS] source-function temperatures
+  const double *sr;       // [nd][TBLNS] source-function radiances
+  CtmView ctm;
+  TabView tab;
+};
+
+// One batch of rays (device pointers).
+struct BatchView {
+  int nprof;
+  atm_t *atm;
+  int nray;
+  const int *ray_prof;
+  const double *ray_geom;   // [nray][7]
+};
+
+}  // namespace jur

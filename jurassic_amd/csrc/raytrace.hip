@@ -1,0 +1,324 @@
+// Ray tracing (FP64) and hydrostatic equilibrium on the device.
+//
+// One lane per ray.  The lane marches its line of sight through its own
+// atmosphere exactly as raytrace (src/jurassic.c:4201-4438) does and writes
+// the LOS points into the ray block's field-major buffer (see LosView), then
+// finds the tangent point (tangent_point, src/jurassic.c:5223-5263) and the
+// solar-incidence factor of formod_pencil (src/jurassic.c:3297-3326).
+// Built with -ffp-contract=off: every operation rounds as in the reference.
+#include "jur_device.hpp"
+
+namespace jur {
+
+// intpol_atm (src/jurassic.c:3685-3706) without the per-gas outputs (the
+// LOS point writer reads q and k itself at the final position).
+__device__ __forceinline__ void intpol_pt(const atm_t *atm, double z, double *p, double *t) {
+  JUR_FP_EXACT
+  const int ip = locate_irr(atm->z, atm->np, z);
+  *p = logy(atm->z[ip], atm->p[ip], atm->z[ip + 1], atm->p[ip + 1], z);
+  *t = lin(atm->z[ip], atm->t[ip], atm->z[ip + 1], atm->t[ip + 1], z);
+}
+
+// sza (src/jurassic.c:5183-5219).
+__device__ __forceinline__ double sza(double sec, double lon, double lat) {
+  JUR_FP_EXACT
+  const double D = sec / 86400 - 0.5;
+  const double g = (357.529 + 0.98560028 * D) * (JUR_PI / 180.0);
+  const double q = 280.459 + 0.98564736 * D;
+  const double L = (q + 1.915 * sin(g) + 0.020 * sin(2 * g)) * (JUR_PI / 180.0);
+  const double e = (23.439 - 0.00000036 * D) * (JUR_PI / 180.0);
+  const double dec = asin(sin(e) * sin(L));
+  const double ra = atan2(cos(e) * sin(L), cos(L));
+  const double GMST = 18.697374558 + 24.06570982441908 * D;
+  const double LST = GMST + lon / 15;
+  const double h = LST / 12 * JUR_PI - ra;
+  const double latr = lat * (JUR_PI / 180.0);
+  return acos(sin(latr) * sin(dec) + cos(latr) * cos(dec) * cos(h)) * (180.0 / JUR_PI);
+}
+
+#define LOSW(f, ip) base[((size_t)(f) * L.cap + (size_t)(ip)) * WAVE]
+
+__global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, LosView L,
+                                                   RayInfo *info, int blk0) {
+  JUR_FP_EXACT
+  const int lane = threadIdx.x;
+  const int blk = blk0 + blockIdx.x;
+  const int ir = blk * WAVE + lane;
+  if (ir >= B.nray) return;
+  double *base = L.los + (size_t)blockIdx.x * L.nfield * L.cap * WAVE + lane;
+
+  const double *g = B.ray_geom + (size_t)7 * ir;
+  const int iprof = B.ray_prof[ir];
+  if (iprof < 0 || iprof >= B.nprof || B.atm[iprof].np < 2 || B.atm[iprof].np > NP) {
+    RayInfo E{};
+    E.status = JUR_ERR_ARG;
+    info[ir] = E;
+    return;
+  }
+  const atm_t *atm = &B.atm[iprof];
+  const double otime = g[0], obsz = g[1], obslon = g[2], obslat = g[3];
+  const double vpz = g[4], vplon = g[5], vplat = g[6];
+
+  const double h = 0.02, zrefrac = 60;
+  double ex0[3], ex1[3], lat, lon, n, ngr[3], norm, p, t, x[3], xh[3], xobs[3], xvp[3];
+  double z = 1e99, zmax, zmin;
+  int stop = 0, np = 0;
+
+  RayInfo R;
+  R.np = 0;
+  R.status = JUR_OK;
+  R.sft = -999;
+  R.tpz = vpz;
+  R.tplon = vplon;
+  R.tplat = vplat;
+  R.rcos = 0;
+  R.pad = 0;
+
+  // altitude range (gsl_stats_minmax semantics, src/jurassic.c:4223)
+  zmin = zmax = atm->z[0];
+  for (int i = 0; i < atm->np; i++) {
+    const double v = atm->z[i];
+    if (v < zmin) zmin = v;
+    if (v > zmax) zmax = v;
+    if (isnan(v)) { zmin = zmax = v; break; }
+  }
+  if (M.nsf > 0) {
+    zmin = (atm->sfz > zmin) ? atm->sfz : zmin;
+    if (atm->sfp > 0) {
+      const int ip = locate_irr(atm->p, atm->np, atm->sfp);
+      const double zip = lin(log(atm->p[ip]), atm->z[ip], log(atm->p[ip + 1]), atm->z[ip + 1],
+                             log(atm->sfp));
+      zmin = (zip > zmin) ? zip : zmin;
+    }
+  }
+
+  if (obsz < zmin) {
+    R.status = JUR_ERR_OBSERVER;
+    info[ir] = R;
+    return;
+  }
+  if (vpz > zmax) {
+    info[ir] = R;
+    return;
+  }
+
+  geo2cart(obsz, obslon, obslat, xobs);
+  geo2cart(vpz, vplon, vplat, xvp);
+  for (int i = 0; i < 3; i++) ex0[i] = xvp[i] - xobs[i];
+  norm = sqrt(ex0[0] * ex0[0] + ex0[1] * ex0[1] + ex0[2] * ex0[2]);
+  for (int i = 0; i < 3; i++) ex0[i] /= norm;
+  for (int i = 0; i < 3; i++) x[i] = xobs[i];
+
+  // entry point of an observer above the atmosphere (bisection to 1 m)
+  if (obsz > zmax) {
+    double dmax = norm, dmin = 0;
+    while (fabs(dmin - dmax) > 0.001) {
+      const double d = (dmax + dmin) / 2;
+      for (int i = 0; i < 3; i++) x[i] = xobs[i] + d * ex0[i];
+      cart2geo(x, &z, &lon, &lat);
+      if (z <= zmax && z > zmax - 0.001) break;
+      if (z < zmax - 0.0005) dmax = d; else dmin = d;
+    }
+  }
+
+  double zprev = 0, lonprev = 0, latprev = 0;
+  for (;;) {
+    double ds = M.rayds;
+    if (M.raydz > 0) {
+      norm = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+      for (int i = 0; i < 3; i++) xh[i] = x[i] / norm;
+      const double cosa = fabs(ex0[0] * xh[0] + ex0[1] * xh[1] + ex0[2] * xh[2]);
+      if (cosa != 0) ds = (M.rayds < M.raydz / cosa) ? M.rayds : M.raydz / cosa;
+    }
+
+    cart2geo(x, &z, &lon, &lat);
+
+    if (z < zmin || z > zmax) {
+      if (np == 0) {
+        // The entry search ended outside the altitude range, so there is no
+        // previous point to clip to (the reference would read los->z[-1]):
+        // the ray has no LOS points.
+        break;
+      }
+      stop = (z < zmin ? 2 : 1);
+      const double frac = ((z < zmin ? zmin : zmax) - zprev) / (z - zprev);
+      geo2cart(zprev, lonprev, latprev, xh);
+      for (int i = 0; i < 3; i++) x[i] = xh[i] + frac * (x[i] - xh[i]);
+      cart2geo(x, &z, &lon, &lat);
+      // los->ds[np-1] = ds * frac (the previous point was already written)
+      LOSW(F_DS, np - 1) = ds * frac;
+      ds = 0;
+    }
+
+    // intpol_atm at the LOS point
+    const int ipa = locate_irr(atm->z, atm->np, z);
+    const double z0 = atm->z[ipa], z1 = atm->z[ipa + 1];
+    p = logy(z0, atm->p[ipa], z1, atm->p[ipa + 1], z);
+    t = lin(z0, atm->t[ipa], z1, atm->t[ipa + 1], z);
+
+    if (np >= L.cap) {
+      // capacity exceeded: the reference limit NLOS -> error, a smaller
+      // working capacity -> the host retries with NLOS
+      R.status = (L.cap >= NLOS) ? JUR_ERR_LOS : 100;
+      info[ir] = R;
+      return;
+    }
+    LOSW(F_Z, np) = z;
+    LOSW(F_LON, np) = lon;
+    LOSW(F_LAT, np) = lat;
+    LOSW(F_P, np) = p;
+    LOSW(F_T, np) = t;
+    for (int ig = 0; ig < M.ng; ig++)
+      LOSW(F_Q0 + ig, np) = lin(z0, atm->q[ig][ipa], z1, atm->q[ig][ipa + 1], z);
+    for (int iw = 0; iw < M.nw; iw++)
+      LOSW(L.f_k + iw, np) = lin(z0, atm->k[iw][ipa], z1, atm->k[iw][ipa + 1], z);
+    LOSW(F_DS, np) = ds;
+    if (L.f_cl >= 0) {
+      const double aux = (M.ncl > 0 && atm->cldz > 0)
+                             ? exp(-0.5 * (((z - atm->clz) / atm->cldz) * ((z - atm->clz) / atm->cldz)))
+                             : 0.0;
+      LOSW(L.f_cl, np) = aux;
+    }
+    zprev = z;
+    lonprev = lon;
+    latprev = lat;
+    np++;
+
+    if (stop) {
+      if (M.nsf > 0 && atm->sft > 0) t = atm->sft;
+      R.sft = (stop == 2 ? t : -999);
+      break;
+    }
+
+    if (M.refrac && z <= zrefrac) n = 1 + 7.753e-05 * p / t;
+    else n = 1;
+
+    for (int i = 0; i < 3; i++) ex1[i] = ex0[i] * n;
+
+    if (M.refrac && z <= zrefrac) {
+      double zz, lo2, la2, pp, tt;
+      for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
+      cart2geo(xh, &zz, &lo2, &la2);
+      intpol_pt(atm, zz, &pp, &tt);
+      n = 7.753e-05 * pp / tt;
+      for (int i = 0; i < 3; i++) {
+        xh[i] += h;
+        cart2geo(xh, &zz, &lo2, &la2);
+        intpol_pt(atm, zz, &pp, &tt);
+        ngr[i] = (7.753e-05 * pp / tt - n) / h;
+        xh[i] -= h;
+      }
+    } else
+      for (int i = 0; i < 3; i++) ngr[i] = 0;
+
+    for (int i = 0; i < 3; i++) ex1[i] += ds * ngr[i];
+    norm = sqrt(ex1[0] * ex1[0] + ex1[1] * ex1[1] + ex1[2] * ex1[2]);
+    for (int i = 0; i < 3; i++) ex1[i] /= norm;
+    for (int i = 0; i < 3; i++) x[i] += 0.5 * ds * (ex0[i] + ex1[i]);
+    for (int i = 0; i < 3; i++) ex0[i] = ex1[i];
+  }
+  R.np = np;
+  if (np == 0) {
+    info[ir] = R;
+    return;
+  }
+
+  // tangent_point: first minimum altitude (gsl_stats_min_index semantics)
+  int imin = 0;
+  {
+    double m = LOSW(F_Z, 0);
+    for (int i = 0; i < np; i++) {
+      const double v = LOSW(F_Z, i);
+      if (v < m) { m = v; imin = i; }
+      if (isnan(v)) { imin = i; break; }
+    }
+  }
+  if (imin <= 0 || imin >= np - 1) {
+    R.tpz = LOSW(F_Z, np - 1);
+    R.tplon = LOSW(F_LON, np - 1);
+    R.tplat = LOSW(F_LAT, np - 1);
+  } else {
+    const double yy0 = LOSW(F_Z, imin - 1);
+    const double yy1 = LOSW(F_Z, imin);
+    const double yy2 = LOSW(F_Z, imin + 1);
+    const double d1 = LOSW(F_DS, imin), d2 = LOSW(F_DS, imin + 1);
+    const double x1 = sqrt(d1 * d1 - (yy1 - yy0) * (yy1 - yy0));
+    const double x2 = x1 + sqrt(d2 * d2 - (yy2 - yy1) * (yy2 - yy1));
+    const double a = 1 / (x1 - x2) * (-(yy0 - yy1) / x1 + (yy0 - yy2) / x2);
+    const double b = -(yy0 - yy1) / x1 - a * x1;
+    const double c = yy0;
+    const double xt = -b / (2 * a);
+    double v[3], v0[3], v2[3], dummy;
+    R.tpz = a * xt * xt + b * xt + c;
+    geo2cart(yy0, LOSW(F_LON, imin - 1), LOSW(F_LAT, imin - 1), v0);
+    geo2cart(yy2, LOSW(F_LON, imin + 1), LOSW(F_LAT, imin + 1), v2);
+    for (int i = 0; i < 3; i++) v[i] = lin(0.0, v0[i], x2, v2[i], xt);
+    cart2geo(v, &dummy, &R.tplon, &R.tplat);
+  }
+
+  // solar incidence (formod_pencil, src/jurassic.c:3297-3326)
+  if (M.sftype >= 3 && R.sft > 0) {
+    const double lonl = LOSW(F_LON, np - 1), latl = LOSW(F_LAT, np - 1);
+    const double sza2 = (M.sfsza < 0) ? sza(otime, lonl, latl) : M.sfsza;
+    if (sza2 < 89.999) {
+      double x0[3], x1[3];
+      geo2cart(LOSW(F_Z, np - 1), lonl, latl, x0);
+      geo2cart(LOSW(F_Z, 0), LOSW(F_LON, 0), LOSW(F_LAT, 0), x1);
+      for (int i = 0; i < 3; i++) x1[i] -= x0[i];
+      const double cosa = (x0[0] * x1[0] + x0[1] * x1[1] + x0[2] * x1[2]) /
+                          sqrt(x0[0] * x0[0] + x0[1] * x0[1] + x0[2] * x0[2]) /
+                          sqrt(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
+      R.rcos = cosa / cos(sza2 * (JUR_PI / 180.0));
+      R.pad = 1.0;
+    }
+  }
+  info[ir] = R;
+}
+
+// hydrostatic (src/jurassic.c:3520-3582), one lane per atmosphere.
+__global__ __launch_bounds__(WAVE) void k_hydrostatic(atm_t *atm, int nprof, double hydz,
+                                                      int ig_h2o) {
+  JUR_FP_EXACT
+  const int ia = blockIdx.x * WAVE + threadIdx.x;
+  if (ia >= nprof) return;
+  atm_t *a = &atm[ia];
+  const double mmair = 28.96456e-3, mmh2o = 18.0153e-3, G0 = 9.80665, RI = 8.3144598;
+  const int ipts = 20;
+  double dzmin = 1e99, e = 0;
+  int ipref = 0;
+  for (int ip = 0; ip < a->np; ip++)
+    if (fabs(a->z[ip] - hydz) < dzmin) {
+      dzmin = fabs(a->z[ip] - hydz);
+      ipref = ip;
+    }
+  for (int ip = ipref + 1; ip < a->np; ip++) {
+    double mean = 0;
+    for (int i = 0; i < ipts; i++) {
+      if (ig_h2o >= 0) e = lin(0.0, a->q[ig_h2o][ip - 1], ipts - 1.0, a->q[ig_h2o][ip], (double)i);
+      mean += (e * mmh2o + (1 - e) * mmair) * G0 / RI /
+              lin(0.0, a->t[ip - 1], ipts - 1.0, a->t[ip], (double)i) / ipts;
+    }
+    a->p[ip] = exp(log(a->p[ip - 1]) - mean * 1000 * (a->z[ip] - a->z[ip - 1]));
+  }
+  for (int ip = ipref - 1; ip >= 0; ip--) {
+    double mean = 0;
+    for (int i = 0; i < ipts; i++) {
+      if (ig_h2o >= 0) e = lin(0.0, a->q[ig_h2o][ip + 1], ipts - 1.0, a->q[ig_h2o][ip], (double)i);
+      mean += (e * mmh2o + (1 - e) * mmair) * G0 / RI /
+              lin(0.0, a->t[ip + 1], ipts - 1.0, a->t[ip], (double)i) / ipts;
+    }
+    a->p[ip] = exp(log(a->p[ip + 1]) - mean * 1000 * (a->z[ip] - a->z[ip + 1]));
+  }
+}
+
+void launch_raytrace(const ModelView &M, const BatchView &B, const LosView &L, RayInfo *info,
+                     int blk0, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_raytrace, dim3(nblk), dim3(WAVE), 0, s, M, B, L, info, blk0);
+}
+
+void launch_hydrostatic(atm_t *atm, int nprof, double hydz, int ig_h2o, hipStream_t s) {
+  hipLaunchKernelGGL(k_hydrostatic, dim3((nprof + WAVE - 1) / WAVE), dim3(WAVE), 0, s, atm, nprof,
+                     hydz, ig_h2o);
+}
+
+}  // namespace jur

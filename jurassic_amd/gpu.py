@@ -1,0 +1,148 @@
+"""Python bindings of libjurassic_gpu.so (the MI355X forward model).
+
+Mirrors the reference operator interface for this path: formod(ctl, atm,
+obs) (src/jurassic.h:890-893) and kernel(ctl, atm, obs, k)
+(src/jurassic.h:1008-1012), on the same ctl_t/atm_t/obs_t structs, plus the
+device-resident batch interface used by the benchmark.  Errors raise
+JurassicError carrying the library's status code and message (the
+reference prints an error and exits instead).
+
+The library must be built (jurassic_amd/build.py); there is no CPU
+fallback: if libjurassic_gpu.so is missing this module raises on import of
+the library.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libjurassic_gpu.so")
+_LIB = None
+
+
+class JurassicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (abi.ERR_NAMES.get(code, code), msg))
+        self.code = code
+
+
+class jur_batch_t(C.Structure):
+    _fields_ = [
+        ("nprof", C.c_int), ("atm", C.c_void_p), ("nray", C.c_int),
+        ("ray_prof", C.c_void_p), ("ray_geom", C.c_void_p), ("mask", C.c_void_p),
+        ("rad", C.c_void_p), ("tau", C.c_void_p), ("tp", C.c_void_p),
+    ]
+
+
+EXPORTS = [
+    "jur_gpu_last_error", "jur_gpu_abi_check", "formod_gpu", "jur_gpu_formod",
+    "jur_gpu_formod_batch", "jur_gpu_kernel", "kernel_gpu", "jur_ctx_create",
+    "jur_ctx_create_wide", "jur_ctx_destroy", "jur_ctx_nd", "jur_ctx_formod_device",
+    "jur_ctx_last_timing", "jur_ctx_last_mean_np",
+]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise OSError("libjurassic_gpu.so is not built (run jurassic_amd/build.py)")
+        L = C.CDLL(LIBPATH)
+        P, I, S, D = C.c_void_p, C.c_int, C.c_size_t, C.c_double
+        L.jur_gpu_last_error.restype = C.c_char_p
+        L.jur_gpu_abi_check.argtypes = [S, S, S]
+        L.jur_gpu_abi_check.restype = I
+        L.jur_gpu_formod.argtypes = [P, P, P, I]
+        L.jur_gpu_formod.restype = I
+        L.jur_gpu_formod_batch.argtypes = [P, P, P, I, I]
+        L.jur_gpu_formod_batch.restype = I
+        L.jur_gpu_kernel.argtypes = [P, P, P, C.POINTER(D), S, S, I, I]
+        L.jur_gpu_kernel.restype = I
+        L.jur_ctx_create.argtypes = [P, C.POINTER(P)]
+        L.jur_ctx_create.restype = I
+        L.jur_ctx_create_wide.argtypes = [P, I, C.POINTER(D), C.POINTER(I), C.POINTER(P)]
+        L.jur_ctx_create_wide.restype = I
+        L.jur_ctx_destroy.argtypes = [P]
+        L.jur_ctx_nd.argtypes = [P]
+        L.jur_ctx_nd.restype = I
+        L.jur_ctx_formod_device.argtypes = [P, C.POINTER(jur_batch_t), I, P]
+        L.jur_ctx_formod_device.restype = I
+        L.jur_ctx_last_timing.argtypes = [P, C.POINTER(D), I]
+        L.jur_ctx_last_timing.restype = I
+        L.jur_ctx_last_mean_np.argtypes = [P]
+        L.jur_ctx_last_mean_np.restype = D
+        if L.jur_gpu_abi_check(C.sizeof(abi.ctl_t), C.sizeof(abi.atm_t), C.sizeof(abi.obs_t)):
+            raise OSError("ABI mismatch between jurassic_amd.abi and libjurassic_gpu.so")
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc:
+        raise JurassicError(rc, lib().jur_gpu_last_error().decode())
+
+
+def formod(ctl, atm, obs, precision=abi.JUR_FP64):
+    """formod() on the GPU: fills obs.rad/tau/tp* in place."""
+    _check(lib().jur_gpu_formod(C.byref(ctl), C.byref(atm), C.byref(obs), precision))
+
+
+def formod_batch(ctl, atms, obss, precision=abi.JUR_FP64):
+    """n packages (ctypes arrays of atm_t and obs_t) in one device pass."""
+    n = len(atms)
+    _check(lib().jur_gpu_formod_batch(C.byref(ctl), C.cast(atms, C.c_void_p),
+                                      C.cast(obss, C.c_void_p), n, precision))
+
+
+def kernel(ctl, atm, obs, method=abi.JUR_JAC_FD, precision=abi.JUR_FP64, m=None, n=None):
+    """kernel() on the GPU: returns the m x n Jacobian."""
+    if m is None or n is None:
+        raise ValueError("give m and n (sizes of obs2y/atm2x)")
+    k = np.zeros((m, n))
+    _check(lib().jur_gpu_kernel(C.byref(ctl), C.byref(atm), C.byref(obs),
+                                k.ctypes.data_as(C.POINTER(C.c_double)), m, n, method, precision))
+    return k
+
+
+class Context:
+    """A device context (tables and source function resident in HBM)."""
+
+    def __init__(self, ctl, nu=None, window=None):
+        self.ptr = C.c_void_p()
+        if nu is None:
+            _check(lib().jur_ctx_create(C.byref(ctl), C.byref(self.ptr)))
+        else:
+            nu = np.ascontiguousarray(nu, dtype=np.float64)
+            w = None if window is None else np.ascontiguousarray(window, dtype=np.int32)
+            _check(lib().jur_ctx_create_wide(
+                C.byref(ctl), len(nu), nu.ctypes.data_as(C.POINTER(C.c_double)),
+                None if w is None else w.ctypes.data_as(C.POINTER(C.c_int)), C.byref(self.ptr)))
+        self.nd = lib().jur_ctx_nd(self.ptr)
+
+    def formod_device(self, nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, rad_ptr, tau_ptr,
+                      tp_ptr=None, mask_ptr=None, precision=abi.JUR_FP64, stream=None):
+        b = jur_batch_t(nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, mask_ptr, rad_ptr,
+                        tau_ptr, tp_ptr)
+        _check(lib().jur_ctx_formod_device(self.ptr, C.byref(b), precision, stream))
+
+    def last_timing(self):
+        ms = (C.c_double * 2)()
+        _check(lib().jur_ctx_last_timing(self.ptr, ms, 2))
+        return ms[0], ms[1]
+
+    def last_mean_np(self):
+        return lib().jur_ctx_last_mean_np(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            lib().jur_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,312 @@
+"""GPU parity of the forward model against the CPU FP64 oracle.
+
+FP64 path: radiance and transmittance within 1e-10 relative of the oracle
+(the oracle restates the reference formod; the GPU kernels keep its
+operation order, so only last-bit differences of the math library remain).
+FP32 path: brightness temperature within 0.01 K of the FP64 oracle.
+All calls go through the C-ABI of libjurassic_gpu.so.
+"""
+import ctypes as C
+import math
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from jurassic_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-10
+BT_TOL32 = 0.01  # K
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from jurassic_amd import gpu as G
+    G.lib()
+    return G
+
+
+def _cmp(a, b, rtol, what, atol=0.0):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    assert a.shape == b.shape
+    both_nan = np.isnan(a) & np.isnan(b)
+    err = np.abs(a - b)
+    lim = rtol * np.abs(b) + atol
+    bad = ~both_nan & ~(err <= lim)
+    if bad.any():
+        i = np.argmax(np.where(bad, err / np.maximum(np.abs(b), 1e-300), 0))
+        raise AssertionError("%s: %d mismatches, worst at %s: %r vs %r" %
+                             (what, bad.sum(), np.unravel_index(i, a.shape), a.flat[i], b.flat[i]))
+
+
+def _run_both(gpu, oracle, ctl, atm, obs, precision=abi.JUR_FP64):
+    o_cpu = abi.copy_struct(obs)
+    a_cpu = abi.copy_struct(atm)
+    oracle.formod(ctl, a_cpu, o_cpu)
+    o_gpu = abi.copy_struct(obs)
+    a_gpu = abi.copy_struct(atm)
+    gpu.formod(ctl, a_gpu, o_gpu, precision)
+    return (a_cpu, o_cpu), (a_gpu, o_gpu)
+
+
+def _assert_obs(ctl, o_gpu, o_cpu, rtol=RTOL64):
+    r1, t1 = abi.obs_arrays(o_gpu, ctl)
+    r0, t0 = abi.obs_arrays(o_cpu, ctl)
+    _cmp(r1, r0, rtol, "rad")
+    _cmp(t1, t0, rtol, "tau", atol=1e-300)
+    nr = o_cpu.nr
+    for f in ("tpz", "tplon", "tplat"):
+        _cmp(np.ctypeslib.as_array(getattr(o_gpu, f))[:nr],
+             np.ctypeslib.as_array(getattr(o_cpu, f))[:nr], 1e-11, f, atol=1e-9)
+
+
+def test_nadir_co2_fp64(gpu, oracle):
+    ctl, atm, obs = scenes.nadir_co2()
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+    rad, tau = abi.obs_arrays(og, ctl)
+    assert np.all(rad > 0) and np.all((tau > 0) & (tau < 1))
+
+
+def test_nadir_co2_bbt_fp64(gpu, oracle):
+    ctl, atm, obs = scenes.nadir_co2()
+    ctl.write_bbt = 1
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_limb_3gas_fp64(gpu, oracle):
+    for iprof in (0, 7):
+        ctl, atm, obs = scenes.limb_3gas(iprof)
+        (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+        _assert_obs(ctl, og, oc)
+
+
+def test_limb_3gas_no_continua_fp64(gpu, oracle):
+    ctl, atm, obs = scenes.limb_3gas(3, ctm=False)
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_cga_fp64(gpu, oracle):
+    ctl, atm, obs = scenes.limb_3gas(1, nrays=12)
+    ctl.formod = 0
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+    ctl, atm, obs = scenes.nadir_co2()
+    ctl.formod = 0
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_batch_matches_oracle_and_single(gpu, oracle):
+    """Packages of different atmospheres in one pass equal separate calls."""
+    n = 5
+    ctl = scenes.limb_3gas_ctl()
+    atms = (abi.atm_t * n)()
+    obss = (abi.obs_t * n)()
+    for k in range(n):
+        atms[k] = scenes.make_profile_atm(["CO2", "H2O", "O3"], iprof=100 + k)
+        obss[k] = scenes.make_obs(scenes.limb_geometry(800.0, scenes.limb_tangent_heights()[k::3]))
+    gpu.formod_batch(ctl, atms, obss)
+    for k in range(n):
+        o1 = scenes.make_obs(scenes.limb_geometry(800.0, scenes.limb_tangent_heights()[k::3]))
+        a1 = abi.copy_struct(atms[k])
+        oracle.formod(ctl, a1, o1)
+        _assert_obs(ctl, obss[k], o1)
+        o2 = scenes.make_obs(scenes.limb_geometry(800.0, scenes.limb_tangent_heights()[k::3]))
+        gpu.formod(ctl, abi.copy_struct(atms[k]), o2)
+        r1, _ = abi.obs_arrays(obss[k], ctl)
+        r2, _ = abi.obs_arrays(o2, ctl)
+        np.testing.assert_array_equal(r1, r2)
+
+
+def _bt(rad, nu):
+    c1, c2 = 1.19104259e-8, 1.43877506
+    return c2 * nu / np.log1p(c1 * nu ** 3 / rad)
+
+
+def test_fp32_bt_error(gpu, oracle):
+    """FP32 lookups with compensated sums stay within 0.01 K of FP64."""
+    worst = 0.0
+    for make in (lambda: scenes.limb_3gas(0), lambda: scenes.limb_3gas(11), scenes.nadir_co2):
+        ctl, atm, obs = make()
+        ctl.write_bbt = 1
+        (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs, abi.JUR_FP32)
+        r1, t1 = abi.obs_arrays(og, ctl)
+        r0, t0 = abi.obs_arrays(oc, ctl)
+        err = np.abs(r1 - r0)
+        worst = max(worst, float(err.max()))
+        assert err.max() <= BT_TOL32, err.max()
+        assert np.abs(t1 - t0).max() <= 1e-4
+    print("worst FP32 BT error %.2e K" % worst)
+
+
+def test_fp32_cga(gpu, oracle):
+    ctl, atm, obs = scenes.limb_3gas(2, nrays=16)
+    ctl.formod = 0
+    ctl.write_bbt = 1
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs, abi.JUR_FP32)
+    r1, _ = abi.obs_arrays(og, ctl)
+    r0, _ = abi.obs_arrays(oc, ctl)
+    assert np.abs(r1 - r0).max() <= BT_TOL32
+
+
+def _golden_ctl(tmp_path, name):
+    """The reference's limb/nadir test geometry and atmosphere with boxcar
+    filters and no emissivity tables (missing tables give eps = 0, as in the
+    reference), for the tangent-point columns of rad.org."""
+    if name == "limb_test":
+        em, nu = ["CO2", "H2O", "O3", "F11", "CCl4"], [792.0, 832.0]
+    else:
+        em, nu = ["CO2"], [667.782, 668.541, 669.811]
+    tb = os.path.join(str(tmp_path), "g")
+    for v in nu:
+        scenes.write_boxcar_filter("%s_%.4f.filt" % (tb, v), v)
+    ctl = abi.make_ctl(em, nu, tblbase=tb)
+    atm = abi.read_atm(os.path.join(GOLD, name, "atm.tab"), ctl)
+    obs = abi.read_obs(os.path.join(GOLD, name, "obs.tab"), ctl)
+    ref = abi._data_rows(os.path.join(GOLD, name, "rad.org"))
+    return ctl, atm, obs, ref
+
+
+@pytest.mark.parametrize("name", ["limb_test", "nadir_test"])
+def test_golden_tangent_points(gpu, tmp_path, name):
+    ctl, atm, obs, ref = _golden_ctl(tmp_path, name)
+    gpu.formod(ctl, atm, obs)
+    for ir in range(obs.nr):
+        got = ["%g" % obs.tpz[ir], "%g" % obs.tplon[ir], "%g" % obs.tplat[ir]]
+        exp = ["%g" % ref[ir][7], "%g" % ref[ir][8], "%g" % ref[ir][9]]
+        assert got == exp, (name, ir)
+    rad, tau = abi.obs_arrays(obs, ctl)
+    # without tables only the continua absorb
+    assert np.all(tau > 0) and np.all(tau <= 1)
+
+
+def test_surface_reflection_and_solar(gpu, oracle):
+    ctl, atm, obs = scenes.nadir_co2()
+    ctl.nsf = 2
+    ctl.sfnu[0], ctl.sfnu[1] = 600.0, 800.0
+    ctl.sftype = 3
+    ctl.sfsza = 30.0
+    atm.sfz = 0.0
+    atm.sfp = 0.0
+    atm.sft = 295.0
+    atm.sfeps[0], atm.sfeps[1] = 0.8, 0.95
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+    ctl.sfsza = -999  # solar zenith angle from time and position
+    for ir in range(obs.nr):
+        obs.time[ir] = 1.5e8 + 3600.0 * ir
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_clouds(gpu, oracle):
+    ctl, atm, obs = scenes.limb_3gas(4, nrays=10)
+    ctl.ncl = 2
+    ctl.clnu[0], ctl.clnu[1] = 650.0, 1500.0
+    atm.clz = 12.0
+    atm.cldz = 1.5
+    atm.clk[0], atm.clk[1] = 0.02, 0.05
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_extinction_window(gpu, oracle):
+    ctl, atm, obs = scenes.limb_3gas(5, nrays=10)
+    ctl.nw = 2
+    for id_ in range(ctl.nd):
+        ctl.window[id_] = id_ % 2
+    for i in range(atm.np):
+        atm.k[0][i] = 1e-3 * math.exp(-atm.z[i] / 10)
+        atm.k[1][i] = 2e-3 * math.exp(-atm.z[i] / 5)
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_hydrostatic(gpu, oracle):
+    ctl, atm, obs = scenes.limb_3gas(6, nrays=8)
+    ctl.hydz = 10.0
+    (ac, oc), (ag, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+    _cmp(np.ctypeslib.as_array(ag.p)[:atm.np], np.ctypeslib.as_array(ac.p)[:atm.np], 1e-13, "p")
+
+
+def test_fov(gpu, oracle, tmp_path):
+    ctl, atm, obs = scenes.limb_3gas(8)
+    fov = os.path.join(str(tmp_path), "fov.tab")
+    with open(fov, "w") as f:
+        for dz in np.linspace(-1.5, 1.5, 31):
+            f.write("%g %g\n" % (dz, math.exp(-dz * dz)))
+    ctl.fov = fov.encode()
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_mask_nan(gpu, oracle):
+    ctl, atm, obs = scenes.nadir_co2()
+    obs.rad[1][3] = float("nan")
+    obs.rad[0][7] = float("nan")
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+    assert math.isnan(og.rad[1][3]) and math.isnan(og.rad[0][7])
+    assert not math.isnan(og.rad[0][3])
+
+
+def test_ascii_tables(gpu, oracle, tmp_path):
+    """TBLFMT = 1: the ASCII reader (with the monotonicity merge of saturated
+    rows) on both sides."""
+    nus = [667.8]
+    d = str(tmp_path)
+    tb = scenes.write_scene_tables(d, ["CO2"], nus, fmt=1)
+    ctl = abi.make_ctl(["CO2"], nus, tblbase=tb, tblfmt=1)
+    atm = scenes.make_profile_atm(["CO2"])
+    obs = scenes.make_obs(scenes.nadir_geometry(705.0, [0.0, 30.0]))
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_missing_table_is_transparent(gpu, oracle, tmp_path):
+    ctl, atm, obs = scenes.nadir_co2()
+    d = str(tmp_path)
+    for v in (667.8, 668.5):
+        shutil.copy("%s_%.4f.filt" % (ctl.tblbase.decode(), v), os.path.join(d, "m_%.4f.filt" % v))
+    shutil.copy("%s_%.4f_CO2.bin" % (ctl.tblbase.decode(), 667.8), os.path.join(d, "m_667.8000_CO2.bin"))
+    ctl.tblbase = os.path.join(d, "m").encode()
+    (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, og, oc)
+
+
+def test_error_observer_below_surface(gpu):
+    ctl, atm, obs = scenes.nadir_co2()
+    obs.obsz[2] = -1.0
+    with pytest.raises(gpu.JurassicError) as e:
+        gpu.formod(ctl, atm, obs)
+    assert e.value.code == 4
+
+
+def test_error_too_many_los_points(gpu):
+    ctl, atm, obs = scenes.nadir_co2()
+    ctl.raydz = 0.01  # 8000 points on an 80 km nadir path > NLOS
+    with pytest.raises(gpu.JurassicError) as e:
+        gpu.formod(ctl, atm, obs)
+    assert e.value.code == 3
+
+
+def test_kernel_fd_matches_oracle(gpu, oracle):
+    ctl, atm, obs = scenes.nadir_co2()
+    ctl.rett_zmin, ctl.rett_zmax = -100, 100
+    ctl.retq_zmin[0], ctl.retq_zmax[0] = -100, 100
+    m, n = oracle.sizes(ctl, atm, obs)
+    k0 = oracle.kernel(ctl, abi.copy_struct(atm), abi.copy_struct(obs))
+    k1 = gpu.kernel(ctl, abi.copy_struct(atm), abi.copy_struct(obs), m=m, n=n)
+    scale = np.abs(k0).max(axis=0, keepdims=True) + 1e-300
+    assert np.abs(k1 - k0).max() / np.abs(k0).max() < 1e-7
+    assert (np.abs(k1 - k0) / scale).max() < 1e-6
