@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X forward model (one GPU).
+
+Workload (limb sounder): 4096 atmospheres (81 levels, standard temperature
+plus a seeded Gaussian random field of 3 K and 5 km correlation length; CO2,
+H2O and O3), 32 tangent heights 5..67 km seen from 800 km, 12 channels
+700..1400 cm^-1, EGA with all continua and refraction: 131,072 rays and
+1,572,864 radiances per step.  One step is one forward pass over all rays
+with atmospheres and geometry already resident in HBM.
+
+The JSON line reports whole-job radiances/s of the FP32 path (FP64 ray
+geometry), the FP64 path's rate beside it, the radiance kernel's
+algorithmic bytes against the HBM roofline (HIP events on the library's
+stream), and the CPU FP64 oracle timed on this host's cores on a bounded
+sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "radiances/s fwd; Jacobians/s adjoint vs FD; p50/p99 retrieval latency"
+HBM_PEAK = 8.0e12
+NPROF, NRAY_PER, GASES = 4096, 32, ["CO2", "H2O", "O3"]
+
+
+def build_workload(nprof):
+    import numpy as np
+    from jurassic_amd import abi, scenes
+    ctl = scenes.limb_3gas_ctl()
+    zt = scenes.limb_tangent_heights()
+    geom = scenes.limb_geometry(800.0, zt)
+    atms = (abi.atm_t * nprof)()
+    for k in range(nprof):
+        atms[k] = scenes.make_profile_atm(GASES, iprof=k)
+    # ray order: tangent height major, profile minor -> each block of 64 rays
+    # holds one tangent height over 64 atmospheres (similar paths)
+    nray = nprof * len(zt)
+    prof = np.empty(nray, dtype=np.int32)
+    g = np.empty((nray, 7))
+    for j in range(len(zt)):
+        sl = slice(j * nprof, (j + 1) * nprof)
+        prof[sl] = np.arange(nprof)
+        g[sl, 0] = 0.0
+        g[sl, 1] = geom["obsz"][j]
+        g[sl, 2] = geom["obslon"][j]
+        g[sl, 3] = geom["obslat"][j]
+        g[sl, 4] = geom["vpz"][j]
+        g[sl, 5] = geom["vplon"][j]
+        g[sl, 6] = geom["vplat"][j]
+    return ctl, atms, prof, g
+
+
+def cpu_baseline(nsample):
+    """FP64 oracle on this host's cores: OpenMP across atmospheres after one
+    warm call (the kernel() pattern of the reference), on nsample profiles of
+    the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from jurassic_amd import abi, scenes
+    ctl = scenes.limb_3gas_ctl()
+    zt = scenes.limb_tangent_heights()
+    atms = (abi.atm_t * nsample)()
+    obss = (abi.obs_t * nsample)()
+    for k in range(nsample):
+        atms[k] = scenes.make_profile_atm(GASES, iprof=k)
+        obss[k] = scenes.make_obs(scenes.limb_geometry(800.0, zt))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # warm: tables and source function
+    O.formod(ctl, abi.copy_struct(atms[0]), abi.copy_struct(obss[0]))
+    t0 = time.perf_counter()
+    O.formod_batch(ctl, atms, obss, nthreads=threads)
+    dt = time.perf_counter() - t0
+    nrad = nsample * len(zt) * ctl.nd
+    return dict(value=nrad / dt, unit="radiances/s", cores=threads, kind="port",
+                sample="%d profiles x %d tangent heights x %d channels (%d radiances), FP64 oracle, %.1f s"
+                % (nsample, len(zt), ctl.nd, nrad, dt))
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nprof", type=int, default=NPROF)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="profiles for the CPU baseline (default: 4 per thread)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(profiles shard across GPUs without any exchange).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from jurassic_amd import abi, gpu
+
+    torch.cuda.set_device(0)
+    ctl, atms, prof, geom = build_workload(args.nprof)
+    nprof, nray, nd = args.nprof, len(prof), ctl.nd
+    log("workload built: %d atmospheres, %d rays, %d channels" % (nprof, nray, nd))
+    ctx = gpu.Context(ctl)
+    log("device context ready (tables and source function in HBM)")
+
+    dev = torch.device("cuda:0")
+    atm_d = torch.frombuffer(bytearray(bytes(atms)), dtype=torch.uint8).to(dev)
+    prof_d = torch.from_numpy(prof).to(dev)
+    geom_d = torch.from_numpy(np.ascontiguousarray(geom)).to(dev)
+    rad_d = torch.empty(nray * nd, dtype=torch.float64, device=dev)
+    tau_d = torch.empty(nray * nd, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step(precision):
+        ctx.formod_device(nprof, atm_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
+                          rad_d.data_ptr(), tau_d.data_ptr(), precision=precision)
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            step(precision)
+        torch.cuda.synchronize()
+        kms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(precision)
+            kms += ctx.last_timing()[1]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return dt, kms / steps
+
+    dt32, k32 = timed(abi.JUR_FP32, args.steps, args.warmup)
+    mean_np = ctx.last_mean_np()
+    log("FP32: %.1f ms/step, radiance kernel %.1f ms, mean LOS points %.1f"
+        % (dt32 * 1e3 / args.steps, k32, mean_np))
+    res = {}
+    if not args.no_fp64:
+        dt64, k64 = timed(abi.JUR_FP64, max(1, args.steps // 2), 1)
+        res["fp64_value"] = nray * nd * max(1, args.steps // 2) / dt64
+        res["fp64_kernel_ms"] = k64
+        log("FP64: %.1f ms/step, radiance kernel %.1f ms" % (dt64 * 1e3 / max(1, args.steps // 2), k64))
+    nrad = nray * nd
+    value = nrad * args.steps / dt32
+    bytes_per_rad = mean_np * (len(GASES) * 176 + 16)
+    achieved = bytes_per_rad * nrad / (k32 * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "radiances/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt32 * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "limb sounder: %d atmospheres x 32 tangent heights x 12 channels "
+                        "(CO2, H2O, O3; EGA; continua and refraction on), %d radiances per step"
+                        % (nprof, nrad),
+            "profiles": nprof, "rays": nray, "channels": nd, "emitters": len(GASES),
+            "mean_los_points": mean_np, "precision": "FP32 lookups, FP64 ray geometry",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "kernel": "k_radiance_f32<3,false>",
+            "kernel_ms": k32,
+            "bytes_per_radiance": bytes_per_rad,
+        },
+    }
+    out.update(res)
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ns = args.cpu_sample or max(8, 4 * threads)
+        log("CPU baseline on %d profiles, %d threads" % (ns, threads))
+        out["cpu_baseline"] = cpu_baseline(ns)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -55,11 +55,17 @@ def _run_both(gpu, oracle, ctl, atm, obs, precision=abi.JUR_FP64):
     return (a_cpu, o_cpu), (a_gpu, o_gpu)
 
 
-def _assert_obs(ctl, o_gpu, o_cpu, rtol=RTOL64):
+def _assert_obs(ctl, o_gpu, o_cpu, rtol=RTOL64, row_scaled=False):
     r1, t1 = abi.obs_arrays(o_gpu, ctl)
     r0, t0 = abi.obs_arrays(o_cpu, ctl)
-    _cmp(r1, r0, rtol, "rad")
-    _cmp(t1, t0, rtol, "tau", atol=1e-300)
+    if row_scaled:
+        # combinations with negative weights (extrapolating interpolation):
+        # the bound applies relative to the largest input of the channel
+        _cmp(r1, r0, 0, "rad", atol=rtol * np.abs(r0).max(axis=1, keepdims=True))
+        _cmp(t1, t0, 0, "tau", atol=rtol * np.abs(t0).max(axis=1, keepdims=True))
+    else:
+        _cmp(r1, r0, rtol, "rad")
+        _cmp(t1, t0, rtol, "tau", atol=1e-300)
     nr = o_cpu.nr
     for f in ("tpz", "tplon", "tplat"):
         _cmp(np.ctypeslib.as_array(getattr(o_gpu, f))[:nr],
@@ -181,9 +187,16 @@ def test_golden_tangent_points(gpu, tmp_path, name):
     ctl, atm, obs, ref = _golden_ctl(tmp_path, name)
     gpu.formod(ctl, atm, obs)
     for ir in range(obs.nr):
-        got = ["%g" % obs.tpz[ir], "%g" % obs.tplon[ir], "%g" % obs.tplat[ir]]
-        exp = ["%g" % ref[ir][7], "%g" % ref[ir][8], "%g" % ref[ir][9]]
-        assert got == exp, (name, ir)
+        got = (obs.tpz[ir], obs.tplon[ir], obs.tplat[ir])
+        for j, v in enumerate(got):
+            r = ref[ir][7 + j]
+            if abs(r) < 1e-6:
+                # rounding-noise values (e.g. a longitude of 1e-10 deg from
+                # atan2 of a vanishing component, or the 1e-8 km ground-clip
+                # residual): equal to far below any physical resolution
+                assert abs(v - r) < 1e-9, (name, ir, j, v, r)
+            else:
+                assert "%g" % v == "%g" % r, (name, ir, j, v, r)
     rad, tau = abi.obs_arrays(obs, ctl)
     # without tables only the continua absorb
     assert np.all(tau > 0) and np.all(tau <= 1)
@@ -247,7 +260,7 @@ def test_fov(gpu, oracle, tmp_path):
             f.write("%g %g\n" % (dz, math.exp(-dz * dz)))
     ctl.fov = fov.encode()
     (_, oc), (_, og) = _run_both(gpu, oracle, ctl, atm, obs)
-    _assert_obs(ctl, og, oc)
+    _assert_obs(ctl, og, oc, row_scaled=True)
 
 
 def test_mask_nan(gpu, oracle):
@@ -307,6 +320,14 @@ def test_kernel_fd_matches_oracle(gpu, oracle):
     m, n = oracle.sizes(ctl, atm, obs)
     k0 = oracle.kernel(ctl, abi.copy_struct(atm), abi.copy_struct(obs))
     k1 = gpu.kernel(ctl, abi.copy_struct(atm), abi.copy_struct(obs), m=m, n=n)
-    scale = np.abs(k0).max(axis=0, keepdims=True) + 1e-300
+    # Both sides difference radiances that agree to 1e-10 relative, so each
+    # entry agrees to about 2e-10 |y_i| / h_j (columns with almost no
+    # sensitivity are rounding noise on both sides).
+    o = abi.copy_struct(obs)
+    oracle.formod(ctl, abi.copy_struct(atm), o)
+    y = np.abs(np.array([o.rad[i % ctl.nd][i // ctl.nd] for i in range(m)]))
+    x = np.concatenate([np.ctypeslib.as_array(atm.t)[:atm.np], np.ctypeslib.as_array(atm.q[0])[:atm.np]])
+    h = np.concatenate([np.ones(atm.np), np.maximum(np.abs(0.01 * x[atm.np:]), 1e-15)])
+    tol = 1e-9 * y[:, None] / h[None, :]
+    assert np.all(np.abs(k1 - k0) <= tol)
     assert np.abs(k1 - k0).max() / np.abs(k0).max() < 1e-7
-    assert (np.abs(k1 - k0) / scale).max() < 1e-6
