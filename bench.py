@@ -130,29 +130,34 @@ def main():
         for _ in range(warmup):
             step(precision)
         torch.cuda.synchronize()
-        kms = 0.0
+        kms = [0.0, 0.0, 0.0]
         t0 = time.perf_counter()
         for _ in range(steps):
             step(precision)
-            kms += ctx.last_timing()[1]
+            kms = [a + b for a, b in zip(kms, ctx.last_timing())]
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        return dt, kms / steps
+        return dt, [k / steps for k in kms]
 
     dt32, k32 = timed(abi.JUR_FP32, args.steps, args.warmup)
     mean_np = ctx.last_mean_np()
-    log("FP32: %.1f ms/step, radiance kernel %.1f ms, mean LOS points %.1f"
-        % (dt32 * 1e3 / args.steps, k32, mean_np))
+    log("FP32: %.1f ms/step; kernels: ray tracing %.1f ms, lookups %.1f ms, accumulation %.1f ms; "
+        "mean LOS points %.1f" % (dt32 * 1e3 / args.steps, k32[0], k32[1], k32[2], mean_np))
     res = {}
     if not args.no_fp64:
         dt64, k64 = timed(abi.JUR_FP64, max(1, args.steps // 2), 1)
         res["fp64_value"] = nray * nd * max(1, args.steps // 2) / dt64
-        res["fp64_kernel_ms"] = k64
-        log("FP64: %.1f ms/step, radiance kernel %.1f ms" % (dt64 * 1e3 / max(1, args.steps // 2), k64))
+        res["fp64_kernel_ms"] = {"raytrace": k64[0], "lookup": k64[1], "accumulate": k64[2]}
+        log("FP64: %.1f ms/step; kernels: ray tracing %.1f ms, lookups %.1f ms, accumulation %.1f ms"
+            % (dt64 * 1e3 / max(1, args.steps // 2), k64[0], k64[1], k64[2]))
     nrad = nray * nd
     value = nrad * args.steps / dt32
+    # algorithmic bytes: 176 B per (segment, channel, emitter) table lookup,
+    # all of them in the lookup kernel (the dominant kernel); the 16 B Planck
+    # bracket per (segment, channel) belongs to the accumulation kernel
     bytes_per_rad = mean_np * (len(GASES) * 176 + 16)
-    achieved = bytes_per_rad * nrad / (k32 * 1e-3)
+    lookup_bytes = mean_np * len(GASES) * 176 * nrad
+    achieved = lookup_bytes / (k32[1] * 1e-3)
     out = {
         "metric": METRIC,
         "value": value,
@@ -180,9 +185,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": None,
-            "kernel": "k_radiance_f32<3,false>",
-            "kernel_ms": k32,
+            "kernel": "k_lookup_f32<false>",
+            "kernel_ms": k32[1],
+            "bytes_per_launch_step": lookup_bytes,
             "bytes_per_radiance": bytes_per_rad,
+            "kernel_ms_all": {"raytrace": k32[0], "lookup": k32[1], "accumulate": k32[2]},
         },
     }
     out.update(res)

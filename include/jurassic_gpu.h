@@ -261,7 +261,8 @@ int jur_ctx_formod_device(jur_ctx_t *ctx, const jur_batch_t *batch,
 
 /* Kernel timing of the last jur_ctx_formod_device call on this context, in
    milliseconds, measured with HIP events on the stream the kernels ran on:
-   [0] raytrace, [1] radiance (table lookups + accumulation). */
+   [0] ray tracing, [1] emissivity-table lookups, [2] radiance accumulation
+   (continua, source function, surface terms).  n <= 3 entries are written. */
 int jur_ctx_last_timing(const jur_ctx_t *ctx, double *ms, int n);
 
 /* Mean number of LOS points per ray of the last pass. */

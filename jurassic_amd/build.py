@@ -30,6 +30,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 SOURCES = [
     ("jurassic_gpu.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ("raytrace.hip", ["-ffp-contract=off"]),
+    ("lookup_f64.hip", ["-ffp-contract=off"]),
+    ("lookup_f32.hip", ["-ffp-contract=fast"]),
     ("radiance_f64.hip", ["-ffp-contract=off"]),
     ("radiance_f32.hip", ["-ffp-contract=fast"]),
     ("aux_kernels.hip", ["-ffp-contract=off"]),

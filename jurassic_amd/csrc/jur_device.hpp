@@ -77,6 +77,23 @@ __device__ __forceinline__ int locate_irr(const T *xx, int n, double x) {
   return ilo;
 }
 
+// locate_irr by a local walk from a previous index g, for strictly
+// monotone grids (dir = +1 ascending, -1 descending).  The result satisfies
+// the bisection's end condition, which for a strictly monotone grid has a
+// single solution, so it equals locate_irr's.
+template <typename T>
+__device__ __forceinline__ int irr_walk(const T *xx, int n, double x, int g, int dir) {
+  g = g < 0 ? 0 : (g > n - 2 ? n - 2 : g);
+  if (dir > 0) {
+    while (g > 0 && (double)xx[g] > x) g--;
+    while (g < n - 2 && !((double)xx[g + 1] > x)) g++;
+  } else {
+    while (g > 0 && (double)xx[g] <= x) g--;
+    while (g < n - 2 && !((double)xx[g + 1] <= x)) g++;
+  }
+  return g;
+}
+
 // locate_reg (src/jurassic.c:4133-4148).
 __device__ __forceinline__ int locate_reg(const double *xx, int n, double x) {
   JUR_FP_EXACT
@@ -157,8 +174,23 @@ __device__ __forceinline__ int col_bisect(const float4 *col, int n, double x, fl
 }
 
 // intpol_tbl_eps on one column (src/jurassic.c:3895-3929).
+// For a monotone column the local walk runs first and the two boundary
+// branches are read off the bracket record it returns: x below the first
+// knot can only leave the walk at record 0, which holds (u_0, eps_0), and x
+// above the last knot only at record n-2, which holds (u_{n-1}, eps_{n-1}).
+// Branch conditions and arithmetic are those of the reference.
 __device__ __forceinline__ double col_eps(const float4 *col, int n, double x, int mono, int &g) {
   JUR_FP_EXACT
+  float4 r;
+  if (mono) {
+    g = col_walk<0>(col, n, x, g, r);
+    if (x < (double)r.x) return lin_f(0.f, 0.f, r.x, r.y, x);
+    if (x > (double)r.z) {
+      const double a = log((double)(1.f - r.w)) / (double)r.z;
+      return 1 - exp(a * x);
+    }
+    return lin_f(r.x, r.y, r.z, r.w, x);
+  }
   const float4 r0 = col[0];
   if (x < (double)r0.x) return lin_f(0.f, 0.f, r0.x, r0.y, x);
   const float4 rl = col[n - 1];
@@ -166,14 +198,23 @@ __device__ __forceinline__ double col_eps(const float4 *col, int n, double x, in
     const double a = log((double)(1.f - rl.y)) / (double)rl.x;
     return 1 - exp(a * x);
   }
-  float4 r;
-  g = mono ? col_walk<0>(col, n, x, g, r) : col_bisect<0>(col, n, x, r);
+  g = col_bisect<0>(col, n, x, r);
   return lin_f(r.x, r.y, r.z, r.w, x);
 }
 
-// intpol_tbl_u on one column (src/jurassic.c:3933-3967).
+// intpol_tbl_u on one column (src/jurassic.c:3933-3967), same scheme.
 __device__ __forceinline__ double col_u(const float4 *col, int n, double x, int mono, int &g) {
   JUR_FP_EXACT
+  float4 r;
+  if (mono) {
+    g = col_walk<1>(col, n, x, g, r);
+    if (x < (double)r.y) return lin_f(0.f, 0.f, r.y, r.x, x);
+    if (x > (double)r.w) {
+      const double a = log((double)(1.f - r.w)) / (double)r.z;
+      return log(1 - x) / a;
+    }
+    return lin_f(r.y, r.x, r.w, r.z, x);
+  }
   const float4 r0 = col[0];
   if (x < (double)r0.y) return lin_f(0.f, 0.f, r0.y, r0.x, x);
   const float4 rl = col[n - 1];
@@ -181,10 +222,50 @@ __device__ __forceinline__ double col_u(const float4 *col, int n, double x, int 
     const double a = log((double)(1.f - rl.y)) / (double)rl.x;
     return log(1 - x) / a;
   }
-  float4 r;
-  g = mono ? col_walk<1>(col, n, x, g, r) : col_bisect<1>(col, n, x, r);
+  g = col_bisect<1>(col, n, x, r);
   return lin_f(r.y, r.x, r.w, r.z, x);
 }
+
+// Table cell of a (p, T) point: pressure bracket ipr and the temperature
+// brackets it0, it1 of the two pressure levels (intpol_tbl_ega,
+// src/jurassic.c:3826-3832).  ok is false when a temperature row has fewer
+// than two entries (the reference then uses eps = 0).
+struct Cell {
+  int ipr, it0, it1;
+  bool ok;
+  double pa, pb;     // p[ipr], p[ipr+1]
+  double ta0, tb0;   // t[ipr][it0], t[ipr][it0+1]
+  double ta1, tb1;   // t[ipr+1][it1], t[ipr+1][it1+1]
+};
+
+__device__ __forceinline__ Cell locate_cell(const TabView &T, int tix, int npt, double p, double t) {
+  Cell c;
+  const double *pg = T.p + (size_t)tix * T.TP;
+  c.ipr = locate_irr(pg, npt, p);
+  c.pa = pg[c.ipr];
+  c.pb = pg[c.ipr + 1];
+  const int *ntp = T.nt + (size_t)tix * T.TP;
+  const int nt0 = ntp[c.ipr], nt1 = ntp[c.ipr + 1];
+  c.ok = (nt0 >= 2 && nt1 >= 2);
+  c.it0 = c.it1 = 0;
+  c.ta0 = c.tb0 = c.ta1 = c.tb1 = 0;
+  if (c.ok) {
+    const double *tg0 = T.t + ((size_t)tix * T.TP + c.ipr) * T.TT;
+    const double *tg1 = tg0 + T.TT;
+    c.it0 = locate_reg(tg0, nt0, t);
+    c.it1 = locate_reg(tg1, nt1, t);
+    c.ta0 = tg0[c.it0];
+    c.tb0 = tg0[c.it0 + 1];
+    c.ta1 = tg1[c.it1];
+    c.tb1 = tg1[c.it1 + 1];
+  }
+  return c;
+}
+
+// Per-(channel, emitter) table constants, loaded once per wavefront.
+struct GasConst {
+  int tix, npt, mono;
+};
 
 // ------------------------------------------------------------------
 // Continua (src/jurassic.c:886-2918), FP64, reference expression trees.

@@ -41,6 +41,12 @@ struct TabView {
   const int *mono;         // [ng][nd]: 1 if every column is non-decreasing
                            // in u and eps (search by local walk), 0 otherwise
                            // (search by the reference bisection)
+  const int *pdir;         // [ng][nd]: +1/-1 strictly ascending/descending p grid,
+                           // 0 otherwise (search by bisection)
+  int shared;              // 1 if every table with np >= 30 has the same p grid,
+                           // T-grid sizes and T grids (one cell search per
+                           // segment serves all of them)
+  int ref_tix;             // a table holding the shared grids
 };
 
 // ------------------------------------------------------------------
@@ -56,8 +62,8 @@ enum {
   F_LAT = 2,
   F_P = 3,
   F_T = 4,
-  F_DS = 5,   // raw step length before the trapezoid rule
-  F_Q0 = 6    // then q[ng], k[nw], cloud factor (if clouds)
+  F_DS = 5,   // step length: raw while tracing, trapezoid rule afterwards
+  F_Q0 = 6    // then q[ng], k[nw], cloud factor (if clouds), u[ng]
 };
 
 struct LosView {
@@ -66,6 +72,17 @@ struct LosView {
   int nfield;
   int f_k;      // first window-extinction field
   int f_cl;     // cloud factor field, -1 if none
+  int f_u;      // first column-density field
+};
+
+// Per-segment emissivities of every (channel, emitter), written by the
+// lookup kernels and read by the radiance kernels.  Per block of WAVE rays:
+//   eps[((blk * nd + id) * ng + ig) * cap + ip) * WAVE + lane]
+// FP64 path: the reference's eps of intpol_tbl_ega/cga (double).
+// FP32 path: the segment emissivity (float); 1 marks an opaque segment.
+struct EpsView {
+  void *eps;
+  int cap;
 };
 
 // Per-ray results of the ray tracer.

@@ -33,10 +33,14 @@ namespace jur {
 void launch_raytrace(const ModelView &, const BatchView &, const LosView &, RayInfo *, int, int,
                      hipStream_t);
 void launch_hydrostatic(atm_t *, int, double, int, hipStream_t);
-void launch_radiance_f64(const ModelView &, const BatchView &, const LosView &, const RayInfo *,
-                         int, int, double *, double *, hipStream_t);
-void launch_radiance_f32(const ModelView &, const BatchView &, const LosView &, const RayInfo *,
-                         int, int, double *, double *, hipStream_t);
+void launch_lookup_f64(const ModelView &, const BatchView &, const LosView &, const EpsView &,
+                       const RayInfo *, int, int, hipStream_t);
+void launch_lookup_f32(const ModelView &, const BatchView &, const LosView &, const EpsView &,
+                       const RayInfo *, int, int, hipStream_t);
+void launch_radiance_f64(const ModelView &, const BatchView &, const LosView &, const EpsView &,
+                         const RayInfo *, int, int, double *, double *, hipStream_t);
+void launch_radiance_f32(const ModelView &, const BatchView &, const LosView &, const EpsView &,
+                         const RayInfo *, int, int, double *, double *, hipStream_t);
 void launch_srcfunc(const double *, const double *, const double *, const int *, const int *,
                     const double *, int, double *, hipStream_t);
 void launch_zrange(const atm_t *, int, double *, hipStream_t);
@@ -329,10 +333,11 @@ struct jur_ctx {
   jur::Setup s;
   int device = 0;
   // tables
-  jur::DevBuf<int> d_np, d_nt, d_nu, d_mono;
+  jur::DevBuf<int> d_np, d_nt, d_nu, d_mono, d_pdir;
   jur::DevBuf<double> d_p, d_t;
   jur::DevBuf<float4> d_rec;
   int TP = 2, TT = 2, TU = 2;
+  int shared = 0, ref_tix = 0;
   size_t table_bytes = 0;
   // model constants
   jur::DevBuf<double> d_st, d_sr, d_clnu, d_sfnu, d_ctm;
@@ -343,6 +348,7 @@ struct jur_ctx {
   jur::DevBuf<double> d_fov_dz, d_fov_w;
   // work buffers
   jur::DevBuf<double> d_los;
+  jur::DevBuf<char> d_eps;
   jur::DevBuf<jur::RayInfo> d_info;
   jur::DevBuf<double> d_zr;
   jur::DevBuf<unsigned long long> d_red;
@@ -350,7 +356,7 @@ struct jur_ctx {
   jur::DevBuf<int> d_err;
   size_t los_budget = (size_t)16 << 30;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double last_ms[2] = {0, 0};
+  double last_ms[3] = {0, 0, 0};
   double last_mean_np = 0;
 
   ~jur_ctx() {
@@ -382,7 +388,7 @@ static int build_tables(jur_ctx *c) {
   c->TU = TU;
   const size_t ntab = (size_t)nd * ng;
   std::vector<int> h_np(ntab + 1, -1), h_nt(ntab * TP + 1, 0), h_nu(ntab * TP * TT + 1, 0),
-      h_mono(ntab + 1, 1);
+      h_mono(ntab + 1, 1), h_pdir(ntab + 1, 0);
   std::vector<double> h_p(ntab * TP + 1, 0.0), h_t(ntab * TP * TT + 1, 0.0);
   std::vector<float4> h_rec(ntab * TP * TT * TU + 1, make_float4(0, 0, 0, 0));
   for (size_t k = 0; k < ntab; k++) {
@@ -407,16 +413,44 @@ static int build_tables(jur_ctx *c) {
       }
     }
     h_mono[k] = mono;
+    int asc = 1, desc = 1;
+    for (int ip = 0; ip + 1 < tb.np; ip++) {
+      if (!(tb.p[ip + 1] > tb.p[ip])) asc = 0;
+      if (!(tb.p[ip + 1] < tb.p[ip])) desc = 0;
+    }
+    h_pdir[k] = (tb.np >= 2 && asc) ? 1 : ((tb.np >= 2 && desc) ? -1 : 0);
   }
   int rc;
   if ((rc = c->d_np.upload(h_np.data(), h_np.size()))) return rc;
   if ((rc = c->d_nt.upload(h_nt.data(), h_nt.size()))) return rc;
   if ((rc = c->d_nu.upload(h_nu.data(), h_nu.size()))) return rc;
   if ((rc = c->d_mono.upload(h_mono.data(), h_mono.size()))) return rc;
+  if ((rc = c->d_pdir.upload(h_pdir.data(), h_pdir.size()))) return rc;
   if ((rc = c->d_p.upload(h_p.data(), h_p.size()))) return rc;
   if ((rc = c->d_t.upload(h_t.data(), h_t.size()))) return rc;
   if ((rc = c->d_rec.upload(h_rec.data(), h_rec.size()))) return rc;
   c->table_bytes = h_rec.size() * sizeof(float4);
+  // shared grids: every table with np >= 30 has the same p grid, T-row sizes
+  // and T grids as the first such table
+  c->shared = 0;
+  c->ref_tix = 0;
+  int ref = -1;
+  for (size_t k = 0; k < ntab; k++)
+    if (h_np[k] >= 30) { ref = (int)k; break; }
+  if (ref >= 0) {
+    bool same = true;
+    for (size_t k = 0; k < ntab && same; k++) {
+      if (h_np[k] < 30) continue;
+      if (h_np[k] != h_np[ref]) { same = false; break; }
+      for (int ip = 0; ip < TP && same; ip++) {
+        if (h_p[k * TP + ip] != h_p[ref * TP + ip] || h_nt[k * TP + ip] != h_nt[ref * TP + ip]) same = false;
+        for (int it = 0; it < TT && same; it++)
+          if (h_t[(k * TP + ip) * TT + it] != h_t[(ref * TP + ip) * TT + it]) same = false;
+      }
+    }
+    c->shared = same ? 1 : 0;
+    c->ref_tix = ref;
+  }
   return JUR_OK;
 }
 
@@ -549,6 +583,9 @@ static ModelView model_view(const jur_ctx *c) {
   T.t = c->d_t.p;
   T.rec = c->d_rec.p;
   T.mono = c->d_mono.p;
+  T.pdir = c->d_pdir.p;
+  T.shared = c->shared;
+  T.ref_tix = c->ref_tix;
   return M;
 }
 
@@ -574,7 +611,7 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
   const Setup &s = c->s;
   ModelView M = model_view(c);
   int rc;
-  c->last_ms[0] = c->last_ms[1] = 0;
+  c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = 0;
   if (B.nray <= 0) return JUR_OK;
 
   if (s.hydz >= 0) {
@@ -612,38 +649,44 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
     L.cap = cap;
     L.f_k = F_Q0 + s.ng;
     L.f_cl = s.ncl > 0 ? L.f_k + s.nw : -1;
-    L.nfield = L.f_k + s.nw + (s.ncl > 0 ? 1 : 0);
-    const size_t per_blk = (size_t)L.nfield * L.cap * WAVE;
-    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, c->los_budget / (per_blk * sizeof(double))));
-    if ((rc = c->d_los.reserve(per_blk * chunk))) return rc;
+    L.f_u = L.f_k + s.nw + (s.ncl > 0 ? 1 : 0);
+    L.nfield = L.f_u + s.ng;
+    const size_t esz = (precision == JUR_FP32) ? sizeof(float) : sizeof(double);
+    const size_t per_blk = (size_t)L.nfield * L.cap * WAVE * sizeof(double) +
+                           (size_t)s.nd * s.ng * L.cap * WAVE * esz;
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, c->los_budget / per_blk));
+    if ((rc = c->d_los.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
+    if ((rc = c->d_eps.reserve((size_t)s.nd * s.ng * L.cap * WAVE * esz * chunk + 1))) return rc;
     L.los = c->d_los.p;
-    float ms_rt = 0, ms_rad = 0;
+    EpsView E{c->d_eps.p, L.cap};
+    float ms[3] = {0, 0, 0};
     for (int b0 = 0; b0 < nblk; b0 += chunk) {
       const int nb = std::min(chunk, nblk - b0);
       HIPCHK(hipEventRecord(c->ev[0], st));
       launch_raytrace(M, B, L, c->d_info.p, b0, nb, st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->ev[1], st));
-      if (precision == JUR_FP32)
-        launch_radiance_f32(M, B, L, c->d_info.p, b0, nb, d_rad, d_tau, st);
-      else
-        launch_radiance_f64(M, B, L, c->d_info.p, b0, nb, d_rad, d_tau, st);
+      if (precision == JUR_FP32) launch_lookup_f32(M, B, L, E, c->d_info.p, b0, nb, st);
+      else launch_lookup_f64(M, B, L, E, c->d_info.p, b0, nb, st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->ev[2], st));
-      HIPCHK(hipEventSynchronize(c->ev[2]));
-      float a = 0, b = 0;
-      HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-      HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-      ms_rt += a;
-      ms_rad += b;
+      if (precision == JUR_FP32) launch_radiance_f32(M, B, L, E, c->d_info.p, b0, nb, d_rad, d_tau, st);
+      else launch_radiance_f64(M, B, L, E, c->d_info.p, b0, nb, d_rad, d_tau, st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c->ev[3], st));
+      HIPCHK(hipEventSynchronize(c->ev[3]));
+      for (int k = 0; k < 3; k++) {
+        float a = 0;
+        HIPCHK(hipEventElapsedTime(&a, c->ev[k], c->ev[k + 1]));
+        ms[k] += a;
+      }
     }
     launch_reduce_info(c->d_info.p, B.nray, c->d_red.p, st);
     HIPCHK(hipGetLastError());
     unsigned long long red[3];
     HIPCHK(hipMemcpyAsync(red, c->d_red.p, sizeof red, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    c->last_ms[0] = ms_rt;
-    c->last_ms[1] = ms_rad;
+    for (int k = 0; k < 3; k++) c->last_ms[k] = ms[k];
     c->last_mean_np = (double)red[1] / B.nray;
     if (red[0] == 100 && cap < NLOS) {
       cap = NLOS;
@@ -1088,7 +1131,7 @@ int jur_ctx_formod_device(jur_ctx_t *ctx, const jur_batch_t *b, int precision, v
 
 int jur_ctx_last_timing(const jur_ctx_t *ctx, double *ms, int n) {
   if (!ctx || !ms) return set_err(JUR_ERR_ARG, "NULL argument");
-  for (int i = 0; i < n && i < 2; i++) ms[i] = ctx->last_ms[i];
+  for (int i = 0; i < n && i < 3; i++) ms[i] = ctx->last_ms[i];
   return JUR_OK;
 }
 

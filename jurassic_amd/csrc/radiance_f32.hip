@@ -1,136 +1,23 @@
-// FP32 radiance kernel: the same EGA/CGA recurrence as radiance_f64.hip
-// with FP32 table interpolation, continua, source function and accumulation.
+// FP32 radiance kernel: combines the FP32 segment emissivities of the lookup
+// kernel with continua and the source function, accumulating radiance and
+// transmittance along each line of sight (formod_pencil,
+// src/jurassic.c:3233-3334) in float with compensated sums.
 //
-// Precision plan (error budget 0.01 K in brightness temperature against the
-// FP64 path):
-//  * The EGA state is carried as the path emissivity eps_path = 1 - tau_path
-//    (the reference computes tau_path *= 1 - eps with eps = 1 - (1 - eps00) /
-//    tau_path, i.e. tau_path becomes exactly 1 - eps00).  Small emissivities
-//    keep full relative precision in float, where 1 - tau_path would not.
-//  * The inverse lookup u(eps_path) and the forward lookup eps(u + u_seg) keep
-//    their column-density arithmetic (the bracket search on u, u_eq + u_seg)
-//    in FP64: u_seg is often 1e-3 of u_eq, below float resolution otherwise.
-//  * Segment emissivities combine as 1 - prod(1 - e_g) in the cancellation-
-//    free form e + e_g - e e_g; radiance and transmittance are accumulated
-//    with compensated (Kahan / two-sum) arithmetic.
-// Ray geometry is the FP64 ray tracer's.
+// * Emitters combine as e = 1 - prod_g (1 - e_g) in the cancellation-free
+//   form e + e_g - e e_g, and the continuum as e + c - e c with
+//   c = 1 - exp(-beta ds).
+// * With d_i = tau_{i-1} e_i the transmittance is updated by a two-sum
+//   (tau -= d with its rounding error carried) and the radiance
+//   sum_i src_i d_i is a Kahan sum.
+// * A segment with an opaque emitter (the reference's tau_gas == 0) is
+//   skipped together with all later ones, as the reference does once
+//   tau_path falls below 1e-9.
 #include "jur_device.hpp"
 
 namespace jur {
 
-struct GasStateF {
-  float ep;          // path emissivity 1 - tau_path
-  int gi[8];
-  double cu, cp, ct; // Curtis-Godson running sums (FP64: sums of up to 4096 terms)
-};
+#define LOSR(f, ip) los[((size_t)(f) * L.cap + (size_t)(ip)) * WAVE]
 
-// Column lookups with float interpolation and a double column-density axis.
-__device__ __forceinline__ float col_eps_f(const float4 *col, int n, double x, int mono, int &g) {
-  const float4 r0 = col[0];
-  if (x < (double)r0.x) return (r0.y / r0.x) * (float)x;
-  const float4 rl = col[n - 1];
-  if (x > (double)rl.x) {
-    const float a = __logf(1.f - rl.y) / rl.x;
-    return 1.f - __expf(a * (float)x);
-  }
-  float4 r;
-  g = mono ? col_walk<0>(col, n, x, g, r) : col_bisect<0>(col, n, x, r);
-  const float s = (r.w - r.y) / (r.z - r.x);
-  return r.y + s * (float)(x - (double)r.x);
-}
-
-__device__ __forceinline__ double col_u_f(const float4 *col, int n, float x, int mono, int &g) {
-  const float4 r0 = col[0];
-  if (x < r0.y) return (double)(r0.x / r0.y) * (double)x;
-  const float4 rl = col[n - 1];
-  if (x > rl.y) {
-    const float a = __logf(1.f - rl.y) / rl.x;
-    return (double)(log1pf(-x) / a);
-  }
-  float4 r;
-  g = mono ? col_walk<1>(col, n, (double)x, g, r) : col_bisect<1>(col, n, (double)x, r);
-  const float s = (r.z - r.x) / (r.w - r.y);
-  return (double)r.x + (double)(s * (x - r.y));
-}
-
-// One (channel, emitter) segment update.  Returns the segment emissivity of
-// the emitter and updates its path emissivity; *sat is set when the segment
-// is opaque for this emitter (the reference's tau_gas == 0 case).
-template <bool CGA>
-__device__ __forceinline__ float seg_eps_f(const TabView &T, int tix, double p, double t, double useg,
-                                           GasStateF &s, bool *sat) {
-  const int npt = T.np[tix];
-  if (npt < 30) return 0.f;
-  if (s.ep >= 1.f) {  // tau_path below float resolution: eps = 1
-    *sat = true;
-    return 1.f;
-  }
-  double pq = p, tq = t;
-  if (CGA) {
-    pq = s.cp / s.cu;
-    tq = s.ct / s.cu;
-  }
-  const double *pg = T.p + (size_t)tix * T.TP;
-  const int ipr = locate_irr(pg, npt, pq);
-  const int *ntp = T.nt + (size_t)tix * T.TP;
-  const size_t c0 = (size_t)tix * T.TP + ipr;
-  const double *tg0 = T.t + c0 * T.TT;
-  const double *tg1 = tg0 + T.TT;
-  // the reference tests all six sizes together; testing the T sizes first
-  // keeps the u-size reads in range and gives the same eps = 0
-  if (ntp[ipr] < 2 || ntp[ipr + 1] < 2) return 0.f;
-  const int it0 = locate_reg(tg0, ntp[ipr], tq);
-  const int it1 = locate_reg(tg1, ntp[ipr + 1], tq);
-  const int *nu0 = T.nu + c0 * T.TT;
-  const int *nu1 = nu0 + T.TT;
-  const int n00 = nu0[it0], n01 = nu0[it0 + 1], n10 = nu1[it1], n11 = nu1[it1 + 1];
-  if (n00 < 2 || n01 < 2 || n10 < 2 || n11 < 2) return 0.f;
-  const float4 *k00 = T.rec + (c0 * T.TT + it0) * T.TU;
-  const float4 *k01 = k00 + T.TU;
-  const float4 *k10 = T.rec + ((c0 + 1) * T.TT + it1) * T.TU;
-  const float4 *k11 = k10 + T.TU;
-  const int mono = T.mono[tix];
-  float e00, e01, e10, e11;
-  if (CGA) {
-    e00 = col_eps_f(k00, n00, s.cu, mono, s.gi[4]);
-    e01 = col_eps_f(k01, n01, s.cu, mono, s.gi[5]);
-    e10 = col_eps_f(k10, n10, s.cu, mono, s.gi[6]);
-    e11 = col_eps_f(k11, n11, s.cu, mono, s.gi[7]);
-  } else {
-    const float x = s.ep;
-    e00 = col_eps_f(k00, n00, col_u_f(k00, n00, x, mono, s.gi[0]) + useg, mono, s.gi[4]);
-    e01 = col_eps_f(k01, n01, col_u_f(k01, n01, x, mono, s.gi[1]) + useg, mono, s.gi[5]);
-    e10 = col_eps_f(k10, n10, col_u_f(k10, n10, x, mono, s.gi[2]) + useg, mono, s.gi[6]);
-    e11 = col_eps_f(k11, n11, col_u_f(k11, n11, x, mono, s.gi[3]) + useg, mono, s.gi[7]);
-  }
-  const float wt0 = (float)((tq - tg0[it0]) / (tg0[it0 + 1] - tg0[it0]));
-  const float wt1 = (float)((tq - tg1[it1]) / (tg1[it1 + 1] - tg1[it1]));
-  float a = e00 + (e01 - e00) * wt0;
-  float b = e10 + (e11 - e10) * wt1;
-  float w;
-  if (CGA) {
-    const double x0 = pg[ipr], x1 = pg[ipr + 1];
-    w = (pq / x0 > 0 && x1 / x0 > 0) ? (float)(log(pq / x0) / log(x1 / x0))
-                                      : (float)((pq - x0) / (x1 - x0));
-  } else {
-    w = (float)((pq - pg[ipr]) / (pg[ipr + 1] - pg[ipr]));
-  }
-  float e = a + (b - a) * w;
-  e = (e < 1.f) ? e : 1.f;
-  e = (e > 0.f) ? e : 0.f;
-  const float ep0 = s.ep;
-  s.ep = e;
-  if (e >= 1.f) {
-    *sat = true;
-    return 1.f;
-  }
-  // segment emissivity 1 - (1 - e)/(1 - ep0) = (e - ep0)/(1 - ep0)
-  return (e - ep0) / (1.f - ep0);
-}
-
-#define LOSR(f, ip) base[((size_t)(f) * L.cap + (size_t)(ip)) * WAVE]
-
-// Kahan-compensated sum.
 struct KSum {
   float s = 0.f, c = 0.f;
   __device__ __forceinline__ void add(float v) {
@@ -141,11 +28,10 @@ struct KSum {
   }
 };
 
-template <int NGAS, bool CGA>
 __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B, LosView L,
-                                                       const RayInfo *info, int blk0, int nblk,
-                                                       double *rad_out, double *tau_out) {
-  const int nd = M.nd;
+                                                       EpsView E, const RayInfo *info, int blk0,
+                                                       int nblk, double *rad_out, double *tau_out) {
+  const int nd = M.nd, ng = M.ng;
   const int w = xcd_remap(blockIdx.x, nblk * nd);
   const int bl = w / nd;
   const int id = w - bl * nd;
@@ -159,7 +45,8 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   const ChanInfo C = M.chan[id];
   const double nu = C.nu;
   const float nuf = (float)nu;
-  const double *base = L.los + (size_t)bl * L.nfield * L.cap * WAVE + lane;
+  const double *los = L.los + (size_t)bl * L.nfield * L.cap * WAVE + lane;
+  const float *eps_g = (const float *)E.eps + ((size_t)bl * nd + id) * ng * E.cap * WAVE + lane;
   const double *sr_col = M.sr + (size_t)id * TBLNS;
 
   float kcl = 0.f;
@@ -168,11 +55,12 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   float sfeps = 1.f;
   if (M.nsf > 0)
     sfeps = (float)lin(M.sfnu[C.isf], atm->sfeps[C.isf], M.sfnu[C.isf + 1], atm->sfeps[C.isf + 1], nu);
+  const bool has_k = C.window < M.nw;
 
   // channel constants of the continua (src/jurassic.c:1724-1741, 2762-2791,
   // 2841-2917), evaluated once in FP64
   bool co2_on = false, h2o_on = false, n2_on = false, o2_on = false;
-  float co2_a = 0, co2_b = 0, co2_c = 0;      // ctw(t) = a dt260 dt296 - b dt230 dt296 + c dt230 dt260
+  float co2_a = 0, co2_b = 0, co2_c = 0;  // ctw = a d260 d296 - b d230 d296 + c d230 d260
   float h2o_slf = 0, h2o_l2r = 0, h2o_frn = 0;
   float n2_b = 0, n2_beta = 0, o2_b = 0, o2_beta = 0;
   if (M.ig_co2 >= 0) {
@@ -223,77 +111,41 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   }
   const float st0 = (float)M.st[0];
   const float inv_dst = (float)(1.0 / (M.st[1] - M.st[0]));
+  const float kctm = (float)(1.0 / (JUR_NA * 1000.0 * JUR_P0));
 
-  constexpr int NA = NGAS > 0 ? NGAS : 1;
-  GasStateF gs[NA];
-#pragma unroll
-  for (int g = 0; g < NGAS; g++) {
-    gs[g].ep = 0.f;
-    gs[g].cu = gs[g].cp = gs[g].ct = 0;
-#pragma unroll
-    for (int c = 0; c < 8; c++) gs[g].gi[c] = M.tab.TU / 2;
-  }
+  // emitters with a table (the others have eps = 0)
+  unsigned gmask = 0;
+  for (int g = 0; g < ng; g++)
+    if (M.tab.np[g * nd + id] >= 30) gmask |= 1u << g;
 
   KSum rad;
-  float tau_hi = 1.f, tau_lo = 0.f;   // two-sum compensated transmittance
+  float tau_hi = 1.f, tau_lo = 0.f;
   float drefl = 0.f, tau_refl = 1.f;
-  double ds_prev = 0;
-  bool dead = false;
-  for (int ip = 0; ip < R.np && !dead; ip++) {
-    const double p = LOSR(F_P, ip);
-    const double t = LOSR(F_T, ip);
-    const double dsr = LOSR(F_DS, ip);
-    const double ds = (ip == 0) ? dsr * 0.5 : 0.5 * (ds_prev + dsr);
-    ds_prev = dsr;
-
-    double u[NA];
-    float qf[NA];
-#pragma unroll
-    for (int g = 0; g < NGAS; g++) {
-      const double q = LOSR(F_Q0 + g, ip);
-      qf[g] = (float)q;
-      u[g] = 10 * q * p / (JUR_KB * t) * ds;
-      if (CGA) {
-        gs[g].cu += u[g];
-        gs[g].cp += u[g] * p;
-        gs[g].ct += u[g] * t;
-      }
-    }
-
-    bool sat = false;
-    float etot = 0.f;  // 1 - prod_g (1 - e_g), cancellation-free
-#pragma unroll
-    for (int g = 0; g < NGAS; g++) {
-      const float e = seg_eps_f<CGA>(M.tab, g * nd + id, p, t, u[g], gs[g], &sat);
+  for (int ip = 0; ip < R.np; ip++) {
+    bool opaque = false;
+    float etot = 0.f;
+    for (int g = 0; g < ng; g++) {
+      if (!(gmask >> g & 1u)) continue;
+      const float e = eps_g[((size_t)g * E.cap + ip) * WAVE];
+      opaque |= (e >= 1.f);
       etot = etot + e - etot * e;
     }
-    if (sat) {
-      // tau_gas == 0: the reference skips this and (its path transmittance
-      // staying below 1e-9) every later segment
-      dead = true;
-      break;
-    }
+    if (opaque) break;
 
-    const float pf = (float)p, tf = (float)t, dsf = (float)ds;
-    float beta = (C.window < M.nw) ? (float)LOSR(L.f_k + C.window, ip) : 0.f;
+    const float pf = (float)LOSR(F_P, ip);
+    const float tf = (float)LOSR(F_T, ip);
+    const float dsf = (float)LOSR(F_DS, ip);
+    float beta = has_k ? (float)LOSR(L.f_k + C.window, ip) : 0.f;
     if (L.f_cl >= 0) beta += (float)LOSR(L.f_cl, ip) * kcl;
     if (co2_on) {
-      float uc = 0.f;
-#pragma unroll
-      for (int g = 0; g < NGAS; g++)
-        if (g == M.ig_co2) uc = (float)u[g];
+      const float uc = (float)LOSR(L.f_u + M.ig_co2, ip);
       const float d230 = tf - 230.f, d260 = tf - 260.f, d296 = tf - 296.f;
       const float ctw = co2_a * d260 * d296 - co2_b * d230 * d296 + co2_c * d230 * d260;
-      beta += uc * pf * (float)(1.0 / (JUR_NA * 1000.0 * JUR_P0)) * ctw / dsf;
+      beta += uc * pf * kctm * ctw / dsf;
     }
     if (h2o_on) {
-      float uh = 0.f, qh = 0.f;
-#pragma unroll
-      for (int g = 0; g < NGAS; g++)
-        if (g == M.ig_h2o) {
-          uh = (float)u[g];
-          qh = qf[g];
-        }
+      const float uh = (float)LOSR(L.f_u + M.ig_h2o, ip);
+      const float qh = (float)LOSR(F_Q0 + M.ig_h2o, ip);
       const float slf = h2o_slf * exp2f(h2o_l2r * (296.f - tf));
       const float a1 = nuf * uh * tanhf(0.7193876f / tf * nuf);
       const float a3 = pf * (float)(1.0 / JUR_P0) * (qh * slf + (1.f - qh) * h2o_frn) * 1e-20f;
@@ -315,12 +167,11 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
     const float s0 = (float)sr_col[it], s1 = (float)sr_col[it + 1];
     const float src = s0 + (s1 - s0) * ((tf - (float)M.st[it]) * inv_dst);
 
-    // e = 1 - (1 - etot) exp(-beta ds)
     const float c = -expm1f(-beta * dsf);
     const float e = etot + c - etot * c;
-    const float d = tau_hi * e;          // tau_{i-1} - tau_i
+    const float d = tau_hi * e;
     rad.add(src * d);
-    {  // tau -= d with a two-sum error term
+    {  // tau -= d with its rounding error carried in tau_lo
       const float s = tau_hi - d;
       const float bb = s - tau_hi;
       tau_lo += (tau_hi - (s - bb)) + (-d - bb);
@@ -345,30 +196,12 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   tau_out[(size_t)ir * nd + id] = taud;
 }
 
-template <int NGAS>
-static void launch_ng(const ModelView &M, const BatchView &B, const LosView &L, const RayInfo *info,
-                      int blk0, int nblk, double *rad, double *tau, hipStream_t s) {
-  const dim3 grid(nblk * M.nd), blk(WAVE);
-  if (M.formod == 0)
-    hipLaunchKernelGGL((k_radiance_f32<NGAS, true>), grid, blk, 0, s, M, B, L, info, blk0, nblk, rad, tau);
-  else
-    hipLaunchKernelGGL((k_radiance_f32<NGAS, false>), grid, blk, 0, s, M, B, L, info, blk0, nblk, rad, tau);
-}
-
-void launch_radiance_f32(const ModelView &M, const BatchView &B, const LosView &L,
+void launch_radiance_f32(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
                          const RayInfo *info, int blk0, int nblk, double *rad, double *tau,
                          hipStream_t s) {
-  switch (M.ng) {
-    case 0: launch_ng<0>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    case 1: launch_ng<1>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    case 2: launch_ng<2>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    case 3: launch_ng<3>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    case 4: launch_ng<4>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    case 5: launch_ng<5>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    case 6: launch_ng<6>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    case 7: launch_ng<7>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-    default: launch_ng<8>(M, B, L, info, blk0, nblk, rad, tau, s); break;
-  }
+  if (M.nd == 0) return;
+  hipLaunchKernelGGL(k_radiance_f32, dim3(nblk * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0,
+                     nblk, rad, tau);
 }
 
 }  // namespace jur

@@ -272,6 +272,16 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       R.pad = 1.0;
     }
   }
+
+  // trapezoid segment lengths and column densities (src/jurassic.c:4410-4419),
+  // after the tangent point, which uses the raw step lengths
+  for (int ip = np - 1; ip >= 1; ip--) LOSW(F_DS, ip) = 0.5 * (LOSW(F_DS, ip - 1) + LOSW(F_DS, ip));
+  LOSW(F_DS, 0) *= 0.5;
+  for (int ip = 0; ip < np; ip++) {
+    const double p = LOSW(F_P, ip), t = LOSW(F_T, ip), ds = LOSW(F_DS, ip);
+    for (int ig = 0; ig < M.ng; ig++)
+      LOSW(L.f_u + ig, ip) = 10 * LOSW(F_Q0 + ig, ip) * p / (JUR_KB * t) * ds;
+  }
   info[ir] = R;
 }
 

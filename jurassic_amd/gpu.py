@@ -129,9 +129,10 @@ class Context:
         _check(lib().jur_ctx_formod_device(self.ptr, C.byref(b), precision, stream))
 
     def last_timing(self):
-        ms = (C.c_double * 2)()
-        _check(lib().jur_ctx_last_timing(self.ptr, ms, 2))
-        return ms[0], ms[1]
+        """(ray tracing, table lookups, radiance accumulation) in ms."""
+        ms = (C.c_double * 3)()
+        _check(lib().jur_ctx_last_timing(self.ptr, ms, 3))
+        return ms[0], ms[1], ms[2]
 
     def last_mean_np(self):
         return lib().jur_ctx_last_mean_np(self.ptr)
