@@ -97,6 +97,8 @@ def main():
                     help="profiles for the CPU baseline (default: 4 per thread)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -108,6 +110,13 @@ def main():
     from jurassic_amd import abi, gpu
 
     torch.cuda.set_device(0)
+    if args.copy_calib:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
     ctl, atms, prof, geom = build_workload(args.nprof)
     nprof, nray, nd = args.nprof, len(prof), ctl.nd
     log("workload built: %d atmospheres, %d rays, %d channels" % (nprof, nray, nd))

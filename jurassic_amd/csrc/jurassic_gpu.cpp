@@ -354,7 +354,7 @@ struct jur_ctx {
   jur::DevBuf<unsigned long long> d_red;
   jur::DevBuf<double> d_rad0, d_tau0;
   jur::DevBuf<int> d_err;
-  size_t los_budget = (size_t)16 << 30;
+  size_t los_budget = 0;  // bytes for LOS + emissivity buffers (0: from free HBM)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
   double last_mean_np = 0;
@@ -654,7 +654,16 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
     const size_t esz = (precision == JUR_FP32) ? sizeof(float) : sizeof(double);
     const size_t per_blk = (size_t)L.nfield * L.cap * WAVE * sizeof(double) +
                            (size_t)s.nd * s.ng * L.cap * WAVE * esz;
-    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, c->los_budget / per_blk));
+    size_t budget = c->los_budget;
+    if (budget == 0) {
+      // half of the HBM free now (plus what these buffers already hold),
+      // at most 96 GiB: one chunk covers ~10^5 limb rays with 36 chains each
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      fr += c->d_los.n * sizeof(double) + c->d_eps.n;
+      budget = std::min(fr / 2, (size_t)96 << 30);
+    }
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, budget / per_blk));
     if ((rc = c->d_los.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
     if ((rc = c->d_eps.reserve((size_t)s.nd * s.ng * L.cap * WAVE * esz * chunk + 1))) return rc;
     L.los = c->d_los.p;

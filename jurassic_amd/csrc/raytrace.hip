@@ -10,11 +10,21 @@
 
 namespace jur {
 
+// Level search of intpol_atm: the bisection of locate_irr, or, for a
+// strictly monotone altitude grid, a walk from the previous point's level
+// (the LOS moves by a fraction of a level per step), which returns the same
+// index.
+__device__ __forceinline__ int atm_level(const atm_t *atm, int zdir, double z, int &g) {
+  g = (zdir != 0 && g >= 0) ? irr_walk(atm->z, atm->np, z, g, zdir) : locate_irr(atm->z, atm->np, z);
+  return g;
+}
+
 // intpol_atm (src/jurassic.c:3685-3706) without the per-gas outputs (the
 // LOS point writer reads q and k itself at the final position).
-__device__ __forceinline__ void intpol_pt(const atm_t *atm, double z, double *p, double *t) {
+__device__ __forceinline__ void intpol_pt(const atm_t *atm, int zdir, int &g, double z, double *p,
+                                          double *t) {
   JUR_FP_EXACT
-  const int ip = locate_irr(atm->z, atm->np, z);
+  const int ip = atm_level(atm, zdir, z, g);
   *p = logy(atm->z[ip], atm->p[ip], atm->z[ip + 1], atm->p[ip + 1], z);
   *t = lin(atm->z[ip], atm->t[ip], atm->z[ip + 1], atm->t[ip + 1], z);
 }
@@ -74,14 +84,22 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   R.rcos = 0;
   R.pad = 0;
 
-  // altitude range (gsl_stats_minmax semantics, src/jurassic.c:4223)
+  // altitude range (gsl_stats_minmax semantics, src/jurassic.c:4223) and
+  // monotonicity of the altitude grid
   zmin = zmax = atm->z[0];
+  int zasc = 1, zdesc = 1;
   for (int i = 0; i < atm->np; i++) {
     const double v = atm->z[i];
     if (v < zmin) zmin = v;
     if (v > zmax) zmax = v;
-    if (isnan(v)) { zmin = zmax = v; break; }
+    if (isnan(v)) { zmin = zmax = v; zasc = zdesc = 0; break; }
+    if (i > 0) {
+      if (!(v > atm->z[i - 1])) zasc = 0;
+      if (!(v < atm->z[i - 1])) zdesc = 0;
+    }
   }
+  const int zdir = zasc ? 1 : (zdesc ? -1 : 0);
+  int zg = -1;  // level of the previous LOS point
   if (M.nsf > 0) {
     zmin = (atm->sfz > zmin) ? atm->sfz : zmin;
     if (atm->sfp > 0) {
@@ -151,7 +169,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     }
 
     // intpol_atm at the LOS point
-    const int ipa = locate_irr(atm->z, atm->np, z);
+    const int ipa = atm_level(atm, zdir, z, zg);
     const double z0 = atm->z[ipa], z1 = atm->z[ipa + 1];
     p = logy(z0, atm->p[ipa], z1, atm->p[ipa + 1], z);
     t = lin(z0, atm->t[ipa], z1, atm->t[ipa + 1], z);
@@ -198,13 +216,14 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     if (M.refrac && z <= zrefrac) {
       double zz, lo2, la2, pp, tt;
       for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
+      int zg2 = zg;
       cart2geo(xh, &zz, &lo2, &la2);
-      intpol_pt(atm, zz, &pp, &tt);
+      intpol_pt(atm, zdir, zg2, zz, &pp, &tt);
       n = 7.753e-05 * pp / tt;
       for (int i = 0; i < 3; i++) {
         xh[i] += h;
         cart2geo(xh, &zz, &lo2, &la2);
-        intpol_pt(atm, zz, &pp, &tt);
+        intpol_pt(atm, zdir, zg2, zz, &pp, &tt);
         ngr[i] = (7.753e-05 * pp / tt - n) / h;
         xh[i] -= h;
       }
