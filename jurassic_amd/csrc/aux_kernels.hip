@@ -68,6 +68,19 @@ __global__ void k_reduce_info(const RayInfo *info, int nray, unsigned long long 
   }
 }
 
+// Longest LOS of each block of WAVE rays (the per-chain stride of the
+// emissivity buffer).  One thread per block.
+__global__ void k_block_np(const RayInfo *info, int nray, int blk0, int nb, int *npb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int m = 0;
+  for (int l = 0; l < WAVE; l++) {
+    const int ir = (blk0 + b) * WAVE + l;
+    if (ir < nray && info[ir].status == JUR_OK) m = max(m, info[ir].np);
+  }
+  npb[b] = m;
+}
+
 __global__ void k_fill_u64(unsigned long long *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -150,6 +163,10 @@ void launch_fov(const double *rad0, const double *tau0, double *rad, double *tau
   const int tot = nray * nd;
   hipLaunchKernelGGL(k_fov, dim3((tot + 255) / 256), dim3(256), 0, s, rad0, tau0, rad, tau, nray, nd,
                      ray_geom, pkg_lo, pkg_hi, dz, w, n, err);
+}
+
+void launch_block_np(const RayInfo *info, int nray, int blk0, int nb, int *npb, hipStream_t s) {
+  hipLaunchKernelGGL(k_block_np, dim3((nb + 63) / 64), dim3(64), 0, s, info, nray, blk0, nb, npb);
 }
 
 void launch_finalize(double *rad, int nray, int nd, const ChanInfo *chan, int bbt,

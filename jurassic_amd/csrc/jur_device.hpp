@@ -161,6 +161,98 @@ __device__ __forceinline__ int col_walk(const float4 *col, int n, double x, int 
   return g;
 }
 
+// col_walk with the record at the start index already in hand (rg = col[g],
+// g in [0, n-2]): the common case, x still inside that bracket, loads nothing.
+template <int C>
+__device__ __forceinline__ int col_walk_r(const float4 *col, int n, double x, int g, const float4 &rg,
+                                          float4 &r) {
+  const bool left = g > 0 && (double)lo_of<C>(rg) > x;
+  const bool right = g < n - 2 && (double)hi_of<C>(rg) <= x;
+  if (!left && !right) {
+    r = rg;
+    return g;
+  }
+  return col_walk<C>(col, n, x, g, r);
+}
+
+// intpol_tbl_u (src/jurassic.c:3933-3967) evaluated from the bracket record
+// r that the walk returned for x on a monotone column: x below the first
+// knot leaves the walk at record 0 = (u_0, eps_0, ...), x above the last knot
+// at record n-2 = (..., u_{n-1}, eps_{n-1}).
+__device__ __forceinline__ double u_from_rec(const float4 &r, double x) {
+  JUR_FP_EXACT
+  if (x < (double)r.y) return lin_f(0.f, 0.f, r.y, r.x, x);
+  if (x > (double)r.w) {
+    const double a = log((double)(1.f - r.w)) / (double)r.z;
+    return log(1 - x) / a;
+  }
+  return lin_f(r.y, r.x, r.w, r.z, x);
+}
+
+// intpol_tbl_eps (src/jurassic.c:3895-3929) from a bracket record.
+__device__ __forceinline__ double eps_from_rec(const float4 &r, double x) {
+  JUR_FP_EXACT
+  if (x < (double)r.x) return lin_f(0.f, 0.f, r.x, r.y, x);
+  if (x > (double)r.z) {
+    const double a = log((double)(1.f - r.w)) / (double)r.z;
+    return 1 - exp(a * x);
+  }
+  return lin_f(r.x, r.y, r.z, r.w, x);
+}
+
+// col_walk for a float key (the FP32 path): same end condition, float
+// comparisons.
+template <int C>
+__device__ __forceinline__ int col_walk_f(const float4 *col, int n, float x, int g, float4 &r) {
+  g = g < 0 ? 0 : (g > n - 2 ? n - 2 : g);
+  r = col[g];
+  if (g > 0 && lo_of<C>(r) > x) {
+    int hi = g, step = 1, lo;
+    float4 rl;
+    for (;;) {
+      int j = hi - step;
+      if (j <= 0) { lo = 0; rl = col[0]; break; }
+      float4 rj = col[j];
+      if (lo_of<C>(rj) <= x) { lo = j; rl = rj; break; }
+      hi = j;
+      step <<= 1;
+    }
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      const float4 rm = col[mid];
+      if (lo_of<C>(rm) > x) hi = mid; else { lo = mid; rl = rm; }
+    }
+    r = rl;
+    return lo;
+  }
+  if (g < n - 2 && hi_of<C>(r) <= x) {
+    int lo = g, step = 1, hi;
+    float4 rh;
+    for (;;) {
+      int j = lo + step;
+      if (j >= n - 2) { hi = n - 2; rh = col[n - 2]; break; }
+      float4 rj = col[j];
+      if (hi_of<C>(rj) > x) { hi = j; rh = rj; break; }
+      lo = j;
+      step <<= 1;
+    }
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      const float4 rm = col[mid];
+      if (hi_of<C>(rm) > x) { hi = mid; rh = rm; } else lo = mid;
+    }
+    r = rh;
+    return hi;
+  }
+  return g;
+}
+
+// Fast float reciprocal and division for the FP32 path (v_rcp_f32, 1 ulp):
+// the default HIP float division here is IEEE-correct, which the FP64
+// path's float slopes need, but the FP32 path does not.
+__device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // The reference bisection itself (for columns that are not monotone).
 template <int C>
 __device__ __forceinline__ int col_bisect(const float4 *col, int n, double x, float4 &r) {
@@ -366,9 +458,26 @@ __device__ __forceinline__ double bright(double rad, double nu) {
   return JUR_C2 * nu / log1p(JUR_C1 * (nu * nu * nu) / rad);
 }
 
-// XCD-aware work mapping: consecutive hardware workgroup ids go to the 8
-// XCDs round-robin; remap so that each XCD gets a contiguous range of
-// logical indices (neighbouring channels of one ray block share an L2).
+// XCD-aware work mapping.  The hardware hands workgroup ids to the 8 XCDs
+// round-robin (id % 8).  A ray block's G work items (its channels, or its
+// channel-emitter chains) should share one XCD's L2, since they read the
+// same LOS data; consecutive ray blocks should go to different XCDs, since
+// block cost varies smoothly with geometry (limb ray length grows with
+// depth) and contiguous ranges would unbalance the XCDs.  With the grid padded
+// to nblk8 = 8 ceil(nblk/8) blocks, workgroup wg maps to block
+// 8 (k / G) + x and item k % G, x = wg % 8, k = wg / 8: a bijection in which
+// XCD x owns blocks x, x+8, x+16, ...  Returns false for padding groups.
+__device__ __forceinline__ bool xcd_block_map(int wg, int G, int nblk, int &blk, int &item) {
+  const int x = wg & 7, k = wg >> 3;
+  const int q = k / G;
+  item = k - q * G;
+  blk = q * 8 + x;
+  return blk < nblk;
+}
+
+__host__ __device__ __forceinline__ int padded_blocks(int nblk) { return (nblk + 7) & ~7; }
+
+// Contiguous-range remap (kept for kernels without a per-block structure).
 __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
   constexpr int NX = 8;
   const int per = nwg / NX, rem = nwg % NX;

@@ -50,11 +50,12 @@ struct TabView {
 };
 
 // ------------------------------------------------------------------
-// Line-of-sight buffer written by the ray tracer, read by the radiance
-// kernels.  Per block of WAVE rays: field-major, point-major, lane fastest:
-//   los[((blk * nfield + f) * cap + ip) * WAVE + lane]
+// Line-of-sight buffer written by the ray tracer, read by the lookup and
+// radiance kernels.  Per block of WAVE rays: point-major, then field, lane
+// fastest:
+//   los[((blk * cap + ip) * nfield + f) * WAVE + lane]
 // so that the 64 lanes of a wavefront read one field of one point with a
-// single coalesced 512-byte access.
+// single coalesced 512-byte access, and all fields of a point are adjacent.
 // ------------------------------------------------------------------
 enum {
   F_Z = 0,
@@ -76,14 +77,18 @@ struct LosView {
 };
 
 // Per-segment emissivities of every (channel, emitter), written by the
-// lookup kernels and read by the radiance kernels.  Per block of WAVE rays:
-//   eps[((blk * nd + id) * ng + ig) * cap + ip) * WAVE + lane]
+// lookup kernels and read by the radiance kernels.  Block blk holds npb[blk]
+// points per chain (the longest LOS of its rays), packed from off[blk]:
+//   eps[off[blk] + (((id * ng + ig) * npb[blk] + ip) * WAVE + lane]
 // FP64 path: the reference's eps of intpol_tbl_ega/cga (double).
 // FP32 path: the segment emissivity (float); 1 marks an opaque segment.
 struct EpsView {
   void *eps;
-  int cap;
+  const size_t *off;   // [nblk] element offset of each block (chunk-relative)
+  const int *npb;      // [nblk] points per chain of each block
 };
+
+#define LOS_AT(base, L, f, ip) (base)[((size_t)(ip) * (L).nfield + (size_t)(f)) * WAVE]
 
 // Per-ray results of the ray tracer.
 struct RayInfo {

@@ -49,6 +49,7 @@ void launch_fov(const double *, const double *, double *, double *, int, int, co
                 const int *, const int *, const double *, const double *, int, int *, hipStream_t);
 void launch_finalize(double *, int, int, const ChanInfo *, int, const unsigned char *,
                      hipStream_t);
+void launch_block_np(const RayInfo *, int, int, int, int *, hipStream_t);
 
 // ------------------------------------------------------------------
 // Errors.
@@ -349,6 +350,8 @@ struct jur_ctx {
   // work buffers
   jur::DevBuf<double> d_los;
   jur::DevBuf<char> d_eps;
+  jur::DevBuf<int> d_npb;
+  jur::DevBuf<size_t> d_eoff;
   jur::DevBuf<jur::RayInfo> d_info;
   jur::DevBuf<double> d_zr;
   jur::DevBuf<unsigned long long> d_red;
@@ -665,9 +668,11 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
     }
     int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, budget / per_blk));
     if ((rc = c->d_los.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
-    if ((rc = c->d_eps.reserve((size_t)s.nd * s.ng * L.cap * WAVE * esz * chunk + 1))) return rc;
+    if ((rc = c->d_npb.reserve(chunk))) return rc;
+    if ((rc = c->d_eoff.reserve(chunk))) return rc;
     L.los = c->d_los.p;
-    EpsView E{c->d_eps.p, L.cap};
+    std::vector<int> npb(chunk);
+    std::vector<size_t> eoff(chunk);
     float ms[3] = {0, 0, 0};
     for (int b0 = 0; b0 < nblk; b0 += chunk) {
       const int nb = std::min(chunk, nblk - b0);
@@ -675,6 +680,19 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
       launch_raytrace(M, B, L, c->d_info.p, b0, nb, st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->ev[1], st));
+      // pack the emissivity buffer by the longest LOS of each block
+      launch_block_np(c->d_info.p, B.nray, b0, nb, c->d_npb.p, st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(npb.data(), c->d_npb.p, nb * sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      size_t tot = 0;
+      for (int b = 0; b < nb; b++) {
+        eoff[b] = tot;
+        tot += (size_t)s.nd * s.ng * npb[b] * WAVE;
+      }
+      if ((rc = c->d_eps.reserve(tot * esz + 1))) return rc;
+      HIPCHK(hipMemcpyAsync(c->d_eoff.p, eoff.data(), nb * sizeof(size_t), hipMemcpyHostToDevice, st));
+      EpsView E{c->d_eps.p, c->d_eoff.p, c->d_npb.p};
       if (precision == JUR_FP32) launch_lookup_f32(M, B, L, E, c->d_info.p, b0, nb, st);
       else launch_lookup_f64(M, B, L, E, c->d_info.p, b0, nb, st);
       HIPCHK(hipGetLastError());

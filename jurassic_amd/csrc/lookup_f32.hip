@@ -7,49 +7,46 @@
 //    tau_path makes tau_path exactly 1 - eps00, so eps_path = eps00 carries
 //    no accumulated rounding, and small emissivities keep full relative
 //    precision in float where 1 - tau_path would not.
-//  * The column-density side of the emissivity-growth step -- the inverse
-//    lookup u(eps_path), the sum u + u_seg and the bracket search on u -- is
-//    FP64: u_seg is often 1e-3 of u or less.
+//  * The emissivity-growth step u_eq = u(eps_path), eps00 = eps(u_eq + u_seg)
+//    is evaluated relative to the knot u_i of the inverse lookup's bracket:
+//    off = (eps_path - eps_i) du/deps + u_seg is a small float with no
+//    cancellation against u_i, and eps00 = eps_i + off deps/du while off stays
+//    inside the bracket (the common case; otherwise the forward bracket is
+//    searched on the absolute column density in FP64).
 //  * Output is the segment emissivity (eps00 - eps_path) / (1 - eps_path);
 //    1 marks an opaque segment (tau_path below float resolution).
 #include "jur_device.hpp"
 
 namespace jur {
 
-#define LOSR(f, ip) los[((size_t)(f) * L.cap + (size_t)(ip)) * WAVE]
+#define LOSR(f, ip) LOS_AT(los, L, f, ip)
 
-// Column lookups with float interpolation on a double column-density axis;
-// boundary branches come from the walk's bracket record as in col_eps.
-__device__ __forceinline__ float col_eps_f(const float4 *col, int n, double x, int mono, int &g) {
-  float4 r;
-  if (mono) {
-    g = col_walk<0>(col, n, x, g, r);
-  } else {
-    const float4 r0 = col[0];
-    if (x < (double)r0.x) return __fdividef(r0.y, r0.x) * (float)x;
-    const float4 rl = col[n - 1];
-    if (x > (double)rl.x) return 1.f - __expf(__logf(1.f - rl.y) / rl.x * (float)x);
-    g = col_bisect<0>(col, n, x, r);
-  }
-  if (x < (double)r.x) return __fdividef(r.y, r.x) * (float)x;
-  if (x > (double)r.z) return 1.f - __expf(__logf(1.f - r.w) / r.z * (float)x);
-  return r.y + __fdividef(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
+// intpol_tbl_eps from a bracket record, float interpolation.
+__device__ __forceinline__ float eps_from_rec_f(const float4 &r, double x) {
+  if (x < (double)r.x) return fdiv(r.y, r.x) * (float)x;
+  if (x > (double)r.z) return 1.f - __expf(fdiv(__logf(1.f - r.w), r.z) * (float)x);
+  return r.y + fdiv(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
 }
 
-__device__ __forceinline__ double col_u_f(const float4 *col, int n, float x, int mono, int &g) {
+// Reference-order lookups for columns that are not monotone (bisection).
+__device__ __forceinline__ float col_eps_bis(const float4 *col, int n, double x) {
+  const float4 r0 = col[0];
+  if (x < (double)r0.x) return fdiv(r0.y, r0.x) * (float)x;
+  const float4 rl = col[n - 1];
+  if (x > (double)rl.x) return 1.f - __expf(fdiv(__logf(1.f - rl.y), rl.x) * (float)x);
   float4 r;
-  if (mono) {
-    g = col_walk<1>(col, n, (double)x, g, r);
-  } else {
-    const float4 r0 = col[0];
-    if (x < r0.y) return (double)(__fdividef(r0.x, r0.y) * x);
-    const float4 rl = col[n - 1];
-    if (x > rl.y) return (double)(log1pf(-x) * __fdividef(rl.x, __logf(1.f - rl.y)));
-    g = col_bisect<1>(col, n, (double)x, r);
-  }
-  if (x < r.y) return (double)(__fdividef(r.x, r.y) * x);
-  if (x > r.w) return (double)(log1pf(-x) * __fdividef(r.z, __logf(1.f - r.w)));
-  return (double)r.x + (double)(__fdividef(r.z - r.x, r.w - r.y) * (x - r.y));
+  col_bisect<0>(col, n, x, r);
+  return r.y + fdiv(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
+}
+
+__device__ __forceinline__ double col_u_bis(const float4 *col, int n, float x) {
+  const float4 r0 = col[0];
+  if (x < r0.y) return (double)(fdiv(r0.x, r0.y) * x);
+  const float4 rl = col[n - 1];
+  if (x > rl.y) return (double)(log1pf(-x) * fdiv(rl.x, __logf(1.f - rl.y)));
+  float4 r;
+  col_bisect<1>(col, n, (double)x, r);
+  return (double)r.x + (double)(fdiv(r.z - r.x, r.w - r.y) * (x - r.y));
 }
 
 template <bool CGA>
@@ -57,9 +54,8 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
                                                      EpsView E, const RayInfo *info, int blk0,
                                                      int nblk) {
   const int nd = M.nd, ng = M.ng;
-  const int w = xcd_remap(blockIdx.x, nblk * nd * ng);
-  const int bl = w / (nd * ng);
-  const int rem = w - bl * nd * ng;
+  int bl, rem;
+  if (!xcd_block_map(blockIdx.x, nd * ng, nblk, bl, rem)) return;
   const int id = rem / ng;
   const int ig = rem - id * ng;
   const int lane = threadIdx.x;
@@ -72,8 +68,9 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
   const RayInfo R = info[ir];
   if (R.status != JUR_OK) return;
 
-  const double *los = L.los + (size_t)bl * L.nfield * L.cap * WAVE + lane;
-  float *out = (float *)E.eps + (((size_t)bl * nd + id) * ng + ig) * E.cap * WAVE + lane;
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const int npb = E.npb[bl];
+  float *out = (float *)E.eps + E.off[bl] + ((size_t)id * ng + ig) * npb * WAVE + lane;
   const int mono = T.mono[tix];
   const int pdir = T.pdir[tix];
   const double *pg = T.p + (size_t)tix * T.TP;
@@ -81,10 +78,18 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
 
   float ep = 0.f;
   double cu = 0, cp = 0, ct = 0;
-  int gi[8];
+  int gi[4];
 #pragma unroll
-  for (int c = 0; c < 8; c++) gi[c] = T.TU / 2;
-  int ipr = -1;
+  for (int c = 0; c < 4; c++) gi[c] = T.TU / 2;
+
+  // cached cell
+  int ipr = -1, it0 = -1, it1 = -1;
+  double pa = 0, pb = 0;
+  float wp_scale = 0, g00 = 0, gi0 = 0, g10 = 0, gi1 = 0;
+  float ta0 = 0, dt0 = 0, ta1 = 0, dt1 = 0, lpr = 0;
+  int nt0 = 0, nt1 = 0, n00 = 0, n01 = 0, n10 = 0, n11 = 0;
+  const double *tg0 = nullptr;
+
   int ip = 0;
   for (; ip < R.np; ip++) {
     const double p = LOSR(F_P, ip);
@@ -96,52 +101,113 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       ct += useg * t;
     }
     const double pq = CGA ? cp / cu : p;
-    const double tq = CGA ? ct / cu : t;
-    ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, pq, ipr, pdir) : locate_irr(pg, npt, pq);
-    const int nt0 = ntp[ipr], nt1 = ntp[ipr + 1];
+    const float tq = (float)(CGA ? ct / cu : t);
+    bool keep = false;
+    if (ipr >= 0 && pdir != 0) {
+      if (pdir > 0)
+        keep = (ipr == 0 || !(pa > pq)) && (ipr == npt - 2 || pb > pq);
+      else
+        keep = (ipr == 0 || !(pa <= pq)) && (ipr == npt - 2 || pb <= pq);
+    }
+    if (!keep) {
+      ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, pq, ipr, pdir) : locate_irr(pg, npt, pq);
+      pa = pg[ipr];
+      pb = pg[ipr + 1];
+      wp_scale = (float)(1.0 / (pb - pa));
+      lpr = (float)(1.0 / log(pb / pa));
+      nt0 = ntp[ipr];
+      nt1 = ntp[ipr + 1];
+      tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
+      g00 = (float)tg0[0];
+      gi0 = (float)(1.0 / (tg0[1] - tg0[0]));
+      g10 = (float)tg0[T.TT];
+      gi1 = (float)(1.0 / (tg0[T.TT + 1] - tg0[T.TT]));
+      it0 = it1 = -1;
+    }
     float e = 0.f;
     if (nt0 >= 2 && nt1 >= 2) {
+      int j0 = (int)((tq - g00) * gi0);
+      j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
+      int j1 = (int)((tq - g10) * gi1);
+      j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
       const size_t c0 = (size_t)tix * T.TP + ipr;
-      const double *tg0 = T.t + c0 * T.TT;
-      const double *tg1 = tg0 + T.TT;
-      const int it0 = locate_reg(tg0, nt0, tq);
-      const int it1 = locate_reg(tg1, nt1, tq);
-      const int *nu0 = T.nu + c0 * T.TT;
-      const int *nu1 = nu0 + T.TT;
-      const int n00 = nu0[it0], n01 = nu0[it0 + 1], n10 = nu1[it1], n11 = nu1[it1 + 1];
+      if (j0 != it0) {
+        it0 = j0;
+        ta0 = (float)tg0[it0];
+        dt0 = (float)(1.0 / (tg0[it0 + 1] - tg0[it0]));
+        n00 = T.nu[c0 * T.TT + it0];
+        n01 = T.nu[c0 * T.TT + it0 + 1];
+      }
+      if (j1 != it1) {
+        it1 = j1;
+        ta1 = (float)tg0[T.TT + it1];
+        dt1 = (float)(1.0 / (tg0[T.TT + it1 + 1] - tg0[T.TT + it1]));
+        n10 = T.nu[(c0 + 1) * T.TT + it1];
+        n11 = T.nu[(c0 + 1) * T.TT + it1 + 1];
+      }
       if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
-        const float4 *k00 = T.rec + (c0 * T.TT + it0) * T.TU;
-        const float4 *k01 = k00 + T.TU;
-        const float4 *k10 = T.rec + ((c0 + 1) * T.TT + it1) * T.TU;
-        const float4 *k11 = k10 + T.TU;
-        float e00, e01, e10, e11;
-        if (CGA) {
-          e00 = col_eps_f(k00, n00, cu, mono, gi[4]);
-          e01 = col_eps_f(k01, n01, cu, mono, gi[5]);
-          e10 = col_eps_f(k10, n10, cu, mono, gi[6]);
-          e11 = col_eps_f(k11, n11, cu, mono, gi[7]);
+        const float4 *kc[4];
+        kc[0] = T.rec + (c0 * T.TT + it0) * T.TU;
+        kc[1] = kc[0] + T.TU;
+        kc[2] = T.rec + ((c0 + 1) * T.TT + it1) * T.TU;
+        kc[3] = kc[2] + T.TU;
+        const int nc[4] = {n00, n01, n10, n11};
+        const float us = (float)useg;
+        float ec[4];
+        if (!mono) {
+#pragma unroll 1
+          for (int c = 0; c < 4; c++)
+            ec[c] = CGA ? col_eps_bis(kc[c], nc[c], cu)
+                        : col_eps_bis(kc[c], nc[c], col_u_bis(kc[c], nc[c], ep) + useg);
         } else {
-          e00 = col_eps_f(k00, n00, col_u_f(k00, n00, ep, mono, gi[0]) + useg, mono, gi[4]);
-          e01 = col_eps_f(k01, n01, col_u_f(k01, n01, ep, mono, gi[1]) + useg, mono, gi[5]);
-          e10 = col_eps_f(k10, n10, col_u_f(k10, n10, ep, mono, gi[2]) + useg, mono, gi[6]);
-          e11 = col_eps_f(k11, n11, col_u_f(k11, n11, ep, mono, gi[3]) + useg, mono, gi[7]);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          float4 r;
+          if (CGA) {
+            gi[c] = col_walk<0>(kc[c], nc[c], cu, gi[c], r);
+            ec[c] = eps_from_rec_f(r, cu);
+            continue;
+          }
+          const int i = col_walk_f<1>(kc[c], nc[c], ep, gi[c], r);
+          gi[c] = i;
+          if (ep < r.y) {
+            // below the first knot: u = ep u_0 / eps_0 (small), absolute
+            const double arg = (double)(fdiv(r.x, r.y) * ep) + useg;
+            float4 r2;
+            col_walk_r<0>(kc[c], nc[c], arg, i, r, r2);
+            ec[c] = eps_from_rec_f(r2, arg);
+          } else if (ep > r.w) {
+            // above the last knot: u = log(1 - ep)/a, a = log(1 - eps_last)/u_last
+            const float a = fdiv(__logf(1.f - r.w), r.z);
+            const double arg = (double)fdiv(log1pf(-ep), a) + useg;
+            ec[c] = 1.f - __expf(a * (float)arg);
+          } else {
+            const float du = r.z - r.x, de = r.w - r.y;
+            const float off = (ep - r.y) * fdiv(du, de) + us;
+            if (off < du) {
+              ec[c] = r.y + off * fdiv(de, du);
+            } else {
+              const double arg = (double)r.x + (double)off;
+              float4 r2;
+              col_walk_r<0>(kc[c], nc[c], arg, i, r, r2);
+              ec[c] = eps_from_rec_f(r2, arg);
+            }
+          }
         }
-        const float w0 = (float)((tq - tg0[it0]) / (tg0[it0 + 1] - tg0[it0]));
-        const float w1 = (float)((tq - tg1[it1]) / (tg1[it1 + 1] - tg1[it1]));
-        const float a = e00 + (e01 - e00) * w0;
-        const float b = e10 + (e11 - e10) * w1;
+        }
+        const float w0 = (tq - ta0) * dt0;
+        const float w1 = (tq - ta1) * dt1;
+        const float a = ec[0] + (ec[1] - ec[0]) * w0;
+        const float b = ec[2] + (ec[3] - ec[2]) * w1;
         float wp;
-        if (CGA) {
-          const double x0 = pg[ipr], x1 = pg[ipr + 1];
-          wp = (pq / x0 > 0 && x1 / x0 > 0) ? (float)(log(pq / x0) / log(x1 / x0))
-                                             : (float)((pq - x0) / (x1 - x0));
-        } else {
-          wp = (float)((pq - pg[ipr]) / (pg[ipr + 1] - pg[ipr]));
-        }
+        if (CGA)
+          wp = (pq / pa > 0 && pb / pa > 0) ? __logf((float)(pq / pa)) * lpr : (float)(pq - pa) * wp_scale;
+        else
+          wp = (float)(pq - pa) * wp_scale;
         float en = a + (b - a) * wp;
         en = (en < 1.f) ? en : 1.f;
         en = (en > 0.f) ? en : 0.f;
-        e = (en >= 1.f) ? 1.f : (en - ep) / (1.f - ep);
+        e = (en >= 1.f) ? 1.f : fdiv(en - ep, 1.f - ep);
         ep = en;
       }
     }
@@ -156,8 +222,8 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
 
 void launch_lookup_f32(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
                        const RayInfo *info, int blk0, int nblk, hipStream_t s) {
-  const dim3 grid(nblk * M.nd * M.ng), blk(WAVE);
   if (M.nd * M.ng == 0) return;
+  const dim3 grid(padded_blocks(nblk) * M.nd * M.ng), blk(WAVE);
   if (M.formod == 0)
     hipLaunchKernelGGL(k_lookup_f32<true>, grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
   else

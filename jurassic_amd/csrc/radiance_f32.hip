@@ -16,7 +16,7 @@
 
 namespace jur {
 
-#define LOSR(f, ip) los[((size_t)(f) * L.cap + (size_t)(ip)) * WAVE]
+#define LOSR(f, ip) LOS_AT(los, L, f, ip)
 
 struct KSum {
   float s = 0.f, c = 0.f;
@@ -32,9 +32,8 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
                                                        EpsView E, const RayInfo *info, int blk0,
                                                        int nblk, double *rad_out, double *tau_out) {
   const int nd = M.nd, ng = M.ng;
-  const int w = xcd_remap(blockIdx.x, nblk * nd);
-  const int bl = w / nd;
-  const int id = w - bl * nd;
+  int bl, id;
+  if (!xcd_block_map(blockIdx.x, nd, nblk, bl, id)) return;
   const int lane = threadIdx.x;
   const int ir = (blk0 + bl) * WAVE + lane;
   if (ir >= B.nray) return;
@@ -45,8 +44,9 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   const ChanInfo C = M.chan[id];
   const double nu = C.nu;
   const float nuf = (float)nu;
-  const double *los = L.los + (size_t)bl * L.nfield * L.cap * WAVE + lane;
-  const float *eps_g = (const float *)E.eps + ((size_t)bl * nd + id) * ng * E.cap * WAVE + lane;
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const int npb = E.npb[bl];
+  const float *eps_g = (const float *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
   const double *sr_col = M.sr + (size_t)id * TBLNS;
 
   float kcl = 0.f;
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
     float etot = 0.f;
     for (int g = 0; g < ng; g++) {
       if (!(gmask >> g & 1u)) continue;
-      const float e = eps_g[((size_t)g * E.cap + ip) * WAVE];
+      const float e = eps_g[((size_t)g * npb + ip) * WAVE];
       opaque |= (e >= 1.f);
       etot = etot + e - etot * e;
     }
@@ -200,7 +200,7 @@ void launch_radiance_f32(const ModelView &M, const BatchView &B, const LosView &
                          const RayInfo *info, int blk0, int nblk, double *rad, double *tau,
                          hipStream_t s) {
   if (M.nd == 0) return;
-  hipLaunchKernelGGL(k_radiance_f32, dim3(nblk * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0,
+  hipLaunchKernelGGL(k_radiance_f32, dim3(padded_blocks(nblk) * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0,
                      nblk, rad, tau);
 }
 

@@ -12,16 +12,15 @@
 
 namespace jur {
 
-#define LOSR(f, ip) los[((size_t)(f) * L.cap + (size_t)(ip)) * WAVE]
+#define LOSR(f, ip) LOS_AT(los, L, f, ip)
 
 __global__ __launch_bounds__(WAVE) void k_radiance_f64(ModelView M, BatchView B, LosView L,
                                                        EpsView E, const RayInfo *info, int blk0,
                                                        int nblk, double *rad_out, double *tau_out) {
   JUR_FP_EXACT
   const int nd = M.nd, ng = M.ng;
-  const int w = xcd_remap(blockIdx.x, nblk * nd);
-  const int bl = w / nd;
-  const int id = w - bl * nd;
+  int bl, id;
+  if (!xcd_block_map(blockIdx.x, nd, nblk, bl, id)) return;
   const int lane = threadIdx.x;
   const int ir = (blk0 + bl) * WAVE + lane;
   if (ir >= B.nray) return;
@@ -31,8 +30,9 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f64(ModelView M, BatchView B,
   const atm_t *atm = &B.atm[B.ray_prof[ir]];
   const ChanInfo C = M.chan[id];
   const double nu = C.nu;
-  const double *los = L.los + (size_t)bl * L.nfield * L.cap * WAVE + lane;
-  const double *eps_g = (const double *)E.eps + ((size_t)bl * nd + id) * ng * E.cap * WAVE + lane;
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const int npb = E.npb[bl];
+  const double *eps_g = (const double *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
   const double *sr_col = M.sr + (size_t)id * TBLNS;
 
   double kcl = 0;
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f64(ModelView M, BatchView B,
     // tau_seg = prod_g (1 - eps_g), emitter order (intpol_tbl_ega)
     double tau_gas = 1;
     for (int g = 0; g < ng; g++) {
-      const double eps = (M.tab.np[g * nd + id] < 30) ? 0.0 : eps_g[((size_t)g * E.cap + ip) * WAVE];
+      const double eps = (M.tab.np[g * nd + id] < 30) ? 0.0 : eps_g[((size_t)g * npb + ip) * WAVE];
       tau_gas *= (1 - eps);
     }
 
@@ -96,7 +96,7 @@ void launch_radiance_f64(const ModelView &M, const BatchView &B, const LosView &
                          const RayInfo *info, int blk0, int nblk, double *rad, double *tau,
                          hipStream_t s) {
   if (M.nd == 0) return;
-  hipLaunchKernelGGL(k_radiance_f64, dim3(nblk * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0,
+  hipLaunchKernelGGL(k_radiance_f64, dim3(padded_blocks(nblk) * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0,
                      nblk, rad, tau);
 }
 

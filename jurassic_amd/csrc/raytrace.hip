@@ -46,7 +46,7 @@ __device__ __forceinline__ double sza(double sec, double lon, double lat) {
   return acos(sin(latr) * sin(dec) + cos(latr) * cos(dec) * cos(h)) * (180.0 / JUR_PI);
 }
 
-#define LOSW(f, ip) base[((size_t)(f) * L.cap + (size_t)(ip)) * WAVE]
+#define LOSW(f, ip) LOS_AT(base, L, f, ip)
 
 __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, LosView L,
                                                    RayInfo *info, int blk0) {
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   const int blk = blk0 + blockIdx.x;
   const int ir = blk * WAVE + lane;
   if (ir >= B.nray) return;
-  double *base = L.los + (size_t)blockIdx.x * L.nfield * L.cap * WAVE + lane;
+  double *base = L.los + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane;
 
   const double *g = B.ray_geom + (size_t)7 * ir;
   const int iprof = B.ray_prof[ir];
