@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nprof", type=int, default=NPROF)
     ap.add_argument("--cpu-sample", type=int, default=None,
-                    help="profiles for the CPU baseline (default: 4 per thread)")
+                    help="profiles for the CPU baseline (default: 16 per thread)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
@@ -204,7 +204,7 @@ def main():
     out.update(res)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ns = args.cpu_sample or max(8, 4 * threads)
+        ns = args.cpu_sample or max(16, 16 * threads)
         log("CPU baseline on %d profiles, %d threads" % (ns, threads))
         out["cpu_baseline"] = cpu_baseline(ns)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

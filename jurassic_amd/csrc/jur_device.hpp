@@ -438,6 +438,116 @@ __device__ __forceinline__ double ctmo2(const CtmView &c, double nu, double p, d
   return 0.1 * (r * r) * exp(beta * (1 / tr - 1 / t)) * q_o2 * b;
 }
 
+// The continua split into a per-channel part (everything that depends only
+// on the wavenumber) and a per-segment part, with the same operations as
+// ctmco2/ctmh2o/ctmn2/ctmo2 above: a radiance kernel evaluates the channel
+// part once per lane instead of once per segment.
+struct CtmChan {
+  bool co2, h2o, n2, o2;
+  double co2_296, co2_260, co2_230;
+  double h2o_slf, h2o_ratio, h2o_frn;  // sfac*cw296, cw260/cw296, cwfrn*fscal
+  double n2_b, n2_beta, o2_b, o2_beta;
+};
+
+__device__ __forceinline__ CtmChan ctm_chan(const CtmView &c, double nu) {
+  JUR_FP_EXACT
+  CtmChan k;
+  k.co2 = k.h2o = k.n2 = k.o2 = false;
+  k.co2_296 = k.co2_260 = k.co2_230 = k.h2o_slf = k.h2o_ratio = k.h2o_frn = 0;
+  k.n2_b = k.n2_beta = k.o2_b = k.o2_beta = 0;
+  {
+    const double xw = nu / 2 + 1;
+    if (xw >= 1 && xw < 2001) {
+      const int iw = (int)xw;
+      const double dw = xw - iw;
+      const double ew = 1 - dw;
+      k.co2 = true;
+      k.co2_296 = ew * c.co2_296[iw - 1] + dw * c.co2_296[iw];
+      k.co2_260 = ew * c.co2_260[iw - 1] + dw * c.co2_260[iw];
+      k.co2_230 = ew * c.co2_230[iw - 1] + dw * c.co2_230[iw];
+    }
+  }
+  {
+    const double xw = nu / 10 + 1;
+    if (xw >= 1 && xw < 2001) {
+      const int iw = (int)xw;
+      const double dw = xw - iw;
+      const double ew = 1 - dw;
+      const double cw296 = ew * c.h2o_296[iw - 1] + dw * c.h2o_296[iw];
+      const double cw260 = ew * c.h2o_260[iw - 1] + dw * c.h2o_260[iw];
+      const double cwfrn = ew * c.h2o_frn[iw - 1] + dw * c.h2o_frn[iw];
+      double sfac;
+      if (nu <= 820 || nu >= 960) {
+        sfac = 1;
+      } else {
+        const double xx = (nu - 820) / 10;
+        const int ix = (int)xx;
+        const double dx = xx - ix;
+        sfac = (1 - dx) * c.h2o_xfcrev[ix] + dx * c.h2o_xfcrev[ix + 1];
+      }
+      const double vf2 = (nu - 370) * (nu - 370);
+      const double vf6 = vf2 * vf2 * vf2;
+      const double fscal = 36100 / (vf2 + vf6 * 1e-8 + 36100) * -.25 + 1;
+      k.h2o = true;
+      k.h2o_slf = sfac * cw296;
+      k.h2o_ratio = cw260 / cw296;
+      k.h2o_frn = cwfrn * fscal;
+    }
+  }
+  if (!(nu < c.n2_nu[0] || nu > c.n2_nu[97])) {
+    const int idx = locate_reg(c.n2_nu, 98, nu);
+    k.n2 = true;
+    k.n2_b = lin(c.n2_nu[idx], c.n2_b[idx], c.n2_nu[idx + 1], c.n2_b[idx + 1], nu);
+    k.n2_beta = lin(c.n2_nu[idx], c.n2_beta[idx], c.n2_nu[idx + 1], c.n2_beta[idx + 1], nu);
+  }
+  if (!(nu < c.o2_nu[0] || nu > c.o2_nu[89])) {
+    const int idx = locate_reg(c.o2_nu, 90, nu);
+    k.o2 = true;
+    k.o2_b = lin(c.o2_nu[idx], c.o2_b[idx], c.o2_nu[idx + 1], c.o2_b[idx + 1], nu);
+    k.o2_beta = lin(c.o2_nu[idx], c.o2_beta[idx], c.o2_nu[idx + 1], c.o2_beta[idx + 1], nu);
+  }
+  return k;
+}
+
+__device__ __forceinline__ double ctmco2_seg(const CtmChan &k, double p, double t, double u) {
+  JUR_FP_EXACT
+  if (!k.co2) return 0;
+  const double dt230 = t - 230;
+  const double dt260 = t - 260;
+  const double dt296 = t - 296;
+  const double ctw = dt260 * 5.050505e-4 * dt296 * k.co2_230 - dt230 * 9.259259e-4 * dt296 * k.co2_260 +
+                     dt230 * 4.208754e-4 * dt260 * k.co2_296;
+  return u / JUR_NA / 1000 * p / JUR_P0 * ctw;
+}
+
+__device__ __forceinline__ double ctmh2o_seg(const CtmChan &k, double nu, double p, double t, double q,
+                                             double u) {
+  JUR_FP_EXACT
+  if (!k.h2o) return 0;
+  const double ctwslf = k.h2o_slf * pow(k.h2o_ratio, (296 - t) / (296 - 260));
+  const double a1 = nu * u * tanh(.7193876 / t * nu);
+  const double a2 = 296 / t;
+  const double a3 = p / JUR_P0 * (q * ctwslf + (1 - q) * k.h2o_frn) * 1e-20;
+  return a1 * a2 * a3;
+}
+
+__device__ __forceinline__ double ctmn2_seg(const CtmChan &k, double p, double t) {
+  JUR_FP_EXACT
+  if (!k.n2) return 0;
+  const double q_n2 = 0.79, t0 = 273.0, tr = 296.0;
+  const double r = p / JUR_P0 * t0 / t;
+  return 0.1 * (r * r) * exp(k.n2_beta * (1 / tr - 1 / t)) * q_n2 * k.n2_b *
+         (q_n2 + (1 - q_n2) * (1.294 - 0.4545 * t / tr));
+}
+
+__device__ __forceinline__ double ctmo2_seg(const CtmChan &k, double p, double t) {
+  JUR_FP_EXACT
+  if (!k.o2) return 0;
+  const double q_o2 = 0.21, t0 = 273, tr = 296;
+  const double r = p / JUR_P0 * t0 / t;
+  return 0.1 * (r * r) * exp(k.o2_beta * (1 / tr - 1 / t)) * q_o2 * k.o2_b;
+}
+
 // formod_srcfunc (src/jurassic.c:3483-3496) for one channel column of the
 // source-function table (sr_col[TBLNS]).
 __device__ __forceinline__ double srcfunc(const double *st, const double *sr_col, double t) {

@@ -42,6 +42,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f64(ModelView M, BatchView B,
   if (M.nsf > 0)
     sfeps = lin(M.sfnu[C.isf], atm->sfeps[C.isf], M.sfnu[C.isf + 1], atm->sfeps[C.isf + 1], nu);
   const bool has_k = C.window < M.nw;
+  const CtmChan K = ctm_chan(M.ctm, nu);
 
   double rad = 0, tau = 1, drefl = 0, tau_refl = 1;
   for (int ip = 0; ip < R.np; ip++) {
@@ -59,11 +60,11 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f64(ModelView M, BatchView B,
     // formod_continua (src/jurassic.c:3075-3117)
     double beta = has_k ? LOSR(L.f_k + C.window, ip) : 0.0;
     if (L.f_cl >= 0) beta += LOSR(L.f_cl, ip) * kcl;
-    if (M.ig_co2 >= 0) beta += ctmco2(M.ctm, nu, p, t, LOSR(L.f_u + M.ig_co2, ip)) / ds;
+    if (M.ig_co2 >= 0) beta += ctmco2_seg(K, p, t, LOSR(L.f_u + M.ig_co2, ip)) / ds;
     if (M.ig_h2o >= 0)
-      beta += ctmh2o(M.ctm, nu, p, t, LOSR(F_Q0 + M.ig_h2o, ip), LOSR(L.f_u + M.ig_h2o, ip)) / ds;
-    if (M.ctm_n2) beta += ctmn2(M.ctm, nu, p, t);
-    if (M.ctm_o2) beta += ctmo2(M.ctm, nu, p, t);
+      beta += ctmh2o_seg(K, nu, p, t, LOSR(F_Q0 + M.ig_h2o, ip), LOSR(L.f_u + M.ig_h2o, ip)) / ds;
+    if (M.ctm_n2) beta += ctmn2_seg(K, p, t);
+    if (M.ctm_o2) beta += ctmo2_seg(K, p, t);
 
     const double src = srcfunc(M.st, sr_col, t);
 

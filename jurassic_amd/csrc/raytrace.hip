@@ -19,14 +19,41 @@ __device__ __forceinline__ int atm_level(const atm_t *atm, int zdir, double z, i
   return g;
 }
 
-// intpol_atm (src/jurassic.c:3685-3706) without the per-gas outputs (the
-// LOS point writer reads q and k itself at the final position).
-__device__ __forceinline__ void intpol_pt(const atm_t *atm, int zdir, int &g, double z, double *p,
-                                          double *t) {
+// Pressure and temperature interpolation of intpol_atm
+// (src/jurassic.c:3685-3706) with the level-pair coefficients cached: LOGY
+// evaluates y0 exp(log(y1/y0)/(x1-x0) (x-x0)) and LIN y0 + (y1-y0)/(x1-x0)
+// (x-x0); the quotients depend only on the two levels, so keeping them while
+// the ray stays between the same levels gives bit-identical values.
+struct LevelCache {
+  int ip = -1;
+  double z0, p0, pq, t0, tq;
+  bool plog;
+};
+
+__device__ __forceinline__ void intpol_pt(const atm_t *atm, int zdir, int &g, LevelCache &lc,
+                                          double z, double *p, double *t) {
   JUR_FP_EXACT
   const int ip = atm_level(atm, zdir, z, g);
-  *p = logy(atm->z[ip], atm->p[ip], atm->z[ip + 1], atm->p[ip + 1], z);
-  *t = lin(atm->z[ip], atm->t[ip], atm->z[ip + 1], atm->t[ip + 1], z);
+  if (ip != lc.ip) {
+    lc.ip = ip;
+    const double x0 = atm->z[ip], x1 = atm->z[ip + 1];
+    const double p0 = atm->p[ip], p1 = atm->p[ip + 1];
+    lc.z0 = x0;
+    lc.p0 = p0;
+    lc.plog = (p1 / p0 > 0);
+    lc.pq = lc.plog ? log(p1 / p0) / (x1 - x0) : (p1 - p0) / (x1 - x0);
+    lc.t0 = atm->t[ip];
+    lc.tq = (atm->t[ip + 1] - lc.t0) / (x1 - x0);
+  }
+  *p = lc.plog ? lc.p0 * exp(lc.pq * (z - lc.z0)) : lc.p0 + lc.pq * (z - lc.z0);
+  *t = lc.t0 + lc.tq * (z - lc.z0);
+}
+
+// Altitude of a Cartesian point: the z output of cart2geo
+// (src/jurassic.c:108-119), for the refraction probes, which use nothing else.
+__device__ __forceinline__ double cart2z(const double *x) {
+  JUR_FP_EXACT
+  return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) - JUR_RE;
 }
 
 // sza (src/jurassic.c:5183-5219).
@@ -100,6 +127,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   }
   const int zdir = zasc ? 1 : (zdesc ? -1 : 0);
   int zg = -1;  // level of the previous LOS point
+  LevelCache lc;
   if (M.nsf > 0) {
     zmin = (atm->sfz > zmin) ? atm->sfz : zmin;
     if (atm->sfp > 0) {
@@ -214,16 +242,14 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     for (int i = 0; i < 3; i++) ex1[i] = ex0[i] * n;
 
     if (M.refrac && z <= zrefrac) {
-      double zz, lo2, la2, pp, tt;
+      double pp, tt;
       for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
       int zg2 = zg;
-      cart2geo(xh, &zz, &lo2, &la2);
-      intpol_pt(atm, zdir, zg2, zz, &pp, &tt);
+      intpol_pt(atm, zdir, zg2, lc, cart2z(xh), &pp, &tt);
       n = 7.753e-05 * pp / tt;
       for (int i = 0; i < 3; i++) {
         xh[i] += h;
-        cart2geo(xh, &zz, &lo2, &la2);
-        intpol_pt(atm, zdir, zg2, zz, &pp, &tt);
+        intpol_pt(atm, zdir, zg2, lc, cart2z(xh), &pp, &tt);
         ngr[i] = (7.753e-05 * pp / tt - n) / h;
         xh[i] -= h;
       }
