@@ -70,7 +70,10 @@
 
 /* ------------------------------------------------------------------
    Data model (field order and types as src/jurassic.h:488-775).
+   A caller that already includes the reference's jurassic.h (include guard
+   JURASSIC_H) uses its own definitions of these types, which are the same.
    ------------------------------------------------------------------ */
+#ifndef JURASSIC_H
 
 /* Atmospheric data (src/jurassic.h:488-538). */
 typedef struct {
@@ -156,6 +159,8 @@ typedef struct {
   double tau[ND][NR];
   double rad[ND][NR];
 } obs_t;
+
+#endif /* JURASSIC_H */
 
 #ifdef __cplusplus
 extern "C" {
