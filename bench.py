@@ -13,6 +13,18 @@ geometry), the FP64 path's rate beside it, the radiance kernel's
 algorithmic bytes against the HBM roofline (HIP events on the library's
 stream), and the CPU FP64 oracle timed on this host's cores on a bounded
 sample of the same workload.
+
+Jacobians (the "jacobian" object): nadir sounder, 16,384 atmospheres x 8
+view angles 0..50 deg x 96 channels 650..1600 cm^-1 (CO2, H2O, O3; EGA,
+continua, refraction), state T and the three volume mixing ratios on 81
+levels (n = 324, m = 768 per atmosphere).  One step is one adjoint pass
+(forward model and all K rows, K written to HBM).  Beside it: the FD path
+that kernel() runs, n + 1 = 325 FP64 forward passes per Jacobian (its rate
+is timed as one FP64 forward pass over the same batch / 325), and the CPU
+oracle kernel() (OpenMP over the 324 columns) on a sample of atmospheres.
+The FP32 adjoint's error is stated against the FP64 adjoint on a subset:
+relative L2 per state block and the brightness temperature of the forward
+part.
 """
 import argparse
 import ctypes as C
@@ -83,6 +95,161 @@ def cpu_baseline(nsample):
                 % (nsample, len(zt), ctl.nd, nrad, dt))
 
 
+NJAC_PROF, NJAC_ANG, NJAC_CH = 16384, 8, 96
+
+
+def build_jac_workload(nprof):
+    """Nadir Jacobian workload: ray order view angle major, atmosphere minor."""
+    import numpy as np
+    from jurassic_amd import abi, scenes
+    ctl = scenes.nadir_jac_ctl(NJAC_CH)
+    geom = scenes.nadir_geometry(705.0, np.arange(NJAC_ANG) * 50.0 / 7.0)
+    atms = (abi.atm_t * nprof)()
+    for k in range(nprof):
+        atms[k] = scenes.make_profile_atm(GASES, iprof=k)
+    nray = nprof * NJAC_ANG
+    prof = np.empty(nray, dtype=np.int32)
+    g = np.zeros((nray, 7))
+    for j in range(NJAC_ANG):
+        sl = slice(j * nprof, (j + 1) * nprof)
+        prof[sl] = np.arange(nprof)
+        for c, key in enumerate(("obsz", "obslon", "obslat", "vpz", "vplon", "vplat")):
+            g[sl, 1 + c] = geom[key][j]
+    return ctl, atms, prof, g
+
+
+def cpu_kernel_baseline(nsample):
+    """The oracle's kernel() (one-sided FD, OpenMP over the state columns, as
+    the reference) on nsample atmospheres of the Jacobian workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from jurassic_amd import abi, scenes
+    import numpy as np
+    ctl = scenes.nadir_jac_ctl(NJAC_CH)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    obs = scenes.make_obs(scenes.nadir_geometry(705.0, np.arange(NJAC_ANG) * 50.0 / 7.0))
+    O.formod(ctl, scenes.make_profile_atm(GASES, iprof=0), abi.copy_struct(obs))  # warm
+    t0 = time.perf_counter()
+    for k in range(nsample):
+        O.kernel(ctl, scenes.make_profile_atm(GASES, iprof=k), abi.copy_struct(obs), nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=nsample / dt, unit="Jacobians/s", cores=threads, kind="port",
+                sample="%d atmospheres x 768 x 324 one-sided FD (325 FP64 oracle formods each), %.1f s"
+                % (nsample, dt))
+
+
+def jacobian_bench(args, torch, gpu, abi):
+    import numpy as np
+    nprof = args.jac_nprof
+    ctl, atms, prof, geom = build_jac_workload(nprof)
+    nray, nd = len(prof), ctl.nd
+    xm, n = gpu.state_map(ctl, atms[0])
+    m = NJAC_ANG * nd
+    log("Jacobian workload: %d atmospheres, %d rays, %d channels, n = %d, K %.1f GB"
+        % (nprof, nray, nd, n, nray * nd * n * 8 / 1e9))
+    ctx = gpu.Context(ctl)
+    dev = torch.device("cuda:0")
+    atm_d = torch.frombuffer(bytearray(bytes(atms)), dtype=torch.uint8).to(dev)
+    prof_d = torch.from_numpy(prof).to(dev)
+    geom_d = torch.from_numpy(np.ascontiguousarray(geom)).to(dev)
+    pmap_d = torch.zeros(nprof, dtype=torch.int32, device=dev)
+    rad_d = torch.empty(nray * nd, dtype=torch.float64, device=dev)
+    tau_d = torch.empty(nray * nd, dtype=torch.float64, device=dev)
+    k_d = torch.empty(nray * nd * n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def adj(precision, natm=nprof, nr=nray, prof_ptr=None, geom_ptr=None, rad=rad_d, tau=tau_d, kk=k_d):
+        ctx.jacobian_device(natm, atm_d.data_ptr(), nr, prof_ptr or prof_d.data_ptr(),
+                            geom_ptr or geom_d.data_ptr(), rad.data_ptr(), tau.data_ptr(), xm[None, :],
+                            pmap_d.data_ptr(), n, kk.data_ptr(), precision=precision)
+
+    steps = max(1, args.jac_steps)
+    adj(abi.JUR_FP32)
+    torch.cuda.synchronize()
+    kms = [0.0, 0.0, 0.0]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        adj(abi.JUR_FP32)
+        kms = [a + b for a, b in zip(kms, ctx.last_timing())]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    kms = [k / steps for k in kms]
+    mean_np = ctx.last_mean_np()
+    jac_rate = nprof / dt
+    log("adjoint FP32: %.1f ms/step (%.0f Jacobians/s); kernels: ray tracing %.1f, lookups %.1f, "
+        "reverse sweep + K store %.1f ms; mean LOS points %.1f" % (dt * 1e3, jac_rate, kms[0], kms[1], kms[2], mean_np))
+
+    # the FD path: 325 FP64 forward passes per Jacobian
+    def fwd(precision):
+        ctx.formod_device(nprof, atm_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
+                          rad_d.data_ptr(), tau_d.data_ptr(), precision=precision)
+    fwd(abi.JUR_FP64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd(abi.JUR_FP64)
+    torch.cuda.synchronize()
+    dt64 = time.perf_counter() - t0
+    fd_rate = nprof / (dt64 * (n + 1))
+    log("FP64 forward pass over the batch: %.1f ms -> FD Jacobians/s %.1f" % (dt64 * 1e3, fd_rate))
+
+    # FP32 adjoint against FP64 adjoint on the first atmospheres of every angle
+    nsub = min(args.jac_check, nprof)
+    sel = np.concatenate([np.arange(j * nprof, j * nprof + nsub) for j in range(NJAC_ANG)])
+    prof_s = torch.from_numpy(prof[sel].copy()).to(dev)
+    geom_s = torch.from_numpy(np.ascontiguousarray(geom[sel])).to(dev)
+    ns = len(sel)
+    outs = {}
+    for p in (abi.JUR_FP32, abi.JUR_FP64):
+        r = torch.empty(ns * nd, dtype=torch.float64, device=dev)
+        t = torch.empty_like(r)
+        kk = torch.empty(ns * nd * n, dtype=torch.float64, device=dev)
+        adj(p, nsub, ns, prof_s.data_ptr(), geom_s.data_ptr(), r, t, kk)
+        torch.cuda.synchronize()
+        outs[p] = (r.cpu().numpy().reshape(ns, nd), kk.cpu().numpy().reshape(ns * nd, n))
+    nu = np.array([ctl.nu[i] for i in range(nd)])
+    c1, c2 = 1.19104259e-8, 1.43877506
+
+    def bt(r):
+        return c2 * nu / np.log1p(c1 * nu ** 3 / r)
+    bt_err = float(np.abs(bt(outs[abi.JUR_FP32][0]) - bt(outs[abi.JUR_FP64][0])).max())
+    k32, k64 = outs[abi.JUR_FP32][1], outs[abi.JUR_FP64][1]
+    blocks = {}
+    for b, name in enumerate(["T"] + GASES):
+        s_ = slice(b * 81, (b + 1) * 81)
+        blocks[name] = float(np.linalg.norm(k32[:, s_] - k64[:, s_]) / np.linalg.norm(k64[:, s_]))
+    log("FP32 vs FP64 adjoint on %d atmospheres: BT %.2e K, K rel L2 %s" % (nsub, bt_err, blocks))
+    ctx.close()
+    del k_d
+
+    # algorithmic bytes per Jacobian: forward lookups plus the recompute sweep
+    # (176 B per segment, channel, emitter; 16 B Planck bracket per segment
+    # and channel) and the K write
+    bytes_per_jac = 2 * m * mean_np * (len(GASES) * 176 + 16) + m * n * 4
+    out = {
+        "value": jac_rate, "unit": "Jacobians/s", "ms_per_step": dt * 1e3, "steps": steps,
+        "workload": "nadir sounder: %d atmospheres x %d view angles x %d channels, n = %d, m = %d "
+                    "(CO2, H2O, O3; EGA; continua and refraction on; LOS held fixed)"
+                    % (nprof, NJAC_ANG, nd, n, m),
+        "precision": "FP32 lookups, FP64 geometry and adjoint accumulation",
+        "mean_los_points": mean_np,
+        "kernel_ms": {"raytrace": kms[0], "lookup": kms[1], "reverse_sweep_and_store": kms[2]},
+        "fd_fp64_gpu": {"value": fd_rate, "unit": "Jacobians/s",
+                        "method": "n + 1 = %d FP64 forward passes per Jacobian (kernel())" % (n + 1),
+                        "forward_ms_batch": dt64 * 1e3},
+        "adjoint_over_fd_fp64_gpu": jac_rate / fd_rate,
+        "fp32_vs_fp64": {"atmospheres": nsub, "bt_max_abs_K": bt_err, "k_rel_l2": blocks},
+        "roofline": {"bound": "hbm", "achieved": bytes_per_jac * jac_rate / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": bytes_per_jac * jac_rate / HBM_PEAK,
+                     "bytes_per_jacobian": bytes_per_jac},
+    }
+    if not args.no_cpu:
+        ns_cpu = args.jac_cpu_sample
+        log("CPU kernel() baseline on %d atmospheres" % ns_cpu)
+        out["cpu_baseline"] = cpu_kernel_baseline(ns_cpu)
+        out["adjoint_over_cpu"] = jac_rate / out["cpu_baseline"]["value"]
+    return out
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -97,6 +264,13 @@ def main():
                     help="profiles for the CPU baseline (default: 16 per thread)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-jac", action="store_true")
+    ap.add_argument("--jac-only", action="store_true", help="skip the forward-model section")
+    ap.add_argument("--jac-nprof", type=int, default=NJAC_PROF)
+    ap.add_argument("--jac-steps", type=int, default=2)
+    ap.add_argument("--jac-check", type=int, default=64,
+                    help="atmospheres per angle for the FP32-vs-FP64 adjoint check")
+    ap.add_argument("--jac-cpu-sample", type=int, default=2)
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -117,6 +291,9 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written")
         del src, dst
+    if args.jac_only:
+        print(json.dumps({"jacobian": jacobian_bench(args, torch, gpu, abi)}))
+        return
     ctl, atms, prof, geom = build_workload(args.nprof)
     nprof, nray, nd = args.nprof, len(prof), ctl.nd
     log("workload built: %d atmospheres, %d rays, %d channels" % (nprof, nray, nd))
@@ -208,6 +385,10 @@ def main():
         log("CPU baseline on %d profiles, %d threads" % (ns, threads))
         out["cpu_baseline"] = cpu_baseline(ns)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    ctx.close()
+    del rad_d, tau_d, atm_d
+    if not args.no_jac:
+        out["jacobian"] = jacobian_bench(args, torch, gpu, abi)
     print(json.dumps(out))
 
 

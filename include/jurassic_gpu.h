@@ -185,6 +185,22 @@ extern "C" {
 #define JUR_JAC_FD 0          /* one-sided finite differences (kernel()) */
 #define JUR_JAC_ADJOINT 1     /* reverse sweep, path held fixed */
 
+/* State map of the device Jacobian: for one atmosphere, the column of K that
+   each state element occupies, -1 if it is not part of the state.  Level
+   quantities iq: 0 pressure, 1 temperature, 2+ig volume mixing ratio of
+   emitter ig, 2+NG+iw extinction of window iw; then the scalars in the
+   order of atm2x (src/jurassic.c:29-83). */
+#define JUR_XMAP_NQ (2 + NG + NW)
+#define JUR_XMAP_LEV(iq, ip) ((iq) * NP + (ip))
+#define JUR_XMAP_CLZ (JUR_XMAP_NQ * NP)
+#define JUR_XMAP_CLDZ (JUR_XMAP_CLZ + 1)
+#define JUR_XMAP_CLK(icl) (JUR_XMAP_CLZ + 2 + (icl))
+#define JUR_XMAP_SFZ (JUR_XMAP_CLZ + 2 + NCL)
+#define JUR_XMAP_SFP (JUR_XMAP_SFZ + 1)
+#define JUR_XMAP_SFT (JUR_XMAP_SFZ + 2)
+#define JUR_XMAP_SFEPS(isf) (JUR_XMAP_SFZ + 3 + (isf))
+#define JUR_XMAP_LEN JUR_XMAP_SFEPS(NSF)
+
 /* Text of the last error of the calling thread ("" if none). */
 const char *jur_gpu_last_error(void);
 
@@ -221,6 +237,10 @@ int jur_gpu_formod_batch(const ctl_t *ctl, atm_t *atm, obs_t *obs, int n,
    column, all columns in one device pass. */
 int jur_gpu_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
                    size_t m, size_t n, int method, int precision);
+
+/* The state map of atm2x (src/jurassic.c:29-83) for ctl and atm: fills
+   xmap[JUR_XMAP_LEN] and returns the state size n. */
+int jur_state_map(const ctl_t *ctl, const atm_t *atm, int *xmap);
 
 /* ERRMSG-style wrapper of jur_gpu_kernel(..., JUR_JAC_FD, JUR_FP64). */
 void kernel_gpu(ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m,
@@ -264,10 +284,30 @@ typedef struct {
 int jur_ctx_formod_device(jur_ctx_t *ctx, const jur_batch_t *batch,
                           int precision, void *hip_stream);
 
+/* Jacobian of a device batch by the adjoint (path held fixed).
+   K row (ray, channel) holds d rad / d x (d BT / d x with WRITE_BBT) over the
+   ncol columns of the ray's state map; masked radiances give NaN rows.
+   The batch's rad/tau receive the forward model of the same pass.
+   Surface altitude and pressure only move the path: their map entries must
+   be -1 (jur_gpu_kernel covers them by finite differences). */
+typedef struct {
+  int nmap;                   /* number of state maps */
+  const int *xmap;            /* host: [nmap][JUR_XMAP_LEN] */
+  const int *prof_map;        /* device: [nprof] map of each atmosphere (<0: none) */
+  int ncol;                   /* row length of K */
+  double *k;                  /* device out: [nray][nd][ncol] */
+} jur_jac_t;
+
+int jur_ctx_jacobian_device(jur_ctx_t *ctx, const jur_batch_t *batch,
+                            const jur_jac_t *jac, int precision,
+                            void *hip_stream);
+
 /* Kernel timing of the last jur_ctx_formod_device call on this context, in
    milliseconds, measured with HIP events on the stream the kernels ran on:
    [0] ray tracing, [1] emissivity-table lookups, [2] radiance accumulation
-   (continua, source function, surface terms).  n <= 3 entries are written. */
+   (continua, source function, surface terms), or for
+   jur_ctx_jacobian_device the reverse sweep and the K store.
+   n <= 3 entries are written. */
 int jur_ctx_last_timing(const jur_ctx_t *ctx, double *ms, int n);
 
 /* Mean number of LOS points per ray of the last pass. */

@@ -20,6 +20,29 @@ JUR_FP64 = 0
 JUR_FP32 = 1
 JUR_JAC_FD = 0
 JUR_JAC_ADJOINT = 1
+JUR_ERR_ARG = 1
+
+# State map layout of the device Jacobian (JUR_XMAP_* in the C header).
+XMAP_NQ = 2 + NG + NW
+XMAP_CLZ = XMAP_NQ * NP
+XMAP_CLDZ = XMAP_CLZ + 1
+XMAP_SFZ = XMAP_CLZ + 2 + NCL
+XMAP_SFP = XMAP_SFZ + 1
+XMAP_SFT = XMAP_SFZ + 2
+XMAP_LEN = XMAP_SFZ + 3 + NSF
+
+
+def xmap_lev(iq, ip):
+    """Slot of level quantity iq (0 p, 1 T, 2+ig q, 2+NG+iw k) at level ip."""
+    return iq * NP + ip
+
+
+def xmap_clk(icl):
+    return XMAP_CLZ + 2 + icl
+
+
+def xmap_sfeps(isf):
+    return XMAP_SFZ + 3 + isf
 
 ERR_NAMES = {
     0: "JUR_OK", 1: "JUR_ERR_ARG", 2: "JUR_ERR_IO", 3: "JUR_ERR_LOS",

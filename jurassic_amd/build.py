@@ -35,6 +35,7 @@ SOURCES = [
     ("radiance_f64.hip", ["-ffp-contract=off"]),
     ("radiance_f32.hip", ["-ffp-contract=fast"]),
     ("aux_kernels.hip", ["-ffp-contract=off"]),
+    ("adjoint.hip", ["-ffp-contract=fast"]),
 ]
 
 

@@ -1,0 +1,904 @@
+// Jacobian of the forward model by a reverse sweep: a hand-written adjoint
+// of formod_pencil (src/jurassic.c:3196-3338) with the line of sight held
+// fixed.  It replaces the n+1 finite-difference formod calls of kernel()
+// (src/jurassic.c:4004-4099).
+//
+// Per ray and channel, with LOS points i = 0..N-1 (N: points before the first
+// segment whose gas transmittance is 0, which the reference skips together
+// with every later one):
+//   P_g,i   path transmittance of emitter g after point i (P_g,-1 = 1);
+//           e_g,i = 1 - P_g,i is the path emissivity that the lookup kernels
+//           store, and  e_g,i = E(e_g,i-1, p_i, T_i, u_g,i)  (EGA) or
+//           E(U_g,i, sum(u p)/U, sum(u T)/U)  (CGA, U = sum of u up to i)
+//   s_i     segment transmittance  prod_g (P_g,i / P_g,i-1) exp(-beta_i ds_i)
+//   tau_i   observer-to-point transmittance  prod_{k<i} s_k
+//   D_i     down-welling sum of the reflection term,
+//           D_{i+1} = D_i s_i + src_i (1 - s_i)
+//   rad     = sum_i src_i (1 - s_i) tau_i + surface terms in (tau_N, D_N)
+// The reverse sweep runs i = N-1..0 with tb = d rad / d tau_{i+1} and
+// db = d rad / d D_{i+1} (seeded by the surface terms):
+//   G_i = d rad / d s_i = tau_i (tb - src_i) + db (D_i - src_i)
+//   d rad / d src_i = (1 - s_i)(tau_i + db)
+//   tb <- src_i (1 - s_i) + tb s_i,   db <- db s_i
+//   H_i = G_i s_i,   d rad / d beta_i = -H_i ds_i
+//   d rad / d e_g,i (through s_i and s_i+1) = (H_i+1 - H_i) / P_g,i
+//   L_g,i = that + L_g,i+1 dE_g,i+1 / d e_g,i         (EGA chain)
+// The table partials dE/d(e_prev, p, T, u) are recomputed from the stored
+// path emissivities with the slopes of the brackets that the lookup used
+// (piecewise-linear in u, linear in T, linear (EGA) or logarithmic (CGA) in
+// p; zero where eps00 is clamped).  Continua and the source function are
+// differentiated analytically.
+//
+// Point adjoints go to atmospheric levels through intpol_atm's weights
+// (src/jurassic.c:3685-3706: LOGY in p, LIN in T, q, k) and the column
+// densities u = 10 q p / (kB T) ds (src/jurassic.c:4417-4419).
+//
+// k_adj_acc, one wavefront per (block of 64 rays, channel), lane = ray: a
+// forward sweep (tau_i, D_i) and the reverse sweep of the accumulation (H_i,
+// kept in place of tau_i).  k_adj_chain, one wavefront per (block, channel,
+// emitter) like the lookup kernels: the emitter's chain in reverse with its
+// table partials recomputed.  A lane sums the contributions of the current
+// level pair in registers and adds them to its row of the block tile
+// kt[(block, channel)][column][lane] when the pair changes, so tile accesses
+// of a wavefront are contiguous; T and p columns, which every emitter chain
+// of a row updates, take atomic adds.  k_jac_store
+// then writes the tile into row-major K, scaled by d BT / d rad when
+// brightness temperatures are written.
+//
+// Not differentiated (the path is fixed): refraction's dependence on p and T
+// and the surface altitude/pressure (sfz, sfp), which only move the path.
+#include "jur_device.hpp"
+
+namespace jur {
+
+#define LOSR(f, ip) LOS_AT(los, L, f, ip)
+
+// The slopes below are those of the arithmetic each forward path runs, not
+// of the exact piecewise-linear table: the FP64 path interpolates with LIN
+// on float table entries, whose slope C forms in float (lin_f), and the FP32
+// path with reciprocal-multiply float slopes.  Within one bracket a chain
+// step then has d e_new / d e_prev = s_eps s_u = 1 + O(1e-7) rather than 1,
+// which compounds over the ~1000 steps of a chain; the adjoint follows the
+// forward model it differentiates.
+
+// FP64 path: intpol_tbl_eps at x from its u-bracket record (eps_from_rec):
+// value e and slope de/du.
+__device__ __forceinline__ double eps_vs(const float4 &r, double x, double &e) {
+  if (x < (double)r.x) {
+    const double s = (double)(r.y / r.x);
+    e = s * x;
+    return s;
+  }
+  if (x > (double)r.z) {
+    const double a = log((double)(1.f - r.w)) / (double)r.z;
+    const double ex = exp(a * x);
+    e = 1 - ex;
+    return ex > 0 ? -a * ex : 0.0;
+  }
+  const double s = (double)((r.w - r.y) / (r.z - r.x));
+  e = (double)r.y + s * (x - (double)r.x);
+  return s;
+}
+
+// FP64 path: intpol_tbl_u at x from its eps-bracket record (u_from_rec):
+// value u and slope du/deps.
+__device__ __forceinline__ double u_vs(const float4 &r, double x, double &u) {
+  if (x < (double)r.y) {
+    const double s = (double)(r.x / r.y);
+    u = s * x;
+    return s;
+  }
+  if (x > (double)r.w) {
+    const double a = log((double)(1.f - r.w)) / (double)r.z;
+    u = log1p(-x) / a;
+    return -1.0 / ((1.0 - x) * a);
+  }
+  const double s = (double)((r.z - r.x) / (r.w - r.y));
+  u = (double)r.x + s * (x - (double)r.y);
+  return s;
+}
+
+// FP32 path: eps_from_rec_f with its slope.
+__device__ __forceinline__ float eps_rec_f_vs(const float4 &r, double x, float &s) {
+  if (x < (double)r.x) {
+    s = fdiv(r.y, r.x);
+    return s * (float)x;
+  }
+  if (x > (double)r.z) {
+    const float a = fdiv(__logf(1.f - r.w), r.z);
+    const float ex = __expf(a * (float)x);
+    s = ex > 0.f ? -a * ex : 0.f;
+    return 1.f - ex;
+  }
+  s = fdiv(r.w - r.y, r.z - r.x);
+  return r.y + s * (float)(x - (double)r.x);
+}
+
+// FP32 path: col_eps_bis and col_u_bis with their slopes.
+__device__ __forceinline__ float col_eps_bis_vs(const float4 *col, int n, double x, float &s) {
+  const float4 r0 = col[0];
+  if (x < (double)r0.x) {
+    s = fdiv(r0.y, r0.x);
+    return s * (float)x;
+  }
+  const float4 rl = col[n - 1];
+  if (x > (double)rl.x) {
+    const float a = fdiv(__logf(1.f - rl.y), rl.x);
+    const float ex = __expf(a * (float)x);
+    s = ex > 0.f ? -a * ex : 0.f;
+    return 1.f - ex;
+  }
+  float4 r;
+  col_bisect<0>(col, n, x, r);
+  s = fdiv(r.w - r.y, r.z - r.x);
+  return r.y + s * (float)(x - (double)r.x);
+}
+
+__device__ __forceinline__ double col_u_bis_vs(const float4 *col, int n, float x, float &s) {
+  const float4 r0 = col[0];
+  if (x < r0.y) {
+    s = fdiv(r0.x, r0.y);
+    return (double)(s * x);
+  }
+  const float4 rl = col[n - 1];
+  if (x > rl.y) {
+    const float c = fdiv(rl.x, __logf(1.f - rl.y));
+    s = -c / (1.f - x);
+    return (double)(log1pf(-x) * c);
+  }
+  float4 r;
+  col_bisect<1>(col, n, (double)x, r);
+  s = fdiv(r.z - r.x, r.w - r.y);
+  return (double)r.x + (double)(s * (x - r.y));
+}
+
+// EGA corner: eps(u(e_prev) + u_seg) with d/d e_prev and d/d u_seg.
+__device__ __forceinline__ void ega_corner(const float4 *col, int n, int mono, double ep, double useg,
+                                           int &g, double &e, double &dep, double &du) {
+  float4 r, r2;
+  int i;
+  if (mono) i = col_walk<1>(col, n, ep, g, r);
+  else i = col_bisect<1>(col, n, ep, r);
+  double u;
+  const double dudx = u_vs(r, ep, u);
+  const double arg = u + useg;
+  if (mono) col_walk_r<0>(col, n, arg, i, r, r2);
+  else col_bisect<0>(col, n, arg, r2);
+  du = eps_vs(r2, arg, e);
+  dep = du * dudx;
+  g = i;
+}
+
+// CGA corner: eps(U) and d/dU.
+__device__ __forceinline__ void cga_corner(const float4 *col, int n, int mono, double cu, int &g,
+                                           double &e, double &du) {
+  float4 r;
+  g = mono ? col_walk<0>(col, n, cu, g, r) : col_bisect<0>(col, n, cu, r);
+  du = eps_vs(r, cu, e);
+}
+
+// Partials of one chain step e_new = E(e_prev, pq, tq, x) (x = u_seg for
+// EGA, the column sum U for CGA).  A cell without data (nt or nu < 2) gives
+// eps = 0, i.e. e_new = e_prev.
+struct LkPart {
+  double dep, dp, dt, du;
+};
+
+template <bool CGA>
+__device__ __forceinline__ LkPart lookup_partials(const TabView &T, int tix, int npt, int mono, int pdir,
+                                                  double ep, double pq, double tq, double x, int &ipr,
+                                                  int *gi) {
+  LkPart d{0, 0, 0, 0};
+  const double *pg = T.p + (size_t)tix * T.TP;
+  ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, pq, ipr, pdir) : locate_irr(pg, npt, pq);
+  const double pa = pg[ipr], pb = pg[ipr + 1];
+  const int *ntp = T.nt + (size_t)tix * T.TP;
+  const int nt0 = ntp[ipr], nt1 = ntp[ipr + 1];
+  if (nt0 < 2 || nt1 < 2) {
+    d.dep = 1;
+    return d;
+  }
+  const double *tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
+  int j0 = (int)((tq - tg0[0]) / (tg0[1] - tg0[0]));
+  j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
+  int j1 = (int)((tq - tg0[T.TT]) / (tg0[T.TT + 1] - tg0[T.TT]));
+  j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
+  const size_t c0 = (size_t)tix * T.TP + ipr;
+  int nc[4];
+  nc[0] = T.nu[c0 * T.TT + j0];
+  nc[1] = T.nu[c0 * T.TT + j0 + 1];
+  nc[2] = T.nu[(c0 + 1) * T.TT + j1];
+  nc[3] = T.nu[(c0 + 1) * T.TT + j1 + 1];
+  if (nc[0] < 2 || nc[1] < 2 || nc[2] < 2 || nc[3] < 2) {
+    d.dep = 1;
+    return d;
+  }
+  // eps00 = sum_c cw_c e_c is linear in the corner values: every partial is a
+  // weighted sum over the corners, accumulated corner by corner
+  const double ta0 = tg0[j0], tb0 = tg0[j0 + 1];
+  const double ta1 = tg0[T.TT + j1], tb1 = tg0[T.TT + j1 + 1];
+  const double s0 = 1 / (tb0 - ta0), s1 = 1 / (tb1 - ta1);
+  const double w0 = (tq - ta0) * s0, w1 = (tq - ta1) * s1;
+  double wp, dwp;
+  if (CGA && pq / pa > 0 && pb / pa > 0) {
+    const double l = 1 / log(pb / pa);
+    wp = log(pq / pa) * l;
+    dwp = l / pq;
+  } else {
+    dwp = 1 / (pb - pa);
+    wp = (pq - pa) * dwp;
+  }
+  const double q0 = 1 - wp, q1 = wp;
+  const double cw[4] = {q0 * (1 - w0), q0 * w0, q1 * (1 - w1), q1 * w1};  // value
+  const double tw[4] = {-q0 * s0, q0 * s0, -q1 * s1, q1 * s1};            // d/dT
+  const double pw[4] = {-(1 - w0) * dwp, -w0 * dwp, (1 - w1) * dwp, w1 * dwp};  // d/dp
+  const float4 *base0 = T.rec + (c0 * T.TT + j0) * T.TU;
+  const float4 *base1 = T.rec + ((c0 + 1) * T.TT + j1) * T.TU;
+  double en = 0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const float4 *col = (c < 2 ? base0 : base1) + (c & 1) * T.TU;
+    double e, de, du;
+    if (CGA) {
+      cga_corner(col, nc[c], mono, x, gi[c], e, du);
+      de = 0;
+    } else {
+      ega_corner(col, nc[c], mono, ep, x, gi[c], e, de, du);
+    }
+    en += cw[c] * e;
+    d.dep += cw[c] * de;
+    d.du += cw[c] * du;
+    d.dt += tw[c] * e;
+    d.dp += pw[c] * e;
+  }
+  if (en < 0 || en > 1) return LkPart{0, 0, 0, 0};  // MAX(MIN(eps00, 1), 0) holds it constant
+  return d;
+}
+
+// FP32 path: partials of one chain step of k_lookup_f32, with its float
+// cell weights, its knot-relative update inside a bracket and its
+// reciprocal-multiply slopes.
+template <bool CGA>
+__device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix, int npt, int mono, int pdir,
+                                                      float ep, double pq, float tq, double x, int &ipr,
+                                                      int *gi) {
+  LkPart d{0, 0, 0, 0};
+  const double *pg = T.p + (size_t)tix * T.TP;
+  ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, pq, ipr, pdir) : locate_irr(pg, npt, pq);
+  const double pa = pg[ipr], pb = pg[ipr + 1];
+  const int *ntp = T.nt + (size_t)tix * T.TP;
+  const int nt0 = ntp[ipr], nt1 = ntp[ipr + 1];
+  if (nt0 < 2 || nt1 < 2) {
+    d.dep = 1;
+    return d;
+  }
+  const double *tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
+  int j0 = (int)((tq - (float)tg0[0]) * (float)(1.0 / (tg0[1] - tg0[0])));
+  j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
+  int j1 = (int)((tq - (float)tg0[T.TT]) * (float)(1.0 / (tg0[T.TT + 1] - tg0[T.TT])));
+  j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
+  const size_t c0 = (size_t)tix * T.TP + ipr;
+  int nc[4];
+  nc[0] = T.nu[c0 * T.TT + j0];
+  nc[1] = T.nu[c0 * T.TT + j0 + 1];
+  nc[2] = T.nu[(c0 + 1) * T.TT + j1];
+  nc[3] = T.nu[(c0 + 1) * T.TT + j1 + 1];
+  if (nc[0] < 2 || nc[1] < 2 || nc[2] < 2 || nc[3] < 2) {
+    d.dep = 1;
+    return d;
+  }
+  const float ta0 = (float)tg0[j0], dt0 = (float)(1.0 / (tg0[j0 + 1] - tg0[j0]));
+  const float ta1 = (float)tg0[T.TT + j1], dt1 = (float)(1.0 / (tg0[T.TT + j1 + 1] - tg0[T.TT + j1]));
+  const float w0 = (tq - ta0) * dt0, w1 = (tq - ta1) * dt1;
+  float wp, dwp;
+  if (CGA && pq / pa > 0 && pb / pa > 0) {
+    const float lpr = (float)(1.0 / log(pb / pa));
+    wp = __logf((float)(pq / pa)) * lpr;
+    dwp = lpr / (float)pq;
+  } else {
+    dwp = (float)(1.0 / (pb - pa));
+    wp = (float)(pq - pa) * dwp;
+  }
+  const float q0 = 1.f - wp, q1 = wp;
+  const float cw[4] = {q0 * (1.f - w0), q0 * w0, q1 * (1.f - w1), q1 * w1};
+  const float tw[4] = {-q0 * dt0, q0 * dt0, -q1 * dt1, q1 * dt1};
+  const float pw[4] = {-(1.f - w0) * dwp, -w0 * dwp, (1.f - w1) * dwp, w1 * dwp};
+  const float4 *base0 = T.rec + (c0 * T.TT + j0) * T.TU;
+  const float4 *base1 = T.rec + ((c0 + 1) * T.TT + j1) * T.TU;
+  float ec[4];
+  float dep = 0.f, du = 0.f, dt = 0.f, dp = 0.f;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const float4 *col = (c < 2 ? base0 : base1) + (c & 1) * T.TU;
+    const int n = nc[c];
+    float e, de, su, se;
+    if (CGA) {
+      if (mono) {
+        float4 r;
+        gi[c] = col_walk<0>(col, n, x, gi[c], r);
+        e = eps_rec_f_vs(r, x, se);
+      } else {
+        e = col_eps_bis_vs(col, n, x, se);
+      }
+      de = 0.f;
+    } else if (!mono) {
+      const double u = col_u_bis_vs(col, n, ep, su);
+      e = col_eps_bis_vs(col, n, u + x, se);
+      de = se * su;
+    } else {
+      float4 r;
+      const int i = col_walk_f<1>(col, n, ep, gi[c], r);
+      gi[c] = i;
+      if (ep < r.y) {
+        su = fdiv(r.x, r.y);
+        const double arg = (double)(su * ep) + x;
+        float4 r2;
+        col_walk_r<0>(col, n, arg, i, r, r2);
+        e = eps_rec_f_vs(r2, arg, se);
+        de = se * su;
+      } else if (ep > r.w) {
+        const float a = fdiv(__logf(1.f - r.w), r.z);
+        const double arg = (double)fdiv(log1pf(-ep), a) + x;
+        const float ex = __expf(a * (float)arg);
+        e = 1.f - ex;
+        se = ex > 0.f ? -a * ex : 0.f;
+        de = ex > 0.f ? ex * a * frcp(a) / (1.f - ep) : 0.f;
+      } else {
+        const float dur = r.z - r.x, der = r.w - r.y;
+        su = fdiv(dur, der);
+        const float off = (ep - r.y) * su + (float)x;
+        if (off < dur) {
+          se = fdiv(der, dur);
+          e = r.y + off * se;
+        } else {
+          const double arg = (double)r.x + (double)off;
+          float4 r2;
+          col_walk_r<0>(col, n, arg, i, r, r2);
+          e = eps_rec_f_vs(r2, arg, se);
+        }
+        de = se * su;
+      }
+    }
+    ec[c] = e;
+    dep += cw[c] * de;
+    du += cw[c] * se;
+    dt += tw[c] * e;
+    dp += pw[c] * e;
+  }
+  const float a = ec[0] + (ec[1] - ec[0]) * w0;
+  const float b = ec[2] + (ec[3] - ec[2]) * w1;
+  const float en = a + (b - a) * wp;
+  if (en < 0.f || en > 1.f) return d;  // clamped: constant
+  d.dep = dep;
+  d.du = du;
+  d.dt = dt;
+  d.dp = dp;
+  return d;
+}
+
+// Continuum extinction beta of one segment (formod_continua,
+// src/jurassic.c:3075-3117) and its partials.
+struct BetaGrad {
+  double dp, dt, du_co2, du_h2o, dq_h2o;
+};
+
+__device__ __forceinline__ double beta_grad(const ModelView &M, const CtmChan &K, double nu, double p,
+                                            double t, double ds, double q_h2o, double u_co2,
+                                            double u_h2o, double kwin, double clx, BetaGrad &g) {
+  g.dp = g.dt = g.du_co2 = g.du_h2o = g.dq_h2o = 0;
+  double beta = kwin + clx;
+  if (M.ig_co2 >= 0 && K.co2) {
+    const double dt230 = t - 230, dt260 = t - 260, dt296 = t - 296;
+    const double ctw = dt260 * 5.050505e-4 * dt296 * K.co2_230 - dt230 * 9.259259e-4 * dt296 * K.co2_260 +
+                       dt230 * 4.208754e-4 * dt260 * K.co2_296;
+    const double dctw = 5.050505e-4 * K.co2_230 * (dt260 + dt296) -
+                        9.259259e-4 * K.co2_260 * (dt230 + dt296) +
+                        4.208754e-4 * K.co2_296 * (dt230 + dt260);
+    const double f = 1 / JUR_NA / 1000 / JUR_P0 / ds;
+    beta += u_co2 * p * ctw * f;
+    g.du_co2 = p * ctw * f;
+    g.dp += u_co2 * ctw * f;
+    g.dt += u_co2 * p * dctw * f;
+  }
+  if (M.ig_h2o >= 0 && K.h2o) {
+    const double S = K.h2o_slf * pow(K.h2o_ratio, (296 - t) / (296 - 260));
+    const double dS = K.h2o_ratio > 0 ? S * log(K.h2o_ratio) * (-1.0 / 36) : 0.0;
+    const double bq = .7193876 * nu;
+    const double th = tanh(bq / t);
+    const double a1u = nu * th;                            // a1 per unit column
+    const double da1u = nu * (1 - th * th) * (-bq / (t * t));
+    const double a2 = 296 / t, da2 = -296 / (t * t);
+    const double mix = q_h2o * S + (1 - q_h2o) * K.h2o_frn;
+    const double a3 = p / JUR_P0 * mix * 1e-20;
+    const double da3 = p / JUR_P0 * q_h2o * dS * 1e-20;
+    const double rds = 1 / ds;
+    beta += a1u * u_h2o * a2 * a3 * rds;
+    g.du_h2o = a1u * a2 * a3 * rds;
+    g.dq_h2o = a1u * u_h2o * a2 * (p / JUR_P0 * (S - K.h2o_frn) * 1e-20) * rds;
+    g.dp += a1u * u_h2o * a2 * (mix / JUR_P0 * 1e-20) * rds;
+    g.dt += u_h2o * (da1u * a2 * a3 + a1u * da2 * a3 + a1u * a2 * da3) * rds;
+  }
+  if (M.ctm_n2 && K.n2) {
+    const double qn = 0.79, r = p / JUR_P0 * 273.0 / t;
+    const double ex = exp(K.n2_beta * (1 / 296.0 - 1 / t));
+    const double F = qn + (1 - qn) * (1.294 - 0.4545 * t / 296.0);
+    const double base = 0.1 * r * r * ex * qn * K.n2_b;
+    const double c = base * F;
+    beta += c;
+    g.dp += 2 * c / p;
+    g.dt += c * (-2 / t + K.n2_beta / (t * t)) + base * (1 - qn) * (-0.4545 / 296.0);
+  }
+  if (M.ctm_o2 && K.o2) {
+    const double r = p / JUR_P0 * 273.0 / t;
+    const double c = 0.1 * r * r * exp(K.o2_beta * (1 / 296.0 - 1 / t)) * 0.21 * K.o2_b;
+    beta += c;
+    g.dp += 2 * c / p;
+    g.dt += c * (-2 / t + K.o2_beta / (t * t));
+  }
+  return beta;
+}
+
+// formod_srcfunc value and its slope in T.
+__device__ __forceinline__ double srcfunc_d(const double *st, const double *sr_col, double t, double &d) {
+  const int it = locate_reg(st, TBLNS, t);
+  d = (sr_col[it + 1] - sr_col[it]) / (st[it + 1] - st[it]);
+  return sr_col[it] + d * (t - st[it]);
+}
+
+// Stored chain state: the FP32 lookups store e (float), the FP64 lookups in
+// path mode store P (double).
+__device__ __forceinline__ double trans_of(float s) { return 1.0 - (double)s; }
+__device__ __forceinline__ double trans_of(double s) { return s; }
+__device__ __forceinline__ double emis_of(float s) { return (double)s; }
+__device__ __forceinline__ double emis_of(double s) { return 1.0 - s; }
+
+// Level bracket of altitude z in the atmosphere (locate_irr, with the
+// previous bracket kept while it still holds) and the LIN weight.
+__device__ __forceinline__ int level_of(const atm_t *atm, double z, int lev, double &w) {
+  const int n = atm->np;
+  if (!(lev >= 0 && lev <= n - 2 && atm->z[lev] <= z && z < atm->z[lev + 1]))
+    lev = locate_irr(atm->z, n, z);
+  const double z0 = atm->z[lev], z1 = atm->z[lev + 1];
+  w = (z - z0) / (z1 - z0);
+  return lev;
+}
+
+// Sums of point adjoints of the current level pair (wa, wa+1) for NQ level
+// quantities (JUR_XMAP_LEV quantity index qi[q], -1 = none); flushed into the
+// lane's tile row when the pair changes.
+// Quantities in the mask ATOM are added atomically (columns that other
+// wavefronts of the same row also update).
+template <int NQ, unsigned ATOM = 0>
+struct LevelWindow {
+  int wa = -1;
+  double acc[NQ][2];
+  __device__ __forceinline__ void flush(const int *xm, const int *qi, double *kt) {
+    if (wa < 0) return;
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        if (qi[q] < 0) continue;
+        const int c = xm[JUR_XMAP_LEV(qi[q], wa + k)];
+        if (c < 0) continue;
+        if (ATOM >> q & 1u) unsafeAtomicAdd(&kt[(size_t)c * WAVE], acc[q][k]);
+        else kt[(size_t)c * WAVE] += acc[q][k];
+      }
+  }
+  // Point adjoints v[q] at altitude z; quantity q == qp is pressure (LOGY
+  // weights p/p_level), the others use the LIN weights.
+  __device__ __forceinline__ void add(const atm_t *atm, double z, int &lev, const double *v, int qp, double p,
+                                      const int *xm, const int *qi, double *kt) {
+    double w;
+    lev = level_of(atm, z, lev, w);
+    if (lev != wa) {
+      flush(xm, qi, kt);
+      wa = lev;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) acc[q][0] = acc[q][1] = 0;
+    }
+    const double P0 = atm->p[lev], P1 = atm->p[lev + 1];
+    const bool plog = P1 / P0 > 0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      if (q == qp && plog) {
+        acc[q][0] += (1 - w) * p / P0 * v[q];
+        acc[q][1] += w * p / P1 * v[q];
+      } else {
+        acc[q][0] += (1 - w) * v[q];
+        acc[q][1] += w * v[q];
+      }
+    }
+  }
+};
+
+// Forward sweep, surface seeds, reverse sweep of the accumulation: writes
+// rad, tau, H_i (scratch), the number of points before the first opaque
+// segment, and the K entries of continua, source function, window
+// extinction, clouds and surface.
+template <typename SF>
+__global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
+                                                  const RayInfo *info, int blk0, int nblk, JacView J,
+                                                  double *rad_out, double *tau_out) {
+  const int nd = M.nd, ng = M.ng;
+  int bl, id;
+  if (!xcd_block_map(blockIdx.x, nd, nblk, bl, id)) return;
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + bl) * WAVE + lane;
+  if (ir >= B.nray) return;
+  const RayInfo R = info[ir];
+  if (R.status != JUR_OK) return;
+  const int ipf = B.ray_prof[ir];
+  const int im = J.prof_map[ipf];
+  const atm_t *atm = &B.atm[ipf];
+  const ChanInfo C = M.chan[id];
+  const double nu = C.nu;
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const int npb = E.npb[bl];
+  const SF *epsb = (const SF *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
+#define EPS_AT(g, ip) epsb[((size_t)(g) * npb + (ip)) * WAVE]
+  double *sc = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE + lane;
+#define SCR(k, ip) sc[((size_t)(ip) * J.nscr + (k)) * WAVE]
+  double *kt = J.kt + ((size_t)bl * nd + id) * J.ncol * WAVE + lane;
+#define KT(col) kt[(size_t)(col) * WAVE]
+  const double *sr_col = M.sr + (size_t)id * TBLNS;
+
+  unsigned gm = 0;  // emitters with a table for this channel
+  for (int g = 0; g < ng; g++)
+    if (M.tab.np[g * nd + id] >= 30) gm |= 1u << g;
+  // product of the path transmittances of those emitters after point ip
+  auto qprod = [&](int ip) {
+    double q = 1;
+    for (int g = 0; g < ng; g++)
+      if (gm >> g & 1u) q *= trans_of(EPS_AT(g, ip));
+    return q;
+  };
+
+  double kcl = 0, wcl = 0;
+  const bool cloud = L.f_cl >= 0 && M.ncl > 0;
+  if (cloud) {
+    const double x0 = M.clnu[C.icl], x1 = M.clnu[C.icl + 1];
+    kcl = lin(x0, atm->clk[C.icl], x1, atm->clk[C.icl + 1], nu);
+    wcl = (nu - x0) / (x1 - x0);
+  }
+  double sfeps = 1.0, wsf = 0;
+  if (M.nsf > 0) {
+    const double x0 = M.sfnu[C.isf], x1 = M.sfnu[C.isf + 1];
+    sfeps = lin(x0, atm->sfeps[C.isf], x1, atm->sfeps[C.isf + 1], nu);
+    wsf = (nu - x0) / (x1 - x0);
+  }
+  const bool has_k = C.window < M.nw;
+  const CtmChan K = ctm_chan(M.ctm, nu);
+  const int fq_h2o = M.ig_h2o >= 0 ? F_Q0 + M.ig_h2o : -1;
+  const int fu_h2o = M.ig_h2o >= 0 ? L.f_u + M.ig_h2o : -1;
+  const int fu_co2 = M.ig_co2 >= 0 ? L.f_u + M.ig_co2 : -1;
+
+  // ---------------- forward sweep: tau_i (and D_i) per point ----------------
+  double rad = 0, tau = 1, dr = 0;
+  int nef = R.np;
+  {
+    double qp = 1;
+    for (int ip = 0; ip < R.np; ip++) {
+      const double q = qprod(ip);
+      if (!(q > 0)) {
+        nef = ip;
+        break;
+      }
+      const double tg = q / qp;
+      qp = q;
+      const double p = LOSR(F_P, ip), t = LOSR(F_T, ip), ds = LOSR(F_DS, ip);
+      BetaGrad bg;
+      const double beta = beta_grad(M, K, nu, p, t, ds, fq_h2o >= 0 ? LOSR(fq_h2o, ip) : 0.0,
+                                    fu_co2 >= 0 ? LOSR(fu_co2, ip) : 0.0,
+                                    fu_h2o >= 0 ? LOSR(fu_h2o, ip) : 0.0,
+                                    has_k ? LOSR(L.f_k + C.window, ip) : 0.0,
+                                    cloud ? LOSR(L.f_cl, ip) * kcl : 0.0, bg);
+      double dsrc;
+      const double src = srcfunc_d(M.st, sr_col, t, dsrc);
+      const double s = tg * exp(-beta * ds);
+      SCR(0, ip) = tau;
+      if (J.nscr > 1) SCR(1, ip) = dr;
+      rad += src * (1 - s) * tau;
+      dr = dr * s + src * (1 - s);
+      tau *= s;
+    }
+  }
+
+  // surface emission, reflection and solar term, and their adjoint seeds
+  double tb = 0, db = 0, sft_bar = 0, sfeps_bar = 0;
+  if (M.sftype >= 1 && R.sft > 0) {
+    double dsrc_sf;
+    const double src_sf = srcfunc_d(M.st, sr_col, R.sft, dsrc_sf);
+    rad += sfeps * src_sf * tau;
+    tb += sfeps * src_sf;
+    sfeps_bar += src_sf * tau;
+    sft_bar = sfeps * tau * dsrc_sf;
+    if (M.sftype >= 2 && sfeps < 1) {
+      rad += dr * tau * (1 - sfeps);
+      tb += dr * (1 - sfeps);
+      db += tau * (1 - sfeps);
+      sfeps_bar -= dr * tau;
+      if (M.sftype >= 3 && R.pad > 0) {
+        const double ks = 6.764e-5 / (2. * JUR_PI) * C.planck_sun * R.rcos;
+        rad += ks * tau * (1 - sfeps) * tau;
+        tb += 2 * ks * tau * (1 - sfeps);
+        sfeps_bar -= ks * tau * tau;
+      }
+    }
+  }
+  rad_out[(size_t)ir * nd + id] = rad;
+  tau_out[(size_t)ir * nd + id] = tau;
+  J.nef[((size_t)bl * nd + id) * WAVE + lane] = nef;
+  if (im < 0 || im >= J.nmap) return;  // no state map: K row stays zero
+  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
+
+  // ---------------- reverse sweep of the accumulation ----------------
+  // H_i replaces tau_i in the scratch; continua, source function, window
+  // extinction and clouds contribute here, the emitter chains below.
+  double clz_bar = 0, cldz_bar = 0, clk_bar = 0;
+  {
+    const int qi[5] = {1, 0, has_k ? 2 + NG + C.window : -1, M.ig_co2 >= 0 ? 2 + M.ig_co2 : -1,
+                       M.ig_h2o >= 0 ? 2 + M.ig_h2o : -1};
+    LevelWindow<5> W;
+    int lev = -1;
+    double qi_cur = nef > 0 ? qprod(nef - 1) : 1;
+    for (int ip = nef - 1; ip >= 0; ip--) {
+      const double qm = ip > 0 ? qprod(ip - 1) : 1;
+      const double tg = qi_cur / qm;
+      qi_cur = qm;
+      const double z = LOSR(F_Z, ip), p = LOSR(F_P, ip), t = LOSR(F_T, ip), ds = LOSR(F_DS, ip);
+      const double q_h2o = fq_h2o >= 0 ? LOSR(fq_h2o, ip) : 0.0;
+      const double u_co2 = fu_co2 >= 0 ? LOSR(fu_co2, ip) : 0.0;
+      const double u_h2o = fu_h2o >= 0 ? LOSR(fu_h2o, ip) : 0.0;
+      const double aux = cloud ? LOSR(L.f_cl, ip) : 0.0;
+      BetaGrad bg;
+      const double beta = beta_grad(M, K, nu, p, t, ds, q_h2o, u_co2, u_h2o,
+                                    has_k ? LOSR(L.f_k + C.window, ip) : 0.0, aux * kcl, bg);
+      double dsrc;
+      const double src = srcfunc_d(M.st, sr_col, t, dsrc);
+      const double s = tg * exp(-beta * ds);
+      const double tau_i = SCR(0, ip);
+      const double dr_i = J.nscr > 1 ? SCR(1, ip) : 0.0;
+      const double G = tau_i * (tb - src) + db * (dr_i - src);
+      const double src_bar = (1 - s) * (tau_i + db);
+      tb = src * (1 - s) + tb * s;
+      db = db * s;
+      const double H = G * s;
+      SCR(0, ip) = H;
+      const double beta_bar = -H * ds;
+
+      // u = 10 q p / (kB T) ds
+      const double ucoef = 10 * p / (JUR_KB * t) * ds;
+      const double uc_bar = beta_bar * bg.du_co2, uh_bar = beta_bar * bg.du_h2o;
+      double v[5];
+      v[0] = beta_bar * bg.dt + src_bar * dsrc - (uc_bar * u_co2 + uh_bar * u_h2o) / t;  // T
+      v[1] = beta_bar * bg.dp + (uc_bar * u_co2 + uh_bar * u_h2o) / p;                  // p
+      v[2] = beta_bar;                                                                  // k
+      v[3] = uc_bar * ucoef;                                                            // q_co2
+      v[4] = uh_bar * ucoef + beta_bar * bg.dq_h2o;                                     // q_h2o
+      W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+
+      if (cloud) {
+        clk_bar += beta_bar * aux;
+        if (atm->cldz > 0) {
+          const double cl_bar = beta_bar * kcl;
+          const double dz = (z - atm->clz) / atm->cldz;
+          clz_bar += cl_bar * aux * dz / atm->cldz;
+          cldz_bar += cl_bar * aux * dz * dz / atm->cldz;
+        }
+      }
+    }
+    W.flush(xm, qi, kt);
+  }
+
+  // surface terms
+  if (M.sftype >= 1 && R.sft > 0) {
+    if (M.nsf > 0 && atm->sft > 0) {
+      const int c = xm[JUR_XMAP_SFT];
+      if (c >= 0) KT(c) += sft_bar;
+    } else if (R.np > 0) {
+      // the surface temperature is the LOS temperature of the last point
+      double w;
+      const int lv = level_of(atm, LOSR(F_Z, R.np - 1), -1, w);
+      int c;
+      if ((c = xm[JUR_XMAP_LEV(1, lv)]) >= 0) KT(c) += (1 - w) * sft_bar;
+      if ((c = xm[JUR_XMAP_LEV(1, lv + 1)]) >= 0) KT(c) += w * sft_bar;
+    }
+    if (M.nsf > 0) {
+      int c;
+      if ((c = xm[JUR_XMAP_SFEPS(C.isf)]) >= 0) KT(c) += (1 - wsf) * sfeps_bar;
+      if (C.isf + 1 < NSF && (c = xm[JUR_XMAP_SFEPS(C.isf + 1)]) >= 0) KT(c) += wsf * sfeps_bar;
+    }
+  }
+  if (cloud) {
+    int c;
+    if ((c = xm[JUR_XMAP_CLK(C.icl)]) >= 0) KT(c) += (1 - wcl) * clk_bar;
+    if (C.icl + 1 < NCL && (c = xm[JUR_XMAP_CLK(C.icl + 1)]) >= 0) KT(c) += wcl * clk_bar;
+    if ((c = xm[JUR_XMAP_CLZ]) >= 0) KT(c) += clz_bar;
+    if ((c = xm[JUR_XMAP_CLDZ]) >= 0) KT(c) += cldz_bar;
+  }
+#undef EPS_AT
+#undef SCR
+#undef KT
+}
+
+// Reverse sweep of one emitter chain per wavefront: L_g,i from H_i and the
+// recomputed table partials; T and p entries are shared with the other
+// emitters of the row (atomic adds), q_g entries belong to this chain.
+template <bool CGA, typename SF>
+__global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, LosView L, EpsView E,
+                                                    const RayInfo *info, int blk0, int nblk, JacView J) {
+  const int nd = M.nd, ng = M.ng;
+  int bl, rem;
+  if (!xcd_block_map(blockIdx.x, nd * ng, nblk, bl, rem)) return;
+  const int id = rem / ng;
+  const int g = rem - id * ng;
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + bl) * WAVE + lane;
+  if (ir >= B.nray) return;
+  const int tix = g * nd + id;
+  const int npt = M.tab.np[tix];
+  if (npt < 30) return;
+  const RayInfo R = info[ir];
+  if (R.status != JUR_OK) return;
+  const int ipf = B.ray_prof[ir];
+  const int im = J.prof_map[ipf];
+  if (im < 0 || im >= J.nmap) return;
+  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
+  const atm_t *atm = &B.atm[ipf];
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const int npb = E.npb[bl];
+  const SF *epsb = (const SF *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
+#define EPS_AT(g, ip) epsb[((size_t)(g) * npb + (ip)) * WAVE]
+  const double *sc = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE + lane;
+#define SCR(k, ip) sc[((size_t)(ip) * J.nscr + (k)) * WAVE]
+  double *kt = J.kt + ((size_t)bl * nd + id) * J.ncol * WAVE + lane;
+  const int nef = J.nef[((size_t)bl * nd + id) * WAVE + lane];
+  const int mono = M.tab.mono[tix], pdir = M.tab.pdir[tix];
+  const int fu = L.f_u + g;
+  double cu = 0, cp = 0, ct = 0;  // Curtis-Godson sums up to the current point
+  if (CGA)
+    for (int ip = 0; ip < nef; ip++) {
+      const double u = LOSR(fu, ip);
+      cu += u;
+      cp += u * LOSR(F_P, ip);
+      ct += u * LOSR(F_T, ip);
+    }
+  const int qi[3] = {1, 0, 2 + g};
+  LevelWindow<3, 3u> W;
+  int lev = -1, ipr = -1;
+  int gi[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) gi[c] = M.tab.TU / 2;
+  double carry = 0, Hn = 0, sU = 0, sP = 0, sT = 0;
+  double trc = nef > 0 ? trans_of(EPS_AT(g, nef - 1)) : 1;
+  constexpr bool F32 = sizeof(SF) == sizeof(float);
+  for (int ip = nef - 1; ip >= 0; ip--) {
+    const double H = SCR(0, ip);
+    double epp = 0, trp = 1;
+    float epf = 0.f;  // the stored float path emissivity (FP32 path)
+    if (ip > 0) {
+      const SF sp = EPS_AT(g, ip - 1);
+      epp = emis_of(sp);
+      trp = trans_of(sp);
+      epf = (float)sp;
+    }
+    const double z = LOSR(F_Z, ip), p = LOSR(F_P, ip), t = LOSR(F_T, ip), ds = LOSR(F_DS, ip);
+    const double useg = LOSR(fu, ip);
+    const double lam = (Hn - H) / trc + carry;
+    double u_bar, p_bar, t_bar;
+    if (CGA) {
+      if (cu > 0) {
+        const double pq = cp / cu, tq = ct / cu;
+        const LkPart d = F32 ? lookup_partials_f32<true>(M.tab, tix, npt, mono, pdir, epf, pq, (float)tq, cu,
+                                                         ipr, gi)
+                             : lookup_partials<true>(M.tab, tix, npt, mono, pdir, epp, pq, tq, cu, ipr, gi);
+        sU += lam * (d.du - (d.dp * pq + d.dt * tq) / cu);
+        sP += lam * d.dp / cu;
+        sT += lam * d.dt / cu;
+        carry = lam * d.dep;
+      } else {
+        carry = lam;
+      }
+      u_bar = sU + p * sP + t * sT;
+      p_bar = useg * sP;
+      t_bar = useg * sT;
+      cu -= useg;
+      cp -= useg * p;
+      ct -= useg * t;
+    } else {
+      const LkPart d = F32 ? lookup_partials_f32<false>(M.tab, tix, npt, mono, pdir, epf, p, (float)t, useg,
+                                                        ipr, gi)
+                           : lookup_partials<false>(M.tab, tix, npt, mono, pdir, epp, p, t, useg, ipr, gi);
+      u_bar = lam * d.du;
+      p_bar = lam * d.dp;
+      t_bar = lam * d.dt;
+      carry = lam * d.dep;
+    }
+    const double ucoef = 10 * p / (JUR_KB * t) * ds;
+    double v[3];
+    v[0] = t_bar - u_bar * useg / t;
+    v[1] = p_bar + u_bar * useg / p;
+    v[2] = u_bar * ucoef;
+    W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+    Hn = H;
+    trc = trp;
+  }
+  W.flush(xm, qi, kt);
+#undef EPS_AT
+#undef SCR
+}
+
+// Tile -> row-major K[ray][channel][col], with d BT / d rad (BRIGHT,
+// src/jurassic.h:126-127) applied to rows written as brightness temperature
+// and NaN rows for masked radiances.  One workgroup per (ray block,
+// channel); an LDS transpose of 64 columns at a time makes both the tile
+// reads and the row writes contiguous.
+__global__ __launch_bounds__(WAVE) void k_jac_store(const double *kt, int nd, int ncol, int blk0,
+                                                    int nray, const double *rad, const ChanInfo *chan,
+                                                    int write_bbt, const unsigned char *mask, double *K) {
+  __shared__ double tile[WAVE][WAVE + 1];
+  __shared__ double scale[WAVE];
+  const int bl = blockIdx.x / nd, id = blockIdx.x - bl * nd;
+  const int lane = threadIdx.x;
+  const int ray0 = (blk0 + bl) * WAVE;
+  const int ir = ray0 + lane;
+  double sc = 1;
+  if (ir < nray) {
+    const size_t o = (size_t)ir * nd + id;
+    if (mask && mask[o]) {
+      sc = __builtin_nan("");
+    } else if (write_bbt) {
+      const double nu = chan[id].nu, r = rad[o];
+      const double a = JUR_C1 * (nu * nu * nu);
+      const double lg = log1p(a / r);
+      sc = JUR_C2 * nu * a / (lg * lg * r * (r + a));
+    }
+  }
+  scale[lane] = sc;
+  const double *src = kt + ((size_t)bl * nd + id) * ncol * WAVE;
+  const int nr = nray - ray0 < WAVE ? nray - ray0 : WAVE;
+  for (int c0 = 0; c0 < ncol; c0 += WAVE) {
+    const int nc = ncol - c0 < WAVE ? ncol - c0 : WAVE;
+    __syncthreads();
+    for (int c = 0; c < nc; c++) tile[c][lane] = src[(size_t)(c0 + c) * WAVE + lane];
+    __syncthreads();
+    if (lane < nc)
+      for (int r = 0; r < nr; r++) K[((size_t)(ray0 + r) * nd + id) * ncol + c0 + lane] = tile[lane][r] * scale[r];
+  }
+}
+
+template <bool CGA, typename SF>
+static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
+                       const RayInfo *info, int blk0, int nblk, const JacView &J, double *rad,
+                       double *tau, hipStream_t s) {
+  const int nb8 = padded_blocks(nblk);
+  hipLaunchKernelGGL((k_adj_acc<SF>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0, nblk, J,
+                     rad, tau);
+  if (M.ng > 0)
+    hipLaunchKernelGGL((k_adj_chain<CGA, SF>), dim3(nb8 * M.nd * M.ng), dim3(WAVE), 0, s, M, B, L, E, info,
+                       blk0, nblk, J);
+}
+
+void launch_adjoint(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
+                    const RayInfo *info, int blk0, int nblk, const JacView &J, bool fp32, double *rad,
+                    double *tau, hipStream_t s) {
+  if (M.nd == 0) return;
+  if (M.formod == 0) {
+    if (fp32) launch_adj<true, float>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
+    else launch_adj<true, double>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
+  } else {
+    if (fp32) launch_adj<false, float>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
+    else launch_adj<false, double>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
+  }
+}
+
+void launch_jac_store(const double *kt, int nd, int ncol, int blk0, int nblk, int nray, const double *rad,
+                      const ChanInfo *chan, int write_bbt, const unsigned char *mask, double *K,
+                      hipStream_t s) {
+  if (nd == 0 || ncol == 0 || nblk == 0) return;
+  hipLaunchKernelGGL(k_jac_store, dim3(nblk * nd), dim3(WAVE), 0, s, kt, nd, ncol, blk0, nray, rad, chan,
+                     write_bbt, mask, K);
+}
+
+}  // namespace jur

@@ -86,6 +86,19 @@ __global__ void k_fill_u64(unsigned long long *p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// Zero n 8-byte words (grid-stride, vector stores).
+__global__ void k_zero(unsigned long long *p, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+void launch_zero(void *p, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const size_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)(nb < 16384 ? nb : 16384)), dim3(256), 0, s,
+                     (unsigned long long *)p, n);
+}
+
 // formod_fov for one (ray, channel): neighbours ir-NFOV..ir+NFOV of the
 // same package with the same time, linear interpolation in view-point
 // altitude, weighted mean over the shape points.  Reads rad0/tau0 (values

@@ -265,6 +265,35 @@ __device__ __forceinline__ int col_bisect(const float4 *col, int n, double x, fl
   return ilo;
 }
 
+// ---- FP32-path table interpolation (lookup_f32.hip, adjoint.hip) ----
+// intpol_tbl_eps from a bracket record, float interpolation.
+__device__ __forceinline__ float eps_from_rec_f(const float4 &r, double x) {
+  if (x < (double)r.x) return fdiv(r.y, r.x) * (float)x;
+  if (x > (double)r.z) return 1.f - __expf(fdiv(__logf(1.f - r.w), r.z) * (float)x);
+  return r.y + fdiv(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
+}
+
+// Reference-order lookups for columns that are not monotone (bisection).
+__device__ __forceinline__ float col_eps_bis(const float4 *col, int n, double x) {
+  const float4 r0 = col[0];
+  if (x < (double)r0.x) return fdiv(r0.y, r0.x) * (float)x;
+  const float4 rl = col[n - 1];
+  if (x > (double)rl.x) return 1.f - __expf(fdiv(__logf(1.f - rl.y), rl.x) * (float)x);
+  float4 r;
+  col_bisect<0>(col, n, x, r);
+  return r.y + fdiv(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
+}
+
+__device__ __forceinline__ double col_u_bis(const float4 *col, int n, float x) {
+  const float4 r0 = col[0];
+  if (x < r0.y) return (double)(fdiv(r0.x, r0.y) * x);
+  const float4 rl = col[n - 1];
+  if (x > rl.y) return (double)(log1pf(-x) * fdiv(rl.x, __logf(1.f - rl.y)));
+  float4 r;
+  col_bisect<1>(col, n, (double)x, r);
+  return (double)r.x + (double)(fdiv(r.z - r.x, r.w - r.y) * (x - r.y));
+}
+
 // intpol_tbl_eps on one column (src/jurassic.c:3895-3929).
 // For a monotone column the local walk runs first and the two boundary
 // branches are read off the bracket record it returns: x below the first

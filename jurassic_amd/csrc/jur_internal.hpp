@@ -80,12 +80,30 @@ struct LosView {
 // lookup kernels and read by the radiance kernels.  Block blk holds npb[blk]
 // points per chain (the longest LOS of its rays), packed from off[blk]:
 //   eps[off[blk] + (((id * ng + ig) * npb[blk] + ip) * WAVE + lane]
-// FP64 path: the reference's eps of intpol_tbl_ega/cga (double).
-// FP32 path: the segment emissivity (float); 1 marks an opaque segment.
+// FP64 path: the reference's eps of intpol_tbl_ega/cga (double), or, for
+// the Jacobian, the path transmittance tau_path after each segment (double).
+// FP32 path: the path emissivity 1 - tau_path after each segment (float);
+// 1 marks an opaque path.
 struct EpsView {
   void *eps;
   const size_t *off;   // [nblk] element offset of each block (chunk-relative)
   const int *npb;      // [nblk] points per chain of each block
+};
+
+// Jacobian pass (adjoint.hip).  Per chunk of ray blocks:
+//   kt  [blk][nd][ncol][WAVE]  K tile of each (ray block, channel), lane fastest
+//   scr [soff[blk] + ((id * npb[blk] + ip) * nscr + k) * WAVE + lane]
+//       forward-sweep values per point: tau_i (k = 0), D_i (k = 1, reflection)
+struct JacView {
+  const int *xmap;      // [nmap][JUR_XMAP_LEN] column of each state element, -1 if absent
+  int nmap;
+  const int *prof_map;  // [nprof] state map of each atmosphere
+  int ncol;             // row length of K
+  double *kt;
+  double *scr;
+  const size_t *soff;   // [nblk]
+  int nscr;             // 1, or 2 with a reflecting surface
+  int *nef;             // [blk][nd][WAVE] points before the first opaque segment
 };
 
 #define LOS_AT(base, L, f, ip) (base)[((size_t)(ip) * (L).nfield + (size_t)(f)) * WAVE]

@@ -34,7 +34,11 @@ void launch_raytrace(const ModelView &, const BatchView &, const LosView &, RayI
                      hipStream_t);
 void launch_hydrostatic(atm_t *, int, double, int, hipStream_t);
 void launch_lookup_f64(const ModelView &, const BatchView &, const LosView &, const EpsView &,
-                       const RayInfo *, int, int, hipStream_t);
+                       const RayInfo *, int, int, bool, hipStream_t);
+void launch_adjoint(const ModelView &, const BatchView &, const LosView &, const EpsView &,
+                    const RayInfo *, int, int, const JacView &, bool, double *, double *, hipStream_t);
+void launch_jac_store(const double *, int, int, int, int, int, const double *, const ChanInfo *, int,
+                      const unsigned char *, double *, hipStream_t);
 void launch_lookup_f32(const ModelView &, const BatchView &, const LosView &, const EpsView &,
                        const RayInfo *, int, int, hipStream_t);
 void launch_radiance_f64(const ModelView &, const BatchView &, const LosView &, const EpsView &,
@@ -50,6 +54,7 @@ void launch_fov(const double *, const double *, double *, double *, int, int, co
 void launch_finalize(double *, int, int, const ChanInfo *, int, const unsigned char *,
                      hipStream_t);
 void launch_block_np(const RayInfo *, int, int, int, int *, hipStream_t);
+void launch_zero(void *, size_t, hipStream_t);
 
 // ------------------------------------------------------------------
 // Errors.
@@ -357,6 +362,10 @@ struct jur_ctx {
   jur::DevBuf<unsigned long long> d_red;
   jur::DevBuf<double> d_rad0, d_tau0;
   jur::DevBuf<int> d_err;
+  // Jacobian work buffers
+  jur::DevBuf<double> d_kt, d_scr;
+  jur::DevBuf<size_t> d_soff;
+  jur::DevBuf<int> d_xmap, d_nef;
   size_t los_budget = 0;  // bytes for LOS + emissivity buffers (0: from free HBM)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
@@ -608,9 +617,20 @@ struct FovPkg {
   const int *lo = nullptr, *hi = nullptr;  // device, per ray
 };
 
+// Jacobian request of a pass: state maps (device), K (device).
+struct JacArgs {
+  const int *xmap = nullptr;
+  int nmap = 0;
+  const int *prof_map = nullptr;
+  int ncol = 0;
+  double *k = nullptr;
+};
+
+// One pass over a device batch: the forward model, or with jac the forward
+// model and its Jacobian (adjoint sweep).
 static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
                         const unsigned char *d_mask, double *d_rad, double *d_tau, double *d_tp,
-                        const FovPkg *fov, hipStream_t st) {
+                        const FovPkg *fov, hipStream_t st, const JacArgs *jac = nullptr) {
   const Setup &s = c->s;
   ModelView M = model_view(c);
   int rc;
@@ -655,24 +675,35 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
     L.f_u = L.f_k + s.nw + (s.ncl > 0 ? 1 : 0);
     L.nfield = L.f_u + s.ng;
     const size_t esz = (precision == JUR_FP32) ? sizeof(float) : sizeof(double);
-    const size_t per_blk = (size_t)L.nfield * L.cap * WAVE * sizeof(double) +
-                           (size_t)s.nd * s.ng * L.cap * WAVE * esz;
+    // D_i is needed only where the surface reflects (sfeps < 1 needs NSF > 0)
+    const int nscr = (s.sftype >= 2 && s.nsf > 0) ? 2 : 1;
+    // Memory per ray block: the LOS at full capacity, and per LOS point of
+    // the block's longest ray the emissivities (and for the Jacobian the
+    // forward-sweep scratch), plus the Jacobian's K tile.
+    const size_t los_blk = (size_t)L.nfield * L.cap * WAVE * sizeof(double);
+    const size_t pt_blk = (size_t)s.nd * s.ng * WAVE * esz + (jac ? (size_t)s.nd * nscr * WAVE * sizeof(double) : 0);
+    const size_t tile_blk = jac ? (size_t)s.nd * jac->ncol * WAVE * sizeof(double) : 0;
     size_t budget = c->los_budget;
     if (budget == 0) {
-      // half of the HBM free now (plus what these buffers already hold),
-      // at most 96 GiB: one chunk covers ~10^5 limb rays with 36 chains each
+      // half of the HBM free now (plus what these buffers already hold), at
+      // most 96 GiB
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
-      fr += c->d_los.n * sizeof(double) + c->d_eps.n;
+      fr += c->d_los.n * sizeof(double) + c->d_eps.n + (c->d_kt.n + c->d_scr.n) * sizeof(double);
       budget = std::min(fr / 2, (size_t)96 << 30);
     }
-    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, budget / per_blk));
+    // Rays are traced in chunks whose LOS takes at most half the budget; the
+    // lookups and the radiance (or adjoint) pass then run over sub-chunks of
+    // a chunk sized by the blocks' actual LOS lengths.
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, (budget / 2) / (los_blk + 1)));
+    const size_t rest = budget > los_blk * chunk ? budget - los_blk * chunk : 0;
     if ((rc = c->d_los.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
     if ((rc = c->d_npb.reserve(chunk))) return rc;
     if ((rc = c->d_eoff.reserve(chunk))) return rc;
+    if (jac && (rc = c->d_soff.reserve(chunk))) return rc;
     L.los = c->d_los.p;
     std::vector<int> npb(chunk);
-    std::vector<size_t> eoff(chunk);
+    std::vector<size_t> eoff(chunk), soff(chunk);
     float ms[3] = {0, 0, 0};
     for (int b0 = 0; b0 < nblk; b0 += chunk) {
       const int nb = std::min(chunk, nblk - b0);
@@ -680,32 +711,67 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
       launch_raytrace(M, B, L, c->d_info.p, b0, nb, st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->ev[1], st));
-      // pack the emissivity buffer by the longest LOS of each block
       launch_block_np(c->d_info.p, B.nray, b0, nb, c->d_npb.p, st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(npb.data(), c->d_npb.p, nb * sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
-      size_t tot = 0;
-      for (int b = 0; b < nb; b++) {
-        eoff[b] = tot;
-        tot += (size_t)s.nd * s.ng * npb[b] * WAVE;
-      }
-      if ((rc = c->d_eps.reserve(tot * esz + 1))) return rc;
-      HIPCHK(hipMemcpyAsync(c->d_eoff.p, eoff.data(), nb * sizeof(size_t), hipMemcpyHostToDevice, st));
-      EpsView E{c->d_eps.p, c->d_eoff.p, c->d_npb.p};
-      if (precision == JUR_FP32) launch_lookup_f32(M, B, L, E, c->d_info.p, b0, nb, st);
-      else launch_lookup_f64(M, B, L, E, c->d_info.p, b0, nb, st);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c->ev[2], st));
-      if (precision == JUR_FP32) launch_radiance_f32(M, B, L, E, c->d_info.p, b0, nb, d_rad, d_tau, st);
-      else launch_radiance_f64(M, B, L, E, c->d_info.p, b0, nb, d_rad, d_tau, st);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c->ev[3], st));
-      HIPCHK(hipEventSynchronize(c->ev[3]));
-      for (int k = 0; k < 3; k++) {
+      {
         float a = 0;
-        HIPCHK(hipEventElapsedTime(&a, c->ev[k], c->ev[k + 1]));
-        ms[k] += a;
+        HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        ms[0] += a;
+      }
+      for (int s0 = 0; s0 < nb;) {
+        // sub-chunk [s0, s1): emissivities packed by each block's longest LOS
+        int s1 = s0;
+        size_t tot = 0, stot = 0, mem = 0;
+        while (s1 < nb) {
+          const size_t need = (size_t)npb[s1] * pt_blk + tile_blk;
+          if (s1 > s0 && mem + need > rest) break;
+          eoff[s1 - s0] = tot;
+          tot += (size_t)s.nd * s.ng * npb[s1] * WAVE;
+          soff[s1 - s0] = stot;
+          stot += (size_t)s.nd * npb[s1] * nscr * WAVE;
+          mem += need;
+          s1++;
+        }
+        const int ns = s1 - s0;
+        if ((rc = c->d_eps.reserve(tot * esz + 1))) return rc;
+        HIPCHK(hipMemcpyAsync(c->d_eoff.p, eoff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
+        LosView Ls = L;
+        Ls.los = L.los + (size_t)s0 * L.cap * L.nfield * WAVE;
+        EpsView E{c->d_eps.p, c->d_eoff.p, c->d_npb.p + s0};
+        const int bs = b0 + s0;
+        HIPCHK(hipEventRecord(c->ev[1], st));
+        if (precision == JUR_FP32) launch_lookup_f32(M, B, Ls, E, c->d_info.p, bs, ns, st);
+        else launch_lookup_f64(M, B, Ls, E, c->d_info.p, bs, ns, jac != nullptr, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[2], st));
+        if (jac) {
+          if ((rc = c->d_scr.reserve(stot + 1))) return rc;
+          if ((rc = c->d_kt.reserve((size_t)ns * s.nd * jac->ncol * WAVE + 1))) return rc;
+          if ((rc = c->d_nef.reserve((size_t)ns * s.nd * WAVE + 1))) return rc;
+          HIPCHK(hipMemcpyAsync(c->d_soff.p, soff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
+          launch_zero(c->d_kt.p, (size_t)ns * s.nd * jac->ncol * WAVE, st);
+          JacView J{jac->xmap, jac->nmap, jac->prof_map, jac->ncol, c->d_kt.p, c->d_scr.p, c->d_soff.p, nscr,
+                    c->d_nef.p};
+          launch_adjoint(M, B, Ls, E, c->d_info.p, bs, ns, J, precision == JUR_FP32, d_rad, d_tau, st);
+          HIPCHK(hipGetLastError());
+          launch_jac_store(c->d_kt.p, s.nd, jac->ncol, bs, ns, B.nray, d_rad, c->d_chan.p, s.write_bbt,
+                           d_mask, jac->k, st);
+        } else if (precision == JUR_FP32) {
+          launch_radiance_f32(M, B, Ls, E, c->d_info.p, bs, ns, d_rad, d_tau, st);
+        } else {
+          launch_radiance_f64(M, B, Ls, E, c->d_info.p, bs, ns, d_rad, d_tau, st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[3], st));
+        HIPCHK(hipEventSynchronize(c->ev[3]));
+        for (int k = 1; k < 3; k++) {
+          float a = 0;
+          HIPCHK(hipEventElapsedTime(&a, c->ev[k], c->ev[k + 1]));
+          ms[k] += a;
+        }
+        s0 = s1;
       }
     }
     launch_reduce_info(c->d_info.p, B.nray, c->d_red.p, st);
@@ -1076,6 +1142,155 @@ static int kernel_fd(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t
   return JUR_OK;
 }
 
+// State map of atm2x: the column of each state element (JUR_XMAP_*).
+static size_t state_map_h(const ctl_t *ctl, const atm_t *atm, int *xmap) {
+  const size_t n = atm2x_h(ctl, atm, nullptr, nullptr, nullptr);
+  std::vector<int> iqa(n + 1), ipa(n + 1);
+  atm2x_h(ctl, atm, nullptr, iqa.data(), ipa.data());
+  for (int i = 0; i < JUR_XMAP_LEN; i++) xmap[i] = -1;
+  const int ng = ctl->ng, nw = ctl->nw, ncl = ctl->ncl;
+  for (size_t j = 0; j < n; j++) {
+    const int q = iqa[j], ip = ipa[j];
+    int slot;
+    if (q < 2 + ng) slot = JUR_XMAP_LEV(q, ip);                              // p, T, q[ig]
+    else if (q < 2 + ng + nw) slot = JUR_XMAP_LEV(2 + NG + (q - 2 - ng), ip);  // k[iw]
+    else if (q == 2 + ng + nw) slot = JUR_XMAP_CLZ;
+    else if (q == 3 + ng + nw) slot = JUR_XMAP_CLDZ;
+    else if (q < 4 + ng + nw + ncl) slot = JUR_XMAP_CLK(q - 4 - ng - nw);
+    else if (q == 4 + ng + nw + ncl) slot = JUR_XMAP_SFZ;
+    else if (q == 5 + ng + nw + ncl) slot = JUR_XMAP_SFP;
+    else if (q == 6 + ng + nw + ncl) slot = JUR_XMAP_SFT;
+    else slot = JUR_XMAP_SFEPS(q - 7 - ng - nw - ncl);
+    xmap[slot] = (int)j;
+  }
+  return n;
+}
+
+static int check_jac_ctl(const jur_ctx *c) {
+  if (c->s.hydz >= 0)
+    return set_err(JUR_ERR_UNSUPPORTED, "adjoint Jacobian with HYDZ >= 0 (hydrostatic p(T)) is not available");
+  if (c->fov_dz.size()) return set_err(JUR_ERR_UNSUPPORTED, "adjoint Jacobian with a FOV is not available");
+  return JUR_OK;
+}
+
+// kernel() by the adjoint: one pass gives y and K; the path-only columns
+// (sfz, sfp) are finite differences as in kernel().
+static int kernel_adjoint(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m, size_t n,
+                          int precision) {
+  jur_ctx *c;
+  int rc = ctx_for(ctl, &c);
+  if (rc) return rc;
+  if ((rc = check_jac_ctl(c))) return rc;
+  if (obs->nr < 0 || obs->nr > NR) return set_err(JUR_ERR_ARG, "Too many rays!");
+  if (atm->np < 2 || atm->np > NP) return set_err(JUR_ERR_ARG, "Too many data points!");
+  std::vector<int> xmap(JUR_XMAP_LEN);
+  if (state_map_h(ctl, atm, xmap.data()) != n)
+    return set_err(JUR_ERR_ARG, "Jacobian size does not match atm2x/obs2y");
+  const int nd = ctl->nd, nr = obs->nr;
+  std::vector<int> fdcol;  // path-only columns
+  for (int slot : {JUR_XMAP_SFZ, JUR_XMAP_SFP})
+    if (xmap[slot] >= 0) {
+      fdcol.push_back(xmap[slot]);
+      xmap[slot] = -1;
+    }
+  size_t m0 = 0;
+  for (int ir = 0; ir < nr; ir++)
+    for (int id = 0; id < nd; id++) m0 += isfinite(obs->rad[id][ir]) ? 1 : 0;
+  if (m0 != m) return set_err(JUR_ERR_ARG, "Jacobian size does not match atm2x/obs2y");
+
+  const int ncol = (int)std::max<size_t>(n, 1);
+  std::vector<double> geom((size_t)7 * std::max(nr, 1));
+  std::vector<unsigned char> mask((size_t)std::max(nr, 1) * std::max(nd, 1));
+  std::vector<int> prof(std::max(nr, 1), 0);
+  for (int ir = 0; ir < nr; ir++) {
+    double *g = &geom[(size_t)7 * ir];
+    g[0] = obs->time[ir];
+    g[1] = obs->obsz[ir];
+    g[2] = obs->obslon[ir];
+    g[3] = obs->obslat[ir];
+    g[4] = obs->vpz[ir];
+    g[5] = obs->vplon[ir];
+    g[6] = obs->vplat[ir];
+    for (int id = 0; id < nd; id++) mask[(size_t)ir * nd + id] = !isfinite(obs->rad[id][ir]);
+  }
+  Staging S;
+  DevBuf<int> pm, xm;
+  DevBuf<double> kd;
+  const int zero = 0;
+  if ((rc = S.atm.upload(atm, 1))) return rc;
+  if ((rc = S.prof.upload(prof.data(), std::max(nr, 1)))) return rc;
+  if ((rc = S.geom.upload(geom.data(), geom.size()))) return rc;
+  if ((rc = S.mask.upload(mask.data(), mask.size()))) return rc;
+  if ((rc = S.rad.reserve((size_t)std::max(nr, 1) * std::max(nd, 1)))) return rc;
+  if ((rc = S.tau.reserve((size_t)std::max(nr, 1) * std::max(nd, 1)))) return rc;
+  if ((rc = S.tp.reserve((size_t)std::max(nr, 1) * 3))) return rc;
+  if ((rc = pm.upload(&zero, 1))) return rc;
+  if ((rc = xm.upload(xmap.data(), xmap.size()))) return rc;
+  if ((rc = kd.reserve((size_t)std::max(nr, 1) * std::max(nd, 1) * ncol))) return rc;
+  JacArgs ja;
+  ja.xmap = xm.p;
+  ja.nmap = 1;
+  ja.prof_map = pm.p;
+  ja.ncol = ncol;
+  ja.k = kd.p;
+  BatchView B{1, S.atm.p, nr, S.prof.p, S.geom.p};
+  if ((rc = forward_pass(c, B, precision, S.mask.p, S.rad.p, S.tau.p, S.tp.p, nullptr, 0, &ja))) return rc;
+  std::vector<double> rad((size_t)nr * nd + 1), tau((size_t)nr * nd + 1), tp((size_t)nr * 3 + 1);
+  std::vector<double> kh((size_t)nr * nd * ncol + 1);
+  HIPCHK(hipMemcpy(rad.data(), S.rad.p, (size_t)nr * nd * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(tau.data(), S.tau.p, (size_t)nr * nd * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(tp.data(), S.tp.p, (size_t)nr * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(kh.data(), kd.p, (size_t)nr * nd * ncol * sizeof(double), hipMemcpyDeviceToHost));
+  for (int ir = 0; ir < nr; ir++) {
+    for (int id = 0; id < nd; id++) {
+      obs->rad[id][ir] = rad[(size_t)ir * nd + id];
+      obs->tau[id][ir] = tau[(size_t)ir * nd + id];
+    }
+    obs->tpz[ir] = tp[(size_t)3 * ir];
+    obs->tplon[ir] = tp[(size_t)3 * ir + 1];
+    obs->tplat[ir] = tp[(size_t)3 * ir + 2];
+  }
+  size_t i = 0;
+  for (int ir = 0; ir < nr; ir++)
+    for (int id = 0; id < nd; id++)
+      if (!mask[(size_t)ir * nd + id]) {
+        memcpy(&k[i * n], &kh[((size_t)ir * nd + id) * ncol], n * sizeof(double));
+        i++;
+      }
+  if (fdcol.empty()) return JUR_OK;
+
+  // finite differences of the path-only columns (src/jurassic.c:4040-4090)
+  std::vector<double> x0(n + 1), x1(n + 1), h(fdcol.size());
+  std::vector<int> iqa(n + 1);
+  atm2x_h(ctl, atm, x0.data(), iqa.data(), nullptr);
+  const int nf = (int)fdcol.size();
+  std::vector<atm_t> atms(nf + 1);
+  memcpy(&atms[0], atm, sizeof(atm_t));
+  for (int f = 0; f < nf; f++) {
+    const int j = fdcol[f];
+    if ((rc = fd_step(ctl, iqa[j], x0[j], &h[f]))) return rc;
+    memcpy(&atms[f + 1], atm, sizeof(atm_t));
+    memcpy(x1.data(), x0.data(), n * sizeof(double));
+    x1[j] = x1[j] + h[f];
+    x2atm_h(ctl, x1.data(), &atms[f + 1]);
+  }
+  std::vector<double> fr((size_t)(nf + 1) * nr * nd + 1), ft((size_t)(nf + 1) * nr * nd + 1);
+  if ((rc = formod_shared_geometry(ctl, atms.data(), nf + 1, obs, precision, fr.data(), ft.data()))) return rc;
+  for (int f = 0; f < nf; f++) {
+    const int j = fdcol[f];
+    size_t r = 0;
+    for (int ir = 0; ir < nr; ir++)
+      for (int id = 0; id < nd; id++)
+        if (!mask[(size_t)ir * nd + id]) {
+          const double y0 = fr[(size_t)ir * nd + id];
+          const double y1 = fr[((size_t)(f + 1) * nr + ir) * nd + id];
+          k[r * n + j] = (y1 - y0) / h[f];
+          r++;
+        }
+  }
+  return JUR_OK;
+}
+
 }  // namespace jur
 
 // ------------------------------------------------------------------
@@ -1113,7 +1328,9 @@ void formod_gpu(const ctl_t *ctl, atm_t *atm, obs_t *obs) {
 int jur_gpu_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m, size_t n,
                    int method, int precision) {
   if (!ctl || !atm || !obs || (!k && m * n)) return set_err(JUR_ERR_ARG, "NULL argument");
+  if (precision != JUR_FP64 && precision != JUR_FP32) return set_err(JUR_ERR_ARG, "invalid precision");
   if (method == JUR_JAC_FD) return kernel_fd(ctl, atm, obs, k, m, n, precision);
+  if (method == JUR_JAC_ADJOINT) return kernel_adjoint(ctl, atm, obs, k, m, n, precision);
   return set_err(JUR_ERR_UNSUPPORTED, "Jacobian method %d is not available", method);
 }
 
@@ -1154,6 +1371,41 @@ int jur_ctx_formod_device(jur_ctx_t *ctx, const jur_batch_t *b, int precision, v
   BatchView B{b->nprof, b->atm, b->nray, b->ray_prof, b->ray_geom};
   return forward_pass(ctx, B, precision, b->mask, b->rad, b->tau, b->tp, nullptr,
                       (hipStream_t)stream);
+}
+
+int jur_state_map(const ctl_t *ctl, const atm_t *atm, int *xmap) {
+  if (!ctl || !atm || !xmap) return set_err(JUR_ERR_ARG, "NULL argument"), -1;
+  return (int)state_map_h(ctl, atm, xmap);
+}
+
+int jur_ctx_jacobian_device(jur_ctx_t *ctx, const jur_batch_t *b, const jur_jac_t *jac, int precision,
+                            void *stream) {
+  if (!ctx || !b || !jac) return set_err(JUR_ERR_ARG, "NULL argument");
+  if (precision != JUR_FP64 && precision != JUR_FP32) return set_err(JUR_ERR_ARG, "invalid precision");
+  if (b->nray < 0 || b->nprof < 0 || (b->nray > 0 && (!b->atm || !b->ray_prof || !b->ray_geom ||
+                                                      !b->rad || !b->tau)))
+    return set_err(JUR_ERR_ARG, "invalid batch");
+  if (jac->nmap < 0 || jac->ncol < 1 || (jac->nmap > 0 && !jac->xmap) || (b->nprof > 0 && !jac->prof_map) ||
+      (b->nray > 0 && !jac->k))
+    return set_err(JUR_ERR_ARG, "invalid Jacobian request");
+  int rc = check_jac_ctl(ctx);
+  if (rc) return rc;
+  for (int im = 0; im < jac->nmap; im++) {
+    const int *x = jac->xmap + (size_t)im * JUR_XMAP_LEN;
+    for (int i = 0; i < JUR_XMAP_LEN; i++)
+      if (x[i] < -1 || x[i] >= jac->ncol) return set_err(JUR_ERR_ARG, "state map %d: column out of range", im);
+    if (x[JUR_XMAP_SFZ] >= 0 || x[JUR_XMAP_SFP] >= 0)
+      return set_err(JUR_ERR_UNSUPPORTED, "surface altitude/pressure columns need finite differences");
+  }
+  if ((rc = ctx->d_xmap.upload(jac->xmap, (size_t)jac->nmap * JUR_XMAP_LEN))) return rc;
+  JacArgs ja;
+  ja.xmap = ctx->d_xmap.p;
+  ja.nmap = jac->nmap;
+  ja.prof_map = jac->prof_map;
+  ja.ncol = jac->ncol;
+  ja.k = jac->k;
+  BatchView B{b->nprof, b->atm, b->nray, b->ray_prof, b->ray_geom};
+  return forward_pass(ctx, B, precision, b->mask, b->rad, b->tau, b->tp, nullptr, (hipStream_t)stream, &ja);
 }
 
 int jur_ctx_last_timing(const jur_ctx_t *ctx, double *ms, int n) {

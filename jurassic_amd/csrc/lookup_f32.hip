@@ -13,41 +13,15 @@
 //    cancellation against u_i, and eps00 = eps_i + off deps/du while off stays
 //    inside the bracket (the common case; otherwise the forward bracket is
 //    searched on the absolute column density in FP64).
-//  * Output is the segment emissivity (eps00 - eps_path) / (1 - eps_path);
-//    1 marks an opaque segment (tau_path below float resolution).
+//  * Output is the path emissivity eps_path after each segment (the radiance
+//    kernel forms the segment emissivity (eps_i - eps_{i-1}) / (1 - eps_{i-1});
+//    the adjoint restarts any segment from eps_{i-1}); 1 marks an opaque
+//    path (tau_path below float resolution).
 #include "jur_device.hpp"
 
 namespace jur {
 
 #define LOSR(f, ip) LOS_AT(los, L, f, ip)
-
-// intpol_tbl_eps from a bracket record, float interpolation.
-__device__ __forceinline__ float eps_from_rec_f(const float4 &r, double x) {
-  if (x < (double)r.x) return fdiv(r.y, r.x) * (float)x;
-  if (x > (double)r.z) return 1.f - __expf(fdiv(__logf(1.f - r.w), r.z) * (float)x);
-  return r.y + fdiv(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
-}
-
-// Reference-order lookups for columns that are not monotone (bisection).
-__device__ __forceinline__ float col_eps_bis(const float4 *col, int n, double x) {
-  const float4 r0 = col[0];
-  if (x < (double)r0.x) return fdiv(r0.y, r0.x) * (float)x;
-  const float4 rl = col[n - 1];
-  if (x > (double)rl.x) return 1.f - __expf(fdiv(__logf(1.f - rl.y), rl.x) * (float)x);
-  float4 r;
-  col_bisect<0>(col, n, x, r);
-  return r.y + fdiv(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
-}
-
-__device__ __forceinline__ double col_u_bis(const float4 *col, int n, float x) {
-  const float4 r0 = col[0];
-  if (x < r0.y) return (double)(fdiv(r0.x, r0.y) * x);
-  const float4 rl = col[n - 1];
-  if (x > rl.y) return (double)(log1pf(-x) * fdiv(rl.x, __logf(1.f - rl.y)));
-  float4 r;
-  col_bisect<1>(col, n, (double)x, r);
-  return (double)r.x + (double)(fdiv(r.z - r.x, r.w - r.y) * (x - r.y));
-}
 
 template <bool CGA>
 __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, LosView L,
@@ -124,7 +98,6 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       gi1 = (float)(1.0 / (tg0[T.TT + 1] - tg0[T.TT]));
       it0 = it1 = -1;
     }
-    float e = 0.f;
     if (nt0 >= 2 && nt1 >= 2) {
       int j0 = (int)((tq - g00) * gi0);
       j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
@@ -207,11 +180,10 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         float en = a + (b - a) * wp;
         en = (en < 1.f) ? en : 1.f;
         en = (en > 0.f) ? en : 0.f;
-        e = (en >= 1.f) ? 1.f : fdiv(en - ep, 1.f - ep);
         ep = en;
       }
     }
-    out[(size_t)ip * WAVE] = e;
+    out[(size_t)ip * WAVE] = ep;
     if (ep >= 1.f) {
       // tau_path below float resolution: every later segment is opaque too
       for (ip++; ip < R.np; ip++) out[(size_t)ip * WAVE] = 1.f;

@@ -26,7 +26,9 @@ namespace jur {
 
 #define LOSR(f, ip) LOS_AT(los, L, f, ip)
 
-template <bool CGA>
+// PATH: store the path transmittance tau_path after each segment instead of
+// the segment emissivity (the Jacobian restarts any segment from it).
+template <bool CGA, bool PATH>
 __global__ __launch_bounds__(WAVE) void k_lookup_f64(ModelView M, BatchView B, LosView L,
                                                      EpsView E, const RayInfo *info, int blk0,
                                                      int nblk) {
@@ -42,7 +44,7 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f64(ModelView M, BatchView B, L
   const TabView &T = M.tab;
   const int tix = ig * nd + id;
   const int npt = T.np[tix];
-  if (npt < 30) return;  // eps = 0 for every segment; the radiance kernel knows
+  if (npt < 30) return;  // eps = 0 for every segment; the consumers know
   const RayInfo R = info[ir];
   if (R.status != JUR_OK) return;
 
@@ -169,18 +171,21 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f64(ModelView M, BatchView B, L
       }
     }
     tp *= (1 - eps);
-    out[(size_t)ip * WAVE] = eps;
+    out[(size_t)ip * WAVE] = PATH ? tp : eps;
   }
 }
 
 void launch_lookup_f64(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
-                       const RayInfo *info, int blk0, int nblk, hipStream_t s) {
+                       const RayInfo *info, int blk0, int nblk, bool path, hipStream_t s) {
   if (M.nd * M.ng == 0) return;
   const dim3 grid(padded_blocks(nblk) * M.nd * M.ng), blk(WAVE);
-  if (M.formod == 0)
-    hipLaunchKernelGGL(k_lookup_f64<true>, grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
-  else
-    hipLaunchKernelGGL(k_lookup_f64<false>, grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+  if (M.formod == 0) {
+    if (path) hipLaunchKernelGGL((k_lookup_f64<true, true>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+    else hipLaunchKernelGGL((k_lookup_f64<true, false>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+  } else {
+    if (path) hipLaunchKernelGGL((k_lookup_f64<false, true>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+    else hipLaunchKernelGGL((k_lookup_f64<false, false>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+  }
 }
 
 }  // namespace jur

@@ -121,13 +121,19 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   KSum rad;
   float tau_hi = 1.f, tau_lo = 0.f;
   float drefl = 0.f, tau_refl = 1.f;
+  float epp[NG];  // path emissivity of each emitter before the segment
+  for (int g = 0; g < NG; g++) epp[g] = 0.f;
   for (int ip = 0; ip < R.np; ip++) {
     bool opaque = false;
     float etot = 0.f;
-    for (int g = 0; g < ng; g++) {
-      if (!(gmask >> g & 1u)) continue;
-      const float e = eps_g[((size_t)g * npb + ip) * WAVE];
-      opaque |= (e >= 1.f);
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+      if (g >= ng || !(gmask >> g & 1u)) continue;
+      const float ep = eps_g[((size_t)g * npb + ip) * WAVE];
+      opaque |= (ep >= 1.f);
+      // segment emissivity 1 - (1 - ep)/(1 - epp) = (ep - epp)/(1 - epp)
+      const float e = fdiv(ep - epp[g], 1.f - epp[g]);
+      epp[g] = ep;
       etot = etot + e - etot * e;
     }
     if (opaque) break;

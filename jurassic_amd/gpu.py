@@ -41,8 +41,15 @@ EXPORTS = [
     "jur_gpu_last_error", "jur_gpu_abi_check", "formod_gpu", "jur_gpu_formod",
     "jur_gpu_formod_batch", "jur_gpu_kernel", "kernel_gpu", "jur_ctx_create",
     "jur_ctx_create_wide", "jur_ctx_destroy", "jur_ctx_nd", "jur_ctx_formod_device",
-    "jur_ctx_last_timing", "jur_ctx_last_mean_np",
+    "jur_ctx_last_timing", "jur_ctx_last_mean_np", "jur_state_map", "jur_ctx_jacobian_device",
 ]
+
+
+class jur_jac_t(C.Structure):
+    _fields_ = [
+        ("nmap", C.c_int), ("xmap", C.c_void_p), ("prof_map", C.c_void_p), ("ncol", C.c_int),
+        ("k", C.c_void_p),
+    ]
 
 
 def lib():
@@ -74,6 +81,10 @@ def lib():
         L.jur_ctx_last_timing.restype = I
         L.jur_ctx_last_mean_np.argtypes = [P]
         L.jur_ctx_last_mean_np.restype = D
+        L.jur_state_map.argtypes = [P, P, C.POINTER(I)]
+        L.jur_state_map.restype = I
+        L.jur_ctx_jacobian_device.argtypes = [P, C.POINTER(jur_batch_t), C.POINTER(jur_jac_t), I, P]
+        L.jur_ctx_jacobian_device.restype = I
         if L.jur_gpu_abi_check(C.sizeof(abi.ctl_t), C.sizeof(abi.atm_t), C.sizeof(abi.obs_t)):
             raise OSError("ABI mismatch between jurassic_amd.abi and libjurassic_gpu.so")
         _LIB = L
@@ -107,6 +118,15 @@ def kernel(ctl, atm, obs, method=abi.JUR_JAC_FD, precision=abi.JUR_FP64, m=None,
     return k
 
 
+def state_map(ctl, atm):
+    """atm2x's state map (JUR_XMAP_LEN ints) and the state size n."""
+    xm = np.zeros(abi.XMAP_LEN, dtype=np.int32)
+    n = lib().jur_state_map(C.byref(ctl), C.byref(atm), xm.ctypes.data_as(C.POINTER(C.c_int)))
+    if n < 0:
+        _check(abi.JUR_ERR_ARG)
+    return xm, n
+
+
 class Context:
     """A device context (tables and source function resident in HBM)."""
 
@@ -127,6 +147,17 @@ class Context:
         b = jur_batch_t(nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, mask_ptr, rad_ptr,
                         tau_ptr, tp_ptr)
         _check(lib().jur_ctx_formod_device(self.ptr, C.byref(b), precision, stream))
+
+    def jacobian_device(self, nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, rad_ptr, tau_ptr,
+                        xmaps, prof_map_ptr, ncol, k_ptr, tp_ptr=None, mask_ptr=None,
+                        precision=abi.JUR_FP32, stream=None):
+        """Forward model and adjoint Jacobian of a device batch; xmaps is a
+        host int32 array [nmap][XMAP_LEN]."""
+        xmaps = np.ascontiguousarray(xmaps, dtype=np.int32).reshape(-1, abi.XMAP_LEN)
+        b = jur_batch_t(nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, mask_ptr, rad_ptr,
+                        tau_ptr, tp_ptr)
+        j = jur_jac_t(xmaps.shape[0], xmaps.ctypes.data, prof_map_ptr, ncol, k_ptr)
+        _check(lib().jur_ctx_jacobian_device(self.ptr, C.byref(b), C.byref(j), precision, stream))
 
     def last_timing(self):
         """(ray tracing, table lookups, radiance accumulation) in ms."""

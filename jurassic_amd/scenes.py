@@ -83,12 +83,22 @@ def vmr_profile(gas, z):
     raise ValueError(gas)
 
 
+_GRF_FACTOR = {}
+
+
+def _grf_factor(z, sigma, corr_len):
+    key = (z.tobytes(), sigma, corr_len)
+    if key not in _GRF_FACTOR:
+        cov = sigma ** 2 * np.exp(-np.abs(z[:, None] - z[None, :]) / corr_len)
+        _GRF_FACTOR[key] = np.linalg.cholesky(cov + 1e-12 * np.eye(len(z)))
+    return _GRF_FACTOR[key]
+
+
 def grf_perturbation(iprof, z, sigma=3.0, corr_len=5.0, seed=1234):
     """Seeded Gaussian random field with covariance
     sigma^2 exp(-|dz|/corr_len) on the levels z; one stream per profile."""
     z = np.asarray(z, dtype=float)
-    cov = sigma ** 2 * np.exp(-np.abs(z[:, None] - z[None, :]) / corr_len)
-    l = np.linalg.cholesky(cov + 1e-12 * np.eye(len(z)))
+    l = _grf_factor(z, sigma, corr_len)
     rng = np.random.default_rng([seed, iprof])
     return l @ rng.standard_normal(len(z))
 

@@ -1288,7 +1288,7 @@ static int fd_step(const ctl_t *ctl, int iq, double x0, double *h) {
 }
 
 static int kernel_impl(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
-                       size_t m, size_t n, int nthreads, int central) {
+                       size_t m, size_t n, int nthreads, int central, double scale) {
   int rc = orc_formod(ctl, atm, obs);
   if (rc) return rc;
   double *x0 = malloc((n + 1) * sizeof(double));
@@ -1305,6 +1305,7 @@ static int kernel_impl(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
   for (size_t j = 0; j < n; j++) {
     double h;
     if (fd_step(ctl, iqa[j], x0[j], &h)) { bad |= 1; continue; }
+    h *= scale;
     double *x1 = malloc((n + 1) * sizeof(double));
     double *yy1 = malloc((m + 1) * sizeof(double));
     double *yy2 = malloc((m + 1) * sizeof(double));
@@ -1338,12 +1339,17 @@ static int kernel_impl(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
 /* kernel (src/jurassic.c:4004-4099). */
 int orc_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m,
                size_t n, int nthreads) {
-  return kernel_impl(ctl, atm, obs, k, m, n, nthreads, 0);
+  return kernel_impl(ctl, atm, obs, k, m, n, nthreads, 0, 1.0);
 }
 
 int orc_kernel_central(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
                        size_t m, size_t n, int nthreads) {
-  return kernel_impl(ctl, atm, obs, k, m, n, nthreads, 1);
+  return kernel_impl(ctl, atm, obs, k, m, n, nthreads, 1, 1.0);
+}
+
+int orc_kernel_scaled(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
+                      size_t m, size_t n, int nthreads, int central, double scale) {
+  return kernel_impl(ctl, atm, obs, k, m, n, nthreads, central, scale);
 }
 
 int orc_trace_ray(const ctl_t *ctl, const atm_t *atm, obs_t *obs, int ir,

@@ -110,6 +110,13 @@ int orc_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m,
 int orc_kernel_central(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
                        size_t m, size_t n, int nthreads);
 
+/* One-sided (central = 0) or central differences with the kernel() step
+   sizes times scale: small steps approach the derivative that an adjoint
+   computes (test reference). */
+int orc_kernel_scaled(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
+                      size_t m, size_t n, int nthreads, int central,
+                      double scale);
+
 /* Retrieval settings (subset of ret_t, src/retrieval.c:31-113). */
 typedef struct {
   int kernel_recomp;

@@ -43,6 +43,8 @@ def lib():
         L.orc_kernel.restype = I
         L.orc_kernel_central.argtypes = [P, P, P, dp, S, S, I]
         L.orc_kernel_central.restype = I
+        L.orc_kernel_scaled.argtypes = [P, P, P, dp, S, S, I, I, C.c_double]
+        L.orc_kernel_scaled.restype = I
         L.orc_atm2x.argtypes = [P, P, dp, ip, ip]
         L.orc_atm2x.restype = S
         L.orc_x2atm.argtypes = [P, dp, P]
@@ -110,11 +112,17 @@ def sizes(ctl, atm, obs):
     return m, n
 
 
-def kernel(ctl, atm, obs, central=False, nthreads=0):
+def kernel(ctl, atm, obs, central=False, nthreads=0, scale=None):
+    """kernel() (one-sided, the reference steps) or central differences;
+    scale multiplies the reference step sizes."""
     m, n = sizes(ctl, atm, obs)
     k = np.zeros((m, n))
-    f = lib().orc_kernel_central if central else lib().orc_kernel
-    rc = f(C.byref(ctl), C.byref(atm), C.byref(obs), _ptr(k), m, n, nthreads)
+    if scale is not None:
+        rc = lib().orc_kernel_scaled(C.byref(ctl), C.byref(atm), C.byref(obs), _ptr(k), m, n,
+                                     nthreads, int(central), float(scale))
+    else:
+        f = lib().orc_kernel_central if central else lib().orc_kernel
+        rc = f(C.byref(ctl), C.byref(atm), C.byref(obs), _ptr(k), m, n, nthreads)
     if rc:
         raise RuntimeError("orc_kernel: %s" % lib().orc_last_error().decode())
     return k

@@ -15,6 +15,20 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
+def hip_runtime():
+    """Start torch's HIP runtime (when torch is installed) before
+    libjurassic_gpu.so loads, so that the process uses one runtime for both
+    (torch brings its own copy of the HIP runtime library)."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    if torch.cuda.is_available():
+        torch.cuda.init()
+    return torch
+
+
+@pytest.fixture(scope="session")
 def oracle():
     import oracle as O
     O.lib()

@@ -25,7 +25,7 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
 @pytest.fixture(scope="module")
-def gpu():
+def gpu(hip_runtime):
     from jurassic_amd import gpu as G
     G.lib()
     return G

@@ -1,0 +1,289 @@
+"""GPU adjoint Jacobian against finite differences of the FP64 oracle.
+
+The adjoint differentiates the forward model with the line of sight held
+fixed.  With refraction off the path does not depend on the state, so the
+adjoint equals the derivative of the oracle's forward model.  Two checks:
+
+* Per state block (T, each emitter, ...), the relative L2 error against
+  central differences of the oracle with kernel()'s step sizes
+  (src/jurassic.c:4046-4068) scaled by 1/100 and 1/1000, the better of the
+  two: larger steps cross cells of the piecewise-linear tables, smaller steps
+  leave rounding noise where a column barely changes the radiance (e.g. a
+  few ppt of O3 at the ground).
+* Directional derivatives: K v for v = kernel()'s steps on all elements of
+  one block against the oracle's central difference along v.  Moving all
+  elements of a block at once changes the radiance enough that rounding
+  stays far below the bound, so this check is the tight one (measured:
+  ~1e-8 for T, CO2 and H2O, ~1e-5 for the trace gas O3).
+
+With refraction on, kernel()'s one-sided differences also move the path;
+that comparison has a looser bound (the adjoint's stated approximation).
+All GPU calls go through the C-ABI.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from jurassic_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu(hip_runtime):
+    from jurassic_amd import gpu as G
+    G.lib()
+    return G
+
+
+def _blocks(oracle, ctl, atm):
+    x = np.zeros(abi.XMAP_LEN)
+    iqa = np.zeros(abi.XMAP_LEN, dtype=np.int32)
+    n = oracle.lib().orc_atm2x(C.byref(ctl), C.byref(atm), oracle._ptr(x),
+                               iqa.ctypes.data_as(C.POINTER(C.c_int)), None)
+    return iqa[:n]
+
+
+def _block_errors(oracle, ctl, atm, k1, k0):
+    iq = _blocks(oracle, ctl, atm)
+    err = {}
+    for q in np.unique(iq):
+        sel = iq == q
+        err[int(q)] = np.linalg.norm(k1[:, sel] - k0[:, sel]) / np.linalg.norm(k0[:, sel])
+    return err
+
+
+def _adjoint(gpu, oracle, ctl, atm, obs, precision=abi.JUR_FP64):
+    m, n = oracle.sizes(ctl, atm, obs)
+    o = abi.copy_struct(obs)
+    k = gpu.kernel(ctl, abi.copy_struct(atm), o, method=abi.JUR_JAC_ADJOINT, precision=precision, m=m, n=n)
+    return k, o
+
+
+def _fd(oracle, ctl, atm, obs, scale=1e-3, central=True):
+    return oracle.kernel(ctl, abi.copy_struct(atm), abi.copy_struct(obs), central=central, scale=scale)
+
+
+def _best_block_errors(oracle, ctl, atm, obs, k1, scales=(1e-2, 1e-3)):
+    errs = [_block_errors(oracle, ctl, atm, k1, _fd(oracle, ctl, atm, obs, scale=s)) for s in scales]
+    return {q: min(e[q] for e in errs) for q in errs[0]}
+
+
+def _y(oracle, ctl, atm, obs):
+    o = abi.copy_struct(obs)
+    oracle.formod(ctl, abi.copy_struct(atm), o)
+    m, _ = oracle.sizes(ctl, atm, obs)
+    y = np.zeros(m + 1)
+    oracle.lib().orc_obs2y(C.byref(ctl), C.byref(o), oracle._ptr(y), None, None)
+    return y[:m]
+
+
+def _directional(oracle, ctl, atm, obs, k1, rel_steps=(1e-4, 1e-3)):
+    """Per block, |K v - (F(x + h v) - F(x - h v)) / 2h| / |F'v| for v = the
+    kernel() step sizes on every element of the block (one sign, so that the
+    response does not cancel), the better of two step sizes h: the smaller
+    keeps table-cell boundaries out of the step, the larger lifts the
+    difference above the rounding of levels that barely change the
+    radiance."""
+    n = k1.shape[1]
+    x = np.zeros(n + 1)
+    iqa = np.zeros(n + 1, dtype=np.int32)
+    oracle.lib().orc_atm2x(C.byref(ctl), C.byref(atm), oracle._ptr(x),
+                           iqa.ctypes.data_as(C.POINTER(C.c_int)), None)
+    x, iqa = x[:n], iqa[:n]
+    out = {}
+    for q in np.unique(iqa):
+        sel = iqa == q
+        # kernel()'s step sizes set the scale of each element
+        step = np.where(q == 1, 1.0, np.maximum(np.abs(0.01 * x), 1e-15))
+        v = np.where(sel, step, 0.0)
+        best = np.inf
+        for h in rel_steps:
+            ys = []
+            for sgn in (1, -1):
+                a = abi.copy_struct(atm)
+                xs = x + sgn * h * v
+                oracle.lib().orc_x2atm(C.byref(ctl), oracle._ptr(xs), C.byref(a))
+                ys.append(_y(oracle, ctl, a, obs))
+            fd = (ys[0] - ys[1]) / (2 * h)
+            best = min(best, np.linalg.norm(k1 @ v - fd) / np.linalg.norm(fd))
+        out[int(q)] = best
+    return out
+
+
+def _nadir(refrac=0, **kw):
+    ctl, atm, obs = scenes.nadir_jac(1, nd=8, nang=4)
+    ctl.refrac = refrac
+    for k, v in kw.items():
+        setattr(ctl, k, v)
+    return ctl, atm, obs
+
+
+def _check(err, tol, what):
+    bad = {q: e for q, e in err.items() if not e <= tol}
+    assert not bad, "%s: block errors above %g: %s (all: %s)" % (what, tol, bad, err)
+
+
+def test_adjoint_fp64_nadir(gpu, oracle):
+    ctl, atm, obs = _nadir()
+    k1, o = _adjoint(gpu, oracle, ctl, atm, obs)
+    _check(_best_block_errors(oracle, ctl, atm, obs, k1), 3e-4, "nadir FP64")
+    _check(_directional(oracle, ctl, atm, obs, k1), 3e-5, "nadir FP64 directional")
+    # the forward part of the pass is the forward model
+    o0 = abi.copy_struct(obs)
+    oracle.formod(ctl, abi.copy_struct(atm), o0)
+    r1, t1 = abi.obs_arrays(o, ctl)
+    r0, t0 = abi.obs_arrays(o0, ctl)
+    assert np.allclose(r1, r0, rtol=1e-12, atol=0) and np.allclose(t1, t0, rtol=1e-12, atol=1e-300)
+
+
+def test_adjoint_fp32_nadir(gpu, oracle):
+    ctl, atm, obs = _nadir()
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs, precision=abi.JUR_FP32)
+    _check(_best_block_errors(oracle, ctl, atm, obs, k1), 1e-3, "nadir FP32")
+    _check(_directional(oracle, ctl, atm, obs, k1), 1e-3, "nadir FP32 directional")
+
+
+def test_adjoint_bbt(gpu, oracle):
+    ctl, atm, obs = _nadir(write_bbt=1)
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs)
+    _check(_best_block_errors(oracle, ctl, atm, obs, k1), 3e-4, "nadir BT")
+    _check(_directional(oracle, ctl, atm, obs, k1), 3e-5, "nadir BT directional")
+
+
+def test_adjoint_cga(gpu, oracle):
+    ctl, atm, obs = _nadir(formod=0)
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs)
+    _check(_best_block_errors(oracle, ctl, atm, obs, k1), 3e-4, "nadir CGA")
+    _check(_directional(oracle, ctl, atm, obs, k1), 3e-5, "nadir CGA directional")
+
+
+def _limb(refrac):
+    ctl, atm, obs = scenes.limb_3gas(2, nrays=8)
+    ctl.rett_zmin, ctl.rett_zmax = -100, 100
+    for ig in range(3):
+        ctl.retq_zmin[ig], ctl.retq_zmax[ig] = -100, 100
+    ctl.refrac = refrac
+    return ctl, atm, obs
+
+
+def test_adjoint_limb(gpu, oracle):
+    ctl, atm, obs = _limb(0)
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs)
+    _check(_best_block_errors(oracle, ctl, atm, obs, k1), 3e-4, "limb FP64")
+    _check(_directional(oracle, ctl, atm, obs, k1), 3e-5, "limb FP64 directional")
+    k2, _ = _adjoint(gpu, oracle, ctl, atm, obs, precision=abi.JUR_FP32)
+    _check(_directional(oracle, ctl, atm, obs, k2), 1e-3, "limb FP32 directional")
+
+
+def test_adjoint_limb_refraction_vs_kernel(gpu, oracle):
+    # kernel()'s one-sided differences (T step 1 K) include the path's
+    # dependence on p and T through refraction and the secant across table
+    # cells; the adjoint holds the path fixed
+    ctl, atm, obs = _limb(1)
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs)
+    k0 = _fd(oracle, ctl, atm, obs, scale=1.0, central=False)
+    _check(_block_errors(oracle, ctl, atm, k1, k0), 0.1, "limb refraction vs kernel()")
+
+
+def _scalar_scene():
+    ctl, atm, obs = scenes.nadir_co2()
+    ctl.refrac = 0
+    ctl.ncl, ctl.nsf = 2, 2
+    ctl.clnu[0], ctl.clnu[1] = 600.0, 700.0
+    ctl.sfnu[0], ctl.sfnu[1] = 600.0, 700.0
+    ctl.sftype = 3
+    ctl.sfsza = 30.0
+    ctl.nw = 1
+    atm.clz, atm.cldz = 8.0, 1.5
+    atm.clk[0], atm.clk[1] = 0.02, 0.03
+    atm.sft, atm.sfeps[0], atm.sfeps[1] = 290.0, 0.9, 0.8
+    for ip in range(atm.np):
+        atm.k[0][ip] = 1e-3 * np.exp(-atm.z[ip] / 10)
+    ctl.retp_zmin, ctl.retp_zmax = -100, 100
+    ctl.rett_zmin, ctl.rett_zmax = -100, 100
+    ctl.retk_zmin[0], ctl.retk_zmax[0] = -100, 100
+    ctl.ret_clz = ctl.ret_cldz = ctl.ret_clk = 1
+    ctl.ret_sft = ctl.ret_sfeps = 1
+    return ctl, atm, obs
+
+
+def test_adjoint_pressure_extinction_clouds_surface(gpu, oracle):
+    ctl, atm, obs = _scalar_scene()
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs)
+    _check(_best_block_errors(oracle, ctl, atm, obs, k1), 3e-4, "p, T, k, clouds, surface")
+    _check(_directional(oracle, ctl, atm, obs, k1), 3e-5, "p, T, k, clouds, surface directional")
+
+
+def test_adjoint_path_columns_by_fd(gpu, oracle):
+    # surface altitude and pressure move the path: jur_gpu_kernel covers them
+    # with kernel()'s one-sided differences
+    ctl, atm, obs = _scalar_scene()
+    ctl.ret_sfz = ctl.ret_sfp = 1
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs)
+    k0 = _fd(oracle, ctl, atm, obs, scale=1.0, central=False)
+    iq = _blocks(oracle, ctl, atm)
+    ng, nw, ncl = ctl.ng, ctl.nw, ctl.ncl
+    for q in (4 + ng + nw + ncl, 5 + ng + nw + ncl):
+        sel = iq == q
+        assert sel.sum() == 1
+        assert np.allclose(k1[:, sel], k0[:, sel], rtol=1e-6, atol=1e-9 * np.abs(k0[:, sel]).max())
+
+
+def test_adjoint_masked_rows(gpu, oracle):
+    ctl, atm, obs = _nadir()
+    obs.rad[1][0] = float("nan")
+    obs.rad[5][2] = float("nan")
+    k1, o = _adjoint(gpu, oracle, ctl, atm, obs)
+    k0 = _fd(oracle, ctl, atm, obs)
+    assert k1.shape == k0.shape == (ctl.nd * obs.nr - 2, k0.shape[1])
+    _check(_directional(oracle, ctl, atm, obs, k1), 3e-5, "masked rows directional")
+    assert np.isnan(o.rad[1][0]) and np.isnan(o.rad[5][2])
+
+
+def test_adjoint_unsupported(gpu, oracle):
+    ctl, atm, obs = _nadir(hydz=0.0)
+    m, n = oracle.sizes(ctl, atm, obs)
+    with pytest.raises(gpu.JurassicError) as e:
+        gpu.kernel(ctl, atm, obs, method=abi.JUR_JAC_ADJOINT, m=m, n=n)
+    assert e.value.code == 6
+
+
+def test_device_jacobian_matches_host(gpu, oracle):
+    torch = pytest.importorskip("torch")
+    ctl, atm0, obs = _nadir()
+    atm1 = scenes.make_profile_atm(["CO2", "H2O", "O3"], iprof=7)
+    m, n = oracle.sizes(ctl, atm0, obs)
+    nr, nd = obs.nr, ctl.nd
+    ks = [_adjoint(gpu, oracle, ctl, a, obs, precision=abi.JUR_FP32)[0] for a in (atm0, atm1)]
+    xm, n2 = gpu.state_map(ctl, atm0)
+    assert n2 == n
+    dev = torch.device("cuda:0")
+    atms = (abi.atm_t * 2)(atm0, atm1)
+    a_d = torch.from_numpy(np.frombuffer(bytes(atms), dtype=np.uint8).copy()).to(dev)
+    # ray order: profile-minor within each geometry
+    prof = np.array([i % 2 for i in range(2 * nr)], dtype=np.int32)
+    geom = np.zeros((2 * nr, 7))
+    for i in range(2 * nr):
+        ir = i // 2
+        geom[i] = [obs.time[ir], obs.obsz[ir], obs.obslon[ir], obs.obslat[ir], obs.vpz[ir],
+                   obs.vplon[ir], obs.vplat[ir]]
+    prof_d = torch.from_numpy(prof).to(dev)
+    geom_d = torch.from_numpy(geom).to(dev)
+    pmap_d = torch.zeros(2, dtype=torch.int32, device=dev)
+    rad = torch.zeros(2 * nr, nd, dtype=torch.float64, device=dev)
+    tau = torch.zeros_like(rad)
+    kd = torch.full((2 * nr, nd, n), float("nan"), dtype=torch.float64, device=dev)
+    ctx = gpu.Context(ctl)
+    try:
+        ctx.jacobian_device(2, a_d.data_ptr(), 2 * nr, prof_d.data_ptr(), geom_d.data_ptr(), rad.data_ptr(),
+                            tau.data_ptr(), xm[None, :], pmap_d.data_ptr(), n, kd.data_ptr(),
+                            precision=abi.JUR_FP32)
+        torch.cuda.synchronize()
+    finally:
+        ctx.close()
+    kh = kd.cpu().numpy()
+    for p in range(2):
+        rows = kh[p::2].reshape(nr * nd, n)
+        assert np.allclose(rows, ks[p], rtol=1e-12, atol=1e-12 * np.abs(ks[p]).max())
