@@ -242,6 +242,37 @@ int jur_gpu_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
    xmap[JUR_XMAP_LEN] and returns the state size n. */
 int jur_state_map(const ctl_t *ctl, const atm_t *atm, int *xmap);
 
+/* Retrieval settings: the part of the reference's ret_t
+   (src/retrieval.c:31-113) that optimal_estimation reads. */
+typedef struct {
+  int kernel_recomp;      /* KERNEL_RECOMP */
+  int conv_itmax;         /* CONV_ITMAX */
+  double conv_dmin;       /* CONV_DMIN */
+  double err_formod[ND];  /* ERR_FORMOD [%] */
+  double err_noise[ND];   /* ERR_NOISE */
+  double err_press, err_press_cz, err_press_ch;
+  double err_temp, err_temp_cz, err_temp_ch;
+  double err_q[NG], err_q_cz[NG], err_q_ch[NG];
+  double err_k[NW], err_k_cz[NW], err_k_ch[NW];
+  double err_clz, err_cldz, err_clk[NCL];
+  double err_sfz, err_sfp, err_sft, err_sfeps[NSF];
+} jur_ret_t;
+
+/* Replaces optimal_estimation (src/retrieval.c:448-666; the file output and
+   error analysis are the caller's): n independent Levenberg-Marquardt
+   retrievals in lock-step, each one exactly the reference iteration, with
+   the model evaluations of all retrievals in shared device passes and the
+   normal equations (K^T S_eps^-1 K + S_a^-1 (1 + lambda)) solved on the
+   GPU by Cholesky.  For retrieval i: obs_meas[i], atm_apr[i] in;
+   atm_i[i], obs_i[i] out (the last evaluated state, as the reference),
+   chisq_hist[i * (conv_itmax + 1) + it] the cost of iteration it, niter[i].
+   method: JUR_JAC_FD (kernel(), one-sided differences) or JUR_JAC_ADJOINT.
+   Returns JUR_OK or an error code. */
+int jur_gpu_optimal_estimation(const jur_ret_t *ret, const ctl_t *ctl, int n,
+                               obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
+                               atm_t *atm_i, double *chisq_hist, int *niter,
+                               int method, int precision);
+
 /* ERRMSG-style wrapper of jur_gpu_kernel(..., JUR_JAC_FD, JUR_FP64). */
 void kernel_gpu(ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m,
                 size_t n);

@@ -44,6 +44,22 @@ def xmap_clk(icl):
 def xmap_sfeps(isf):
     return XMAP_SFZ + 3 + isf
 
+class ret_t(C.Structure):
+    """Retrieval settings (jur_ret_t; the part of the reference's ret_t,
+    src/retrieval.c:31-113, that optimal_estimation reads)."""
+    _fields_ = [
+        ("kernel_recomp", C.c_int), ("conv_itmax", C.c_int), ("conv_dmin", C.c_double),
+        ("err_formod", C.c_double * ND), ("err_noise", C.c_double * ND),
+        ("err_press", C.c_double), ("err_press_cz", C.c_double), ("err_press_ch", C.c_double),
+        ("err_temp", C.c_double), ("err_temp_cz", C.c_double), ("err_temp_ch", C.c_double),
+        ("err_q", C.c_double * NG), ("err_q_cz", C.c_double * NG), ("err_q_ch", C.c_double * NG),
+        ("err_k", C.c_double * NW), ("err_k_cz", C.c_double * NW), ("err_k_ch", C.c_double * NW),
+        ("err_clz", C.c_double), ("err_cldz", C.c_double), ("err_clk", C.c_double * NCL),
+        ("err_sfz", C.c_double), ("err_sfp", C.c_double), ("err_sft", C.c_double),
+        ("err_sfeps", C.c_double * NSF),
+    ]
+
+
 ERR_NAMES = {
     0: "JUR_OK", 1: "JUR_ERR_ARG", 2: "JUR_ERR_IO", 3: "JUR_ERR_LOS",
     4: "JUR_ERR_OBSERVER", 5: "JUR_ERR_HIP", 6: "JUR_ERR_UNSUPPORTED",

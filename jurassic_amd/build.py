@@ -36,6 +36,8 @@ SOURCES = [
     ("radiance_f32.hip", ["-ffp-contract=fast"]),
     ("aux_kernels.hip", ["-ffp-contract=off"]),
     ("adjoint.hip", ["-ffp-contract=fast"]),
+    ("lm_kernels.hip", ["-ffp-contract=off"]),
+    ("retrieval.cpp", ["-x", "hip", "-ffp-contract=off"]),
 ]
 
 

@@ -23,7 +23,7 @@
 #include <vector>
 
 #include "ctm_coeffs.h"
-#include "jur_internal.hpp"
+#include "jur_host.hpp"
 
 #pragma clang fp contract(off)
 
@@ -59,87 +59,15 @@ void launch_zero(void *, size_t, hipStream_t);
 // ------------------------------------------------------------------
 // Errors.
 // ------------------------------------------------------------------
-static thread_local char g_err[1024] = "";
+thread_local char g_err[1024] = "";
 
-static int set_err(int code, const char *fmt, ...) {
+int set_err(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof g_err, fmt, ap);
   va_end(ap);
   return code;
 }
-
-#define HIPCHK(x)                                                                    \
-  do {                                                                               \
-    hipError_t e_ = (x);                                                             \
-    if (e_ != hipSuccess)                                                            \
-      return set_err(JUR_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_),      \
-                     __FILE__, __LINE__);                                            \
-  } while (0)
-
-template <typename T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  int reserve(size_t m) {
-    if (m <= n && p) return JUR_OK;
-    release();
-    if (m == 0) m = 1;
-    if (hipMalloc((void **)&p, m * sizeof(T)) != hipSuccess) {
-      p = nullptr;
-      return set_err(JUR_ERR_MEM, "device allocation of %zu bytes failed", m * sizeof(T));
-    }
-    n = m;
-    return JUR_OK;
-  }
-  int upload(const T *h, size_t m) {
-    int rc = reserve(m);
-    if (rc) return rc;
-    if (m) HIPCHK(hipMemcpy(p, h, m * sizeof(T), hipMemcpyHostToDevice));
-    return JUR_OK;
-  }
-};
-
-// ------------------------------------------------------------------
-// Settings copied out of ctl_t (plus a channel list of any length).
-// ------------------------------------------------------------------
-struct Setup {
-  int ng = 0, nd = 0, nw = 1, ncl = 0, nsf = 0, sftype = 2, tblfmt = 1, formod = 1;
-  int ctm_co2 = 1, ctm_h2o = 1, ctm_n2 = 1, ctm_o2 = 1, refrac = 1, write_bbt = 0;
-  double sfsza = -999, hydz = -999, rayds = 10, raydz = 0.1;
-  std::vector<std::string> emitter;
-  std::vector<double> nu;
-  std::vector<int> window;
-  std::vector<double> clnu, sfnu;
-  std::string tblbase, fov;
-
-  std::string key() const {
-    std::string k;
-    char b[64];
-    k += tblbase + "|" + fov + "|";
-    for (auto &e : emitter) k += e + ",";
-    k += "|";
-    for (size_t i = 0; i < nu.size(); i++) {
-      snprintf(b, sizeof b, "%.17g:%d,", nu[i], window[i]);
-      k += b;
-    }
-    for (double v : clnu) { snprintf(b, sizeof b, "c%.17g,", v); k += b; }
-    for (double v : sfnu) { snprintf(b, sizeof b, "s%.17g,", v); k += b; }
-    snprintf(b, sizeof b, "|%d %d %d %d %d %d %d %d", ng, nd, nw, ncl, nsf, sftype, tblfmt, formod);
-    k += b;
-    snprintf(b, sizeof b, "|%d%d%d%d %d %d", ctm_co2, ctm_h2o, ctm_n2, ctm_o2, refrac, write_bbt);
-    k += b;
-    snprintf(b, sizeof b, "|%.17g %.17g %.17g %.17g", sfsza, hydz, rayds, raydz);
-    k += b;
-    return k;
-  }
-};
 
 static int setup_from_ctl(const ctl_t *ctl, int nd, const double *nu, const int *window,
                           Setup &s) {
@@ -332,50 +260,6 @@ static int locate_irr_h(const double *xx, int n, double x) {
 
 }  // namespace jur
 
-// ------------------------------------------------------------------
-// The device context.
-// ------------------------------------------------------------------
-struct jur_ctx {
-  jur::Setup s;
-  int device = 0;
-  // tables
-  jur::DevBuf<int> d_np, d_nt, d_nu, d_mono, d_pdir;
-  jur::DevBuf<double> d_p, d_t;
-  jur::DevBuf<float4> d_rec;
-  int TP = 2, TT = 2, TU = 2;
-  int shared = 0, ref_tix = 0;
-  size_t table_bytes = 0;
-  // model constants
-  jur::DevBuf<double> d_st, d_sr, d_clnu, d_sfnu, d_ctm;
-  jur::DevBuf<jur::ChanInfo> d_chan;
-  jur::CtmView ctm{};
-  // FOV shape
-  std::vector<double> fov_dz, fov_w;
-  jur::DevBuf<double> d_fov_dz, d_fov_w;
-  // work buffers
-  jur::DevBuf<double> d_los;
-  jur::DevBuf<char> d_eps;
-  jur::DevBuf<int> d_npb;
-  jur::DevBuf<size_t> d_eoff;
-  jur::DevBuf<jur::RayInfo> d_info;
-  jur::DevBuf<double> d_zr;
-  jur::DevBuf<unsigned long long> d_red;
-  jur::DevBuf<double> d_rad0, d_tau0;
-  jur::DevBuf<int> d_err;
-  // Jacobian work buffers
-  jur::DevBuf<double> d_kt, d_scr;
-  jur::DevBuf<size_t> d_soff;
-  jur::DevBuf<int> d_xmap, d_nef;
-  size_t los_budget = 0;  // bytes for LOS + emissivity buffers (0: from free HBM)
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double last_ms[3] = {0, 0, 0};
-  double last_mean_np = 0;
-
-  ~jur_ctx() {
-    for (auto &e : ev)
-      if (e) (void)hipEventDestroy(e);
-  }
-};
 
 namespace jur {
 
@@ -613,24 +497,10 @@ static int ctx_build(jur_ctx *c) {
 // ------------------------------------------------------------------
 // One forward pass over a device batch.
 // ------------------------------------------------------------------
-struct FovPkg {
-  const int *lo = nullptr, *hi = nullptr;  // device, per ray
-};
-
-// Jacobian request of a pass: state maps (device), K (device).
-struct JacArgs {
-  const int *xmap = nullptr;
-  int nmap = 0;
-  const int *prof_map = nullptr;
-  int ncol = 0;
-  double *k = nullptr;
-};
-
 // One pass over a device batch: the forward model, or with jac the forward
 // model and its Jacobian (adjoint sweep).
-static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
-                        const unsigned char *d_mask, double *d_rad, double *d_tau, double *d_tp,
-                        const FovPkg *fov, hipStream_t st, const JacArgs *jac = nullptr) {
+int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned char *d_mask, double *d_rad,
+                 double *d_tau, double *d_tp, const FovPkg *fov, hipStream_t st, const JacArgs *jac) {
   const Setup &s = c->s;
   ModelView M = model_view(c);
   int rc;
@@ -827,7 +697,7 @@ static int forward_pass(jur_ctx *c, const BatchView &B, int precision,
 static std::mutex g_mu;
 static std::map<std::string, std::unique_ptr<jur_ctx>> g_cache;
 
-static int ctx_for(const ctl_t *ctl, jur_ctx **out) {
+int ctx_for(const ctl_t *ctl, jur_ctx **out) {
   Setup s;
   if (ctl->nd < 0 || ctl->nd > ND) return set_err(JUR_ERR_ARG, "Set 0 <= ND <= MAX!");
   int rc = setup_from_ctl(ctl, ctl->nd, ctl->nu, ctl->window, s);
@@ -968,9 +838,8 @@ static int formod_packages(const ctl_t *ctl, atm_t *atm, obs_t *obs, int n, int 
 // 5896-5946, 4174-4197) and the finite-difference Jacobian
 // (kernel, src/jurassic.c:4004-4099).
 // ------------------------------------------------------------------
-enum { IDXP = 0, IDXT = 1 };
 
-static size_t atm2x_h(const ctl_t *ctl, const atm_t *atm, double *x, int *iqa, int *ipa) {
+size_t atm2x_h(const ctl_t *ctl, const atm_t *atm, double *x, int *iqa, int *ipa) {
   size_t n = 0;
   auto add = [&](double v, int q, int ip) {
     if (x) x[n] = v;
@@ -1003,7 +872,7 @@ static size_t atm2x_h(const ctl_t *ctl, const atm_t *atm, double *x, int *iqa, i
   return n;
 }
 
-static void x2atm_h(const ctl_t *ctl, const double *x, atm_t *atm) {
+void x2atm_h(const ctl_t *ctl, const double *x, atm_t *atm) {
   size_t n = 0;
   const int ng = ctl->ng, nw = ctl->nw;
   for (int ip = 0; ip < atm->np; ip++)
@@ -1027,7 +896,7 @@ static void x2atm_h(const ctl_t *ctl, const double *x, atm_t *atm) {
     for (int isf = 0; isf < ctl->nsf; isf++) atm->sfeps[isf] = x[n++];
 }
 
-static size_t obs2y_h(const ctl_t *ctl, const obs_t *obs, double *y) {
+size_t obs2y_h(const ctl_t *ctl, const obs_t *obs, double *y) {
   size_t m = 0;
   for (int ir = 0; ir < obs->nr; ir++)
     for (int id = 0; id < ctl->nd; id++)
@@ -1039,7 +908,7 @@ static size_t obs2y_h(const ctl_t *ctl, const obs_t *obs, double *y) {
 }
 
 // Perturbation size of kernel() (src/jurassic.c:4046-4068).
-static int fd_step(const ctl_t *ctl, int iq, double x0, double *h) {
+int fd_step(const ctl_t *ctl, int iq, double x0, double *h) {
   const int ng = ctl->ng, nw = ctl->nw, ncl = ctl->ncl, nsf = ctl->nsf;
   if (iq == IDXP) *h = std::max(fabs(0.01 * x0), 1e-7);
   else if (iq == IDXT) *h = 1.0;
@@ -1142,8 +1011,12 @@ static int kernel_fd(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t
   return JUR_OK;
 }
 
+int kernel_fd_host(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m, size_t n, int precision) {
+  return kernel_fd(ctl, atm, obs, k, m, n, precision);
+}
+
 // State map of atm2x: the column of each state element (JUR_XMAP_*).
-static size_t state_map_h(const ctl_t *ctl, const atm_t *atm, int *xmap) {
+size_t state_map_h(const ctl_t *ctl, const atm_t *atm, int *xmap) {
   const size_t n = atm2x_h(ctl, atm, nullptr, nullptr, nullptr);
   std::vector<int> iqa(n + 1), ipa(n + 1);
   atm2x_h(ctl, atm, nullptr, iqa.data(), ipa.data());
@@ -1166,7 +1039,7 @@ static size_t state_map_h(const ctl_t *ctl, const atm_t *atm, int *xmap) {
   return n;
 }
 
-static int check_jac_ctl(const jur_ctx *c) {
+int check_jac_ctl(const jur_ctx *c) {
   if (c->s.hydz >= 0)
     return set_err(JUR_ERR_UNSUPPORTED, "adjoint Jacobian with HYDZ >= 0 (hydrostatic p(T)) is not available");
   if (c->fov_dz.size()) return set_err(JUR_ERR_UNSUPPORTED, "adjoint Jacobian with a FOV is not available");

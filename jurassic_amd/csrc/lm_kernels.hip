@@ -1,0 +1,299 @@
+// Dense algebra of the Levenberg-Marquardt optimal estimation
+// (optimal_estimation, src/retrieval.c:448-666) for a batch of independent
+// retrievals.  Each retrieval r has a state of n_r <= NMAX elements and m_r
+// measurements; its matrices live in device arenas at offsets given by
+// LmDesc.  The kernels take a list of the retrievals that need the step, so
+// retrievals at different stages of their iteration share launches.
+//
+//  k_lm_gather   K rows of a retrieval from a pass's K [ray][channel][ncol]
+//                into a compact m x n matrix, rows scaled by 1/sigma_eps
+//                (rows of masked measurements are zero)
+//  k_lm_normal   C = (S^-1/2 K)^T (S^-1/2 K), lower triangle by 64x64 LDS
+//                tiles, mirrored
+//  k_lm_cholinv  S_a^-1 (matrix_invert, src/retrieval.c:373-399): Cholesky,
+//                then one column of the inverse per thread
+//  k_lm_rhs      b = K^T S_eps^-1 dy - S_a^-1 dx
+//  k_lm_solve    A = S_a^-1 (1 + lambda) + C, Cholesky A = L L^T, L L^T s = b
+//  k_lm_quad     dx^T S_a^-1 dx (the a priori part of cost_function,
+//                src/retrieval.c:340-369)
+//
+// Cholesky is the right-looking column algorithm in global memory (the
+// matrices stay in L2: n <= 324 is 0.8 MB); every step is one barrier.
+#include "jur_device.hpp"
+
+namespace jur {
+
+struct LmDesc {
+  int n, m;          // state and measurement sizes
+  int ray0, nray;    // rays of the retrieval in the pass (K rows ray0*nd ..)
+  size_t o_nn;       // offset of the n x n matrices (S_a^-1, C, A)
+  size_t o_k;        // offset of the compact m x n K
+  size_t o_n;        // offset of n-vectors
+  size_t o_m;        // offset of m-vectors
+};
+
+constexpr int LM_T = 256;
+
+// Compact scaled K: kc[i][j] = K[row(i)][j] * sig_inv[i]; rows of masked
+// measurements (sig_inv = 0) are zero.  Rows are (ray, channel) in order.
+__global__ __launch_bounds__(LM_T) void k_lm_gather(const LmDesc *D, const int *list, const double *kpass,
+                                                    int nd, int ncol, const double *sig_inv, double *kc) {
+  const LmDesc d = D[list[blockIdx.y]];
+  const int i = blockIdx.x;  // measurement row (ray-major, channel-minor)
+  if (i >= d.nray * nd) return;
+  const double s = sig_inv[d.o_m + i];
+  const double *src = kpass + ((size_t)d.ray0 * nd + i) * ncol;
+  double *dst = kc + d.o_k + (size_t)i * d.n;
+  for (int j = threadIdx.x; j < d.n; j += LM_T) dst[j] = (s != 0) ? src[j] * s : 0.0;
+}
+
+// C = Kc^T Kc: one workgroup per 64x64 tile (ti >= tj), 256 threads with a
+// 4x4 block each, rows streamed through LDS 32 at a time.
+__global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *list, const double *kc,
+                                                    double *cmat) {
+  __shared__ double a[32][65], b[32][65];
+  const LmDesc d = D[list[blockIdx.y]];
+  const int nt = (d.n + 63) / 64;
+  // lower-triangular tile index -> (ti, tj)
+  int t = blockIdx.x, ti = 0;
+  while (t > ti) {
+    t -= ti + 1;
+    ti++;
+  }
+  const int tj = t;
+  if (ti >= nt) return;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; p++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[p][q] = 0;
+  const double *K = kc + d.o_k;
+  const int i0 = ti * 64, j0 = tj * 64;
+  for (int r0 = 0; r0 < d.m; r0 += 32) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 64; e += LM_T) {
+      const int rr = e >> 6, cc = e & 63;
+      const int r = r0 + rr;
+      a[rr][cc] = (r < d.m && i0 + cc < d.n) ? K[(size_t)r * d.n + i0 + cc] : 0.0;
+      b[rr][cc] = (r < d.m && j0 + cc < d.n) ? K[(size_t)r * d.n + j0 + cc] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = 0; rr < 32; rr++) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int p = 0; p < 4; p++) av[p] = a[rr][ty * 4 + p];
+#pragma unroll
+      for (int q = 0; q < 4; q++) bv[q] = b[rr][tx * 4 + q];
+#pragma unroll
+      for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[p][q] += av[p] * bv[q];
+    }
+  }
+  double *C = cmat + d.o_nn;
+#pragma unroll
+  for (int p = 0; p < 4; p++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = i0 + ty * 4 + p, j = j0 + tx * 4 + q;
+      if (i < d.n && j < d.n) {
+        C[(size_t)i * d.n + j] = acc[p][q];
+        C[(size_t)j * d.n + i] = acc[p][q];
+      }
+    }
+}
+
+// In-place Cholesky A = L L^T of an n x n row-major matrix (lower triangle
+// of a holds L; the upper triangle is left as it was).  Returns false if a
+// pivot is not positive.  Called by all threads of the workgroup.
+__device__ bool chol_wg(double *a, int n, int *bad) {
+  for (int j = 0; j < n; j++) {
+    if (threadIdx.x == 0) {
+      const double s = a[(size_t)j * n + j];
+      if (!(s > 0)) *bad = 1;
+      a[(size_t)j * n + j] = sqrt(s);
+    }
+    __syncthreads();
+    if (*bad) return false;
+    const double l = a[(size_t)j * n + j];
+    for (int i = j + 1 + threadIdx.x; i < n; i += LM_T) a[(size_t)i * n + j] /= l;
+    __syncthreads();
+    // trailing update of the lower triangle, rows i > j, columns j < k <= i
+    const int w = n - j - 1;
+    const long tot = (long)w * (w + 1) / 2;
+    for (long e = threadIdx.x; e < tot; e += LM_T) {
+      // e -> (ii, kk) with 0 <= kk <= ii < w
+      int ii = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+      while ((long)ii * (ii + 1) / 2 > e) ii--;
+      while ((long)(ii + 1) * (ii + 2) / 2 <= e) ii++;
+      const int kk = (int)(e - (long)ii * (ii + 1) / 2);
+      const int i = j + 1 + ii, k = j + 1 + kk;
+      a[(size_t)i * n + k] -= a[(size_t)i * n + j] * a[(size_t)k * n + j];
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
+// Solve L L^T x = b in place in x (x holds b on entry); all threads.
+__device__ void chol_solve_wg(const double *l, int n, double *x) {
+  for (int j = 0; j < n; j++) {
+    if (threadIdx.x == 0) x[j] /= l[(size_t)j * n + j];
+    __syncthreads();
+    const double xj = x[j];
+    for (int i = j + 1 + threadIdx.x; i < n; i += LM_T) x[i] -= l[(size_t)i * n + j] * xj;
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; j--) {
+    if (threadIdx.x == 0) x[j] /= l[(size_t)j * n + j];
+    __syncthreads();
+    const double xj = x[j];
+    for (int i = threadIdx.x; i < j; i += LM_T) x[i] -= l[(size_t)j * n + i] * xj;
+    __syncthreads();
+  }
+}
+
+// S_a^-1: a holds S_a on entry and is overwritten by its Cholesky factor;
+// inv receives the inverse.  A diagonal S_a is inverted elementwise, as the
+// reference does.  status[r] = 1 if S_a is not positive definite.
+__global__ __launch_bounds__(LM_T) void k_lm_cholinv(const LmDesc *D, const int *list, double *a_arena,
+                                                     double *inv_arena, int *status) {
+  __shared__ int bad, offdiag;
+  const int r = list[blockIdx.x];
+  const LmDesc d = D[r];
+  double *a = a_arena + d.o_nn;
+  double *inv = inv_arena + d.o_nn;
+  const int n = d.n;
+  if (threadIdx.x == 0) bad = offdiag = 0;
+  __syncthreads();
+  for (long e = threadIdx.x; e < (long)n * n; e += LM_T)
+    if (e / n != e % n && a[e] != 0) offdiag = 1;
+  __syncthreads();
+  if (!offdiag) {
+    for (long e = threadIdx.x; e < (long)n * n; e += LM_T) inv[e] = (e / n == e % n) ? 1 / a[e] : 0.0;
+    if (threadIdx.x == 0) status[r] = 0;
+    return;
+  }
+  if (!chol_wg(a, n, &bad)) {
+    if (threadIdx.x == 0) status[r] = 1;
+    return;
+  }
+  // column c of the inverse: L y = e_c, L^T z = y, one thread per column
+  for (int c = threadIdx.x; c < n; c += LM_T) {
+    // y (forward), stored in column c of inv, rows >= c
+    for (int i = 0; i < n; i++) {
+      double v = (i == c) ? 1.0 : 0.0;
+      if (i > c)
+        for (int k = c; k < i; k++) v -= a[(size_t)i * n + k] * inv[(size_t)k * n + c];
+      else if (i < c)
+        v = 0.0;
+      inv[(size_t)i * n + c] = (i >= c) ? v / a[(size_t)i * n + i] : 0.0;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double v = inv[(size_t)i * n + c];
+      for (int k = i + 1; k < n; k++) v -= a[(size_t)k * n + i] * inv[(size_t)k * n + c];
+      inv[(size_t)i * n + c] = v / a[(size_t)i * n + i];
+    }
+  }
+  if (threadIdx.x == 0) status[r] = 0;
+}
+
+// b = Kc^T (sig_inv * dy) - S_a^-1 dx  (Kc is already scaled by sig_inv).
+__global__ __launch_bounds__(LM_T) void k_lm_rhs(const LmDesc *D, const int *list, const double *kc,
+                                                 const double *sig_inv, const double *dy, const double *sainv,
+                                                 const double *dx, double *b) {
+  const LmDesc d = D[list[blockIdx.y]];
+  const int j = blockIdx.x * LM_T + threadIdx.x;
+  if (j >= d.n) return;
+  const double *K = kc + d.o_k;
+  double v = 0;
+  for (int i = 0; i < d.m; i++) v += K[(size_t)i * d.n + j] * (sig_inv[d.o_m + i] * dy[d.o_m + i]);
+  const double *S = sainv + d.o_nn;
+  double u = 0;
+  for (int k = 0; k < d.n; k++) u += S[(size_t)j * d.n + k] * dx[d.o_n + k];
+  b[d.o_n + j] = -1.0 * u + v;
+}
+
+// A = S_a^-1 (1 + lmpar) + C; A = L L^T; step = A^-1 b.
+__global__ __launch_bounds__(LM_T) void k_lm_solve(const LmDesc *D, const int *list, const double *sainv,
+                                                   const double *cmat, const double *lmpar, const double *b,
+                                                   double *a_arena, double *step, int *status) {
+  __shared__ int bad;
+  const int r = list[blockIdx.x];
+  const LmDesc d = D[r];
+  const int n = d.n;
+  const double lp = lmpar[r];
+  const double *S = sainv + d.o_nn, *Cm = cmat + d.o_nn;
+  double *A = a_arena + d.o_nn;
+  if (threadIdx.x == 0) bad = 0;
+  for (long e = threadIdx.x; e < (long)n * n; e += LM_T) A[e] = S[e] * (1 + lp) + Cm[e];
+  double *x = step + d.o_n;
+  for (int i = threadIdx.x; i < n; i += LM_T) x[i] = b[d.o_n + i];
+  __syncthreads();
+  if (!chol_wg(A, n, &bad)) {
+    if (threadIdx.x == 0) status[r] = 1;
+    return;
+  }
+  chol_solve_wg(A, n, x);
+  if (threadIdx.x == 0) status[r] = 0;
+}
+
+// q[r] = dx^T S_a^-1 dx.
+__global__ __launch_bounds__(LM_T) void k_lm_quad(const LmDesc *D, const int *list, const double *sainv,
+                                                  const double *dx, double *q) {
+  __shared__ double part[LM_T];
+  const int r = list[blockIdx.x];
+  const LmDesc d = D[r];
+  const double *S = sainv + d.o_nn;
+  const double *x = dx + d.o_n;
+  double acc = 0;
+  for (int i = threadIdx.x; i < d.n; i += LM_T) {
+    double v = 0;
+    for (int k = 0; k < d.n; k++) v += S[(size_t)i * d.n + k] * x[k];
+    acc += x[i] * v;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = LM_T / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) q[r] = part[0];
+}
+
+// ---------------- launchers ----------------
+void launch_lm_gather(const LmDesc *D, const int *list, int cnt, int max_rows, const double *kpass, int nd,
+                      int ncol, const double *sig_inv, double *kc, hipStream_t s) {
+  if (cnt == 0 || max_rows == 0) return;
+  hipLaunchKernelGGL(k_lm_gather, dim3(max_rows, cnt), dim3(LM_T), 0, s, D, list, kpass, nd, ncol, sig_inv, kc);
+}
+void launch_lm_normal(const LmDesc *D, const int *list, int cnt, int max_n, const double *kc, double *cmat,
+                      hipStream_t s) {
+  if (cnt == 0) return;
+  const int nt = (max_n + 63) / 64;
+  hipLaunchKernelGGL(k_lm_normal, dim3(nt * (nt + 1) / 2, cnt), dim3(LM_T), 0, s, D, list, kc, cmat);
+}
+void launch_lm_cholinv(const LmDesc *D, const int *list, int cnt, double *a, double *inv, int *status,
+                       hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_lm_cholinv, dim3(cnt), dim3(LM_T), 0, s, D, list, a, inv, status);
+}
+void launch_lm_rhs(const LmDesc *D, const int *list, int cnt, int max_n, const double *kc, const double *sig_inv,
+                   const double *dy, const double *sainv, const double *dx, double *b, hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_lm_rhs, dim3((max_n + LM_T - 1) / LM_T, cnt), dim3(LM_T), 0, s, D, list, kc, sig_inv, dy,
+                     sainv, dx, b);
+}
+void launch_lm_solve(const LmDesc *D, const int *list, int cnt, const double *sainv, const double *cmat,
+                     const double *lmpar, const double *b, double *a, double *step, int *status, hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_lm_solve, dim3(cnt), dim3(LM_T), 0, s, D, list, sainv, cmat, lmpar, b, a, step, status);
+}
+void launch_lm_quad(const LmDesc *D, const int *list, int cnt, const double *sainv, const double *dx, double *q,
+                    hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_lm_quad, dim3(cnt), dim3(LM_T), 0, s, D, list, sainv, dx, q);
+}
+
+}  // namespace jur

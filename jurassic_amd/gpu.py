@@ -42,6 +42,7 @@ EXPORTS = [
     "jur_gpu_formod_batch", "jur_gpu_kernel", "kernel_gpu", "jur_ctx_create",
     "jur_ctx_create_wide", "jur_ctx_destroy", "jur_ctx_nd", "jur_ctx_formod_device",
     "jur_ctx_last_timing", "jur_ctx_last_mean_np", "jur_state_map", "jur_ctx_jacobian_device",
+    "jur_gpu_optimal_estimation",
 ]
 
 
@@ -85,6 +86,8 @@ def lib():
         L.jur_state_map.restype = I
         L.jur_ctx_jacobian_device.argtypes = [P, C.POINTER(jur_batch_t), C.POINTER(jur_jac_t), I, P]
         L.jur_ctx_jacobian_device.restype = I
+        L.jur_gpu_optimal_estimation.argtypes = [P, P, I, P, P, P, P, C.POINTER(D), C.POINTER(I), I, I]
+        L.jur_gpu_optimal_estimation.restype = I
         if L.jur_gpu_abi_check(C.sizeof(abi.ctl_t), C.sizeof(abi.atm_t), C.sizeof(abi.obs_t)):
             raise OSError("ABI mismatch between jurassic_amd.abi and libjurassic_gpu.so")
         _LIB = L
@@ -116,6 +119,34 @@ def kernel(ctl, atm, obs, method=abi.JUR_JAC_FD, precision=abi.JUR_FP64, m=None,
     _check(lib().jur_gpu_kernel(C.byref(ctl), C.byref(atm), C.byref(obs),
                                 k.ctypes.data_as(C.POINTER(C.c_double)), m, n, method, precision))
     return k
+
+
+def optimal_estimation(ret, ctl, obs_meas, atm_apr, method=abi.JUR_JAC_ADJOINT,
+                       precision=abi.JUR_FP64):
+    """optimal_estimation on the GPU for one retrieval (obs_t, atm_t) or a
+    list of them.  Returns (atm_i, obs_i, chisq_hist, niter), each a list
+    when lists were given."""
+    single = not isinstance(obs_meas, (list, tuple))
+    obss = [obs_meas] if single else list(obs_meas)
+    atms = [atm_apr] if single else list(atm_apr)
+    n = len(obss)
+    om = (abi.obs_t * n)(*obss)
+    aa = (abi.atm_t * n)(*atms)
+    oi = (abi.obs_t * n)()
+    ai = (abi.atm_t * n)()
+    hist = np.zeros(n * (ret.conv_itmax + 1))
+    nit = (C.c_int * n)()
+    _check(lib().jur_gpu_optimal_estimation(
+        C.byref(ret), C.byref(ctl), n, C.cast(om, C.c_void_p), C.cast(oi, C.c_void_p), C.cast(aa, C.c_void_p),
+        C.cast(ai, C.c_void_p), hist.ctypes.data_as(C.POINTER(C.c_double)), nit, method, precision))
+    hist = hist.reshape(n, ret.conv_itmax + 1)
+    out_a = [abi.copy_struct(ai[i]) for i in range(n)]
+    out_o = [abi.copy_struct(oi[i]) for i in range(n)]
+    out_h = [hist[i, :nit[i] + 1].copy() for i in range(n)]
+    out_n = [nit[i] for i in range(n)]
+    if single:
+        return out_a[0], out_o[0], out_h[0], out_n[0]
+    return out_a, out_o, out_h, out_n
 
 
 def state_map(ctl, atm):

@@ -154,19 +154,12 @@ def srcfunc_table(ctl):
     return st, sr.reshape(abi.TBLNS, ctl.nd)
 
 
-class orc_ret_t(C.Structure):
-    from jurassic_amd.abi import ND as _ND, NG as _NG, NW as _NW, NCL as _NCL, NSF as _NSF
-    _fields_ = [
-        ("kernel_recomp", C.c_int), ("conv_itmax", C.c_int), ("conv_dmin", C.c_double),
-        ("err_formod", C.c_double * _ND), ("err_noise", C.c_double * _ND),
-        ("err_press", C.c_double), ("err_press_cz", C.c_double), ("err_press_ch", C.c_double),
-        ("err_temp", C.c_double), ("err_temp_cz", C.c_double), ("err_temp_ch", C.c_double),
-        ("err_q", C.c_double * _NG), ("err_q_cz", C.c_double * _NG), ("err_q_ch", C.c_double * _NG),
-        ("err_k", C.c_double * _NW), ("err_k_cz", C.c_double * _NW), ("err_k_ch", C.c_double * _NW),
-        ("err_clz", C.c_double), ("err_cldz", C.c_double), ("err_clk", C.c_double * _NCL),
-        ("err_sfz", C.c_double), ("err_sfp", C.c_double), ("err_sft", C.c_double),
-        ("err_sfeps", C.c_double * _NSF),
-    ]
+def _ret_t():
+    from jurassic_amd.abi import ret_t
+    return ret_t
+
+
+orc_ret_t = _ret_t()  # the oracle's orc_ret_t has the layout of jur_ret_t
 
 
 FWD_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
