@@ -18,6 +18,7 @@
 #define JURASSIC_GPU_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 /* ------------------------------------------------------------------
    Capacities (same names and defaults as src/jurassic.h:346-444).
@@ -272,6 +273,69 @@ int jur_gpu_optimal_estimation(const jur_ret_t *ret, const ctl_t *ctl, int n,
                                obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
                                atm_t *atm_i, double *chisq_hist, int *niter,
                                int method, int precision);
+
+/* ------------------------------------------------------------------
+   Request server.  Forward models, Jacobians and retrievals are submitted
+   at any time from any thread; a worker thread batches everything in
+   flight into shared device passes (continuous batching: requests that
+   arrive while a pass runs join the next one, admitted in arrival order
+   while their device memory fits the budget).  No counterpart in the
+   reference, whose programs run one job at a time.
+   ------------------------------------------------------------------ */
+#define JUR_REQ_FORWARD 0     /* formod */
+#define JUR_REQ_JACOBIAN 1    /* formod and kernel */
+#define JUR_REQ_RETRIEVAL 2   /* optimal_estimation */
+#define JUR_ERR_TIMEOUT 9     /* jur_server_wait: not finished in time */
+
+typedef struct jur_server jur_server_t;
+
+typedef struct {
+  int method;             /* JUR_JAC_ADJOINT or JUR_JAC_FD */
+  int precision;          /* JUR_FP32 or JUR_FP64 */
+  size_t max_bytes;       /* device memory for requests in flight (0: 16 GiB) */
+  int max_batch;          /* most requests in flight (0: no limit) */
+} jur_server_opts_t;
+
+typedef struct {
+  int kind;               /* JUR_REQ_* */
+  const atm_t *atm;       /* the state (retrieval: the a priori) */
+  const obs_t *obs;       /* geometry; retrieval: also the measurement.
+                             Non-finite rad entries are masked as in formod */
+  /* state windows of Jacobian and retrieval requests (RET*_ZMIN/ZMAX,
+     src/jurassic.c:4622-4633): the state vector is that of atm2x */
+  double retp_zmin, retp_zmax, rett_zmin, rett_zmax;
+  double retq_zmin[NG], retq_zmax[NG];
+  double retk_zmin[NW], retk_zmax[NW];
+} jur_request_t;
+
+typedef struct {
+  uint64_t id;
+  int kind;
+  int status;             /* JUR_OK or the request's error code */
+  char message[256];
+  double t_submit, t_start, t_done;  /* seconds on a steady clock */
+  int n, m;               /* state and measurement sizes */
+  int niter;              /* retrieval iterations */
+} jur_result_t;
+
+/* A server for one forward model (ctl) and one set of retrieval settings. */
+int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret,
+                      const jur_server_opts_t *opt, jur_server_t **srv);
+void jur_server_destroy(jur_server_t *srv);
+/* Copies the request's inputs and queues it; *id names it. */
+int jur_server_submit(jur_server_t *srv, const jur_request_t *req,
+                      uint64_t *id);
+/* Waits until request id is finished (timeout_s < 0: no limit). */
+int jur_server_wait(jur_server_t *srv, uint64_t id, double timeout_s,
+                    jur_result_t *res);
+/* Copies a finished request's outputs and releases it: obs (the model at
+   the final state), atm (retrieval: the final state), k (Jacobian: m x n
+   row-major, rows of obs2y), chisq_hist (retrieval: niter + 1 costs).
+   Any output may be NULL.  Returns the request's status. */
+int jur_server_fetch(jur_server_t *srv, uint64_t id, obs_t *obs, atm_t *atm,
+                     double *k, double *chisq_hist);
+/* Device passes run so far and the most requests in flight at once. */
+int jur_server_stats(const jur_server_t *srv, long *rounds, long *max_active);
 
 /* ERRMSG-style wrapper of jur_gpu_kernel(..., JUR_JAC_FD, JUR_FP64). */
 void kernel_gpu(ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m,

@@ -60,10 +60,37 @@ class ret_t(C.Structure):
     ]
 
 
+JUR_REQ_FORWARD, JUR_REQ_JACOBIAN, JUR_REQ_RETRIEVAL = 0, 1, 2
+JUR_ERR_TIMEOUT = 9
+
+
+class server_opts_t(C.Structure):
+    _fields_ = [("method", C.c_int), ("precision", C.c_int), ("max_bytes", C.c_size_t),
+                ("max_batch", C.c_int)]
+
+
+class request_t(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("atm", C.c_void_p), ("obs", C.c_void_p),
+        ("retp_zmin", C.c_double), ("retp_zmax", C.c_double),
+        ("rett_zmin", C.c_double), ("rett_zmax", C.c_double),
+        ("retq_zmin", C.c_double * NG), ("retq_zmax", C.c_double * NG),
+        ("retk_zmin", C.c_double * NW), ("retk_zmax", C.c_double * NW),
+    ]
+
+
+class result_t(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64), ("kind", C.c_int), ("status", C.c_int), ("message", C.c_char * 256),
+        ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+        ("n", C.c_int), ("m", C.c_int), ("niter", C.c_int),
+    ]
+
+
 ERR_NAMES = {
     0: "JUR_OK", 1: "JUR_ERR_ARG", 2: "JUR_ERR_IO", 3: "JUR_ERR_LOS",
     4: "JUR_ERR_OBSERVER", 5: "JUR_ERR_HIP", 6: "JUR_ERR_UNSUPPORTED",
-    7: "JUR_ERR_MEM", 8: "JUR_ERR_FOV",
+    7: "JUR_ERR_MEM", 8: "JUR_ERR_FOV", 9: "JUR_ERR_TIMEOUT",
 }
 
 

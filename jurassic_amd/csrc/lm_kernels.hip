@@ -1,9 +1,9 @@
 // Dense algebra of the Levenberg-Marquardt optimal estimation
 // (optimal_estimation, src/retrieval.c:448-666) for a batch of independent
-// retrievals.  Each retrieval r has a state of n_r <= NMAX elements and m_r
-// measurements; its matrices live in device arenas at offsets given by
-// LmDesc.  The kernels take a list of the retrievals that need the step, so
-// retrievals at different stages of their iteration share launches.
+// retrievals.  Retrieval r has a state of n_r elements and m_r measurement
+// rows; LmDesc holds its sizes and device buffers.  The kernels take a list
+// of the retrievals that need the step, so retrievals at different stages
+// of their iteration share launches.
 //
 //  k_lm_gather   K rows of a retrieval from a pass's K [ray][channel][ncol]
 //                into a compact m x n matrix, rows scaled by 1/sigma_eps
@@ -24,12 +24,12 @@
 namespace jur {
 
 struct LmDesc {
-  int n, m;          // state and measurement sizes
-  int ray0, nray;    // rays of the retrieval in the pass (K rows ray0*nd ..)
-  size_t o_nn;       // offset of the n x n matrices (S_a^-1, C, A)
-  size_t o_k;        // offset of the compact m x n K
-  size_t o_n;        // offset of n-vectors
-  size_t o_m;        // offset of m-vectors
+  int n, m;           // state size, measurement rows (rays x channels)
+  int ray0, nray;     // rays of the retrieval in the current pass
+  double *sainv, *cmat, *a;  // n x n: S_a^-1, K^T S_eps^-1 K, work
+  double *k;          // m x n: K scaled by 1/sigma_eps
+  double *sig, *dy;   // m: 1/sigma_eps (0: masked), y_meas - y
+  double *dx, *b, *step;     // n
 };
 
 constexpr int LM_T = 256;
@@ -37,20 +37,19 @@ constexpr int LM_T = 256;
 // Compact scaled K: kc[i][j] = K[row(i)][j] * sig_inv[i]; rows of masked
 // measurements (sig_inv = 0) are zero.  Rows are (ray, channel) in order.
 __global__ __launch_bounds__(LM_T) void k_lm_gather(const LmDesc *D, const int *list, const double *kpass,
-                                                    int nd, int ncol, const double *sig_inv, double *kc) {
+                                                    int nd, int ncol) {
   const LmDesc d = D[list[blockIdx.y]];
   const int i = blockIdx.x;  // measurement row (ray-major, channel-minor)
   if (i >= d.nray * nd) return;
-  const double s = sig_inv[d.o_m + i];
+  const double s = d.sig[i];
   const double *src = kpass + ((size_t)d.ray0 * nd + i) * ncol;
-  double *dst = kc + d.o_k + (size_t)i * d.n;
+  double *dst = d.k + (size_t)i * d.n;
   for (int j = threadIdx.x; j < d.n; j += LM_T) dst[j] = (s != 0) ? src[j] * s : 0.0;
 }
 
 // C = Kc^T Kc: one workgroup per 64x64 tile (ti >= tj), 256 threads with a
 // 4x4 block each, rows streamed through LDS 32 at a time.
-__global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *list, const double *kc,
-                                                    double *cmat) {
+__global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *list) {
   __shared__ double a[32][65], b[32][65];
   const LmDesc d = D[list[blockIdx.y]];
   const int nt = (d.n + 63) / 64;
@@ -68,7 +67,7 @@ __global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *
   for (int p = 0; p < 4; p++)
 #pragma unroll
     for (int q = 0; q < 4; q++) acc[p][q] = 0;
-  const double *K = kc + d.o_k;
+  const double *K = d.k;
   const int i0 = ti * 64, j0 = tj * 64;
   for (int r0 = 0; r0 < d.m; r0 += 32) {
     __syncthreads();
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *
         for (int q = 0; q < 4; q++) acc[p][q] += av[p] * bv[q];
     }
   }
-  double *C = cmat + d.o_nn;
+  double *C = d.cmat;
 #pragma unroll
   for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -157,13 +156,12 @@ __device__ void chol_solve_wg(const double *l, int n, double *x) {
 // S_a^-1: a holds S_a on entry and is overwritten by its Cholesky factor;
 // inv receives the inverse.  A diagonal S_a is inverted elementwise, as the
 // reference does.  status[r] = 1 if S_a is not positive definite.
-__global__ __launch_bounds__(LM_T) void k_lm_cholinv(const LmDesc *D, const int *list, double *a_arena,
-                                                     double *inv_arena, int *status) {
+__global__ __launch_bounds__(LM_T) void k_lm_cholinv(const LmDesc *D, const int *list, int *status) {
   __shared__ int bad, offdiag;
   const int r = list[blockIdx.x];
   const LmDesc d = D[r];
-  double *a = a_arena + d.o_nn;
-  double *inv = inv_arena + d.o_nn;
+  double *a = d.a;
+  double *inv = d.sainv;
   const int n = d.n;
   if (threadIdx.x == 0) bad = offdiag = 0;
   __syncthreads();
@@ -200,36 +198,31 @@ __global__ __launch_bounds__(LM_T) void k_lm_cholinv(const LmDesc *D, const int 
 }
 
 // b = Kc^T (sig_inv * dy) - S_a^-1 dx  (Kc is already scaled by sig_inv).
-__global__ __launch_bounds__(LM_T) void k_lm_rhs(const LmDesc *D, const int *list, const double *kc,
-                                                 const double *sig_inv, const double *dy, const double *sainv,
-                                                 const double *dx, double *b) {
+__global__ __launch_bounds__(LM_T) void k_lm_rhs(const LmDesc *D, const int *list) {
   const LmDesc d = D[list[blockIdx.y]];
   const int j = blockIdx.x * LM_T + threadIdx.x;
   if (j >= d.n) return;
-  const double *K = kc + d.o_k;
   double v = 0;
-  for (int i = 0; i < d.m; i++) v += K[(size_t)i * d.n + j] * (sig_inv[d.o_m + i] * dy[d.o_m + i]);
-  const double *S = sainv + d.o_nn;
+  for (int i = 0; i < d.m; i++) v += d.k[(size_t)i * d.n + j] * (d.sig[i] * d.dy[i]);
   double u = 0;
-  for (int k = 0; k < d.n; k++) u += S[(size_t)j * d.n + k] * dx[d.o_n + k];
-  b[d.o_n + j] = -1.0 * u + v;
+  for (int k = 0; k < d.n; k++) u += d.sainv[(size_t)j * d.n + k] * d.dx[k];
+  d.b[j] = -1.0 * u + v;
 }
 
 // A = S_a^-1 (1 + lmpar) + C; A = L L^T; step = A^-1 b.
-__global__ __launch_bounds__(LM_T) void k_lm_solve(const LmDesc *D, const int *list, const double *sainv,
-                                                   const double *cmat, const double *lmpar, const double *b,
-                                                   double *a_arena, double *step, int *status) {
+__global__ __launch_bounds__(LM_T) void k_lm_solve(const LmDesc *D, const int *list, const double *lmpar,
+                                                   int *status) {
   __shared__ int bad;
   const int r = list[blockIdx.x];
   const LmDesc d = D[r];
   const int n = d.n;
   const double lp = lmpar[r];
-  const double *S = sainv + d.o_nn, *Cm = cmat + d.o_nn;
-  double *A = a_arena + d.o_nn;
+  const double *S = d.sainv, *Cm = d.cmat;
+  double *A = d.a;
   if (threadIdx.x == 0) bad = 0;
   for (long e = threadIdx.x; e < (long)n * n; e += LM_T) A[e] = S[e] * (1 + lp) + Cm[e];
-  double *x = step + d.o_n;
-  for (int i = threadIdx.x; i < n; i += LM_T) x[i] = b[d.o_n + i];
+  double *x = d.step;
+  for (int i = threadIdx.x; i < n; i += LM_T) x[i] = d.b[i];
   __syncthreads();
   if (!chol_wg(A, n, &bad)) {
     if (threadIdx.x == 0) status[r] = 1;
@@ -240,13 +233,12 @@ __global__ __launch_bounds__(LM_T) void k_lm_solve(const LmDesc *D, const int *l
 }
 
 // q[r] = dx^T S_a^-1 dx.
-__global__ __launch_bounds__(LM_T) void k_lm_quad(const LmDesc *D, const int *list, const double *sainv,
-                                                  const double *dx, double *q) {
+__global__ __launch_bounds__(LM_T) void k_lm_quad(const LmDesc *D, const int *list, double *q) {
   __shared__ double part[LM_T];
   const int r = list[blockIdx.x];
   const LmDesc d = D[r];
-  const double *S = sainv + d.o_nn;
-  const double *x = dx + d.o_n;
+  const double *S = d.sainv;
+  const double *x = d.dx;
   double acc = 0;
   for (int i = threadIdx.x; i < d.n; i += LM_T) {
     double v = 0;
@@ -264,36 +256,30 @@ __global__ __launch_bounds__(LM_T) void k_lm_quad(const LmDesc *D, const int *li
 
 // ---------------- launchers ----------------
 void launch_lm_gather(const LmDesc *D, const int *list, int cnt, int max_rows, const double *kpass, int nd,
-                      int ncol, const double *sig_inv, double *kc, hipStream_t s) {
+                      int ncol, hipStream_t s) {
   if (cnt == 0 || max_rows == 0) return;
-  hipLaunchKernelGGL(k_lm_gather, dim3(max_rows, cnt), dim3(LM_T), 0, s, D, list, kpass, nd, ncol, sig_inv, kc);
+  hipLaunchKernelGGL(k_lm_gather, dim3(max_rows, cnt), dim3(LM_T), 0, s, D, list, kpass, nd, ncol);
 }
-void launch_lm_normal(const LmDesc *D, const int *list, int cnt, int max_n, const double *kc, double *cmat,
-                      hipStream_t s) {
+void launch_lm_normal(const LmDesc *D, const int *list, int cnt, int max_n, hipStream_t s) {
   if (cnt == 0) return;
   const int nt = (max_n + 63) / 64;
-  hipLaunchKernelGGL(k_lm_normal, dim3(nt * (nt + 1) / 2, cnt), dim3(LM_T), 0, s, D, list, kc, cmat);
+  hipLaunchKernelGGL(k_lm_normal, dim3(nt * (nt + 1) / 2, cnt), dim3(LM_T), 0, s, D, list);
 }
-void launch_lm_cholinv(const LmDesc *D, const int *list, int cnt, double *a, double *inv, int *status,
-                       hipStream_t s) {
+void launch_lm_cholinv(const LmDesc *D, const int *list, int cnt, int *status, hipStream_t s) {
   if (cnt == 0) return;
-  hipLaunchKernelGGL(k_lm_cholinv, dim3(cnt), dim3(LM_T), 0, s, D, list, a, inv, status);
+  hipLaunchKernelGGL(k_lm_cholinv, dim3(cnt), dim3(LM_T), 0, s, D, list, status);
 }
-void launch_lm_rhs(const LmDesc *D, const int *list, int cnt, int max_n, const double *kc, const double *sig_inv,
-                   const double *dy, const double *sainv, const double *dx, double *b, hipStream_t s) {
+void launch_lm_rhs(const LmDesc *D, const int *list, int cnt, int max_n, hipStream_t s) {
   if (cnt == 0) return;
-  hipLaunchKernelGGL(k_lm_rhs, dim3((max_n + LM_T - 1) / LM_T, cnt), dim3(LM_T), 0, s, D, list, kc, sig_inv, dy,
-                     sainv, dx, b);
+  hipLaunchKernelGGL(k_lm_rhs, dim3((max_n + LM_T - 1) / LM_T, cnt), dim3(LM_T), 0, s, D, list);
 }
-void launch_lm_solve(const LmDesc *D, const int *list, int cnt, const double *sainv, const double *cmat,
-                     const double *lmpar, const double *b, double *a, double *step, int *status, hipStream_t s) {
+void launch_lm_solve(const LmDesc *D, const int *list, int cnt, const double *lmpar, int *status, hipStream_t s) {
   if (cnt == 0) return;
-  hipLaunchKernelGGL(k_lm_solve, dim3(cnt), dim3(LM_T), 0, s, D, list, sainv, cmat, lmpar, b, a, step, status);
+  hipLaunchKernelGGL(k_lm_solve, dim3(cnt), dim3(LM_T), 0, s, D, list, lmpar, status);
 }
-void launch_lm_quad(const LmDesc *D, const int *list, int cnt, const double *sainv, const double *dx, double *q,
-                    hipStream_t s) {
+void launch_lm_quad(const LmDesc *D, const int *list, int cnt, double *q, hipStream_t s) {
   if (cnt == 0) return;
-  hipLaunchKernelGGL(k_lm_quad, dim3(cnt), dim3(LM_T), 0, s, D, list, sainv, dx, q);
+  hipLaunchKernelGGL(k_lm_quad, dim3(cnt), dim3(LM_T), 0, s, D, list, q);
 }
 
 }  // namespace jur

@@ -42,7 +42,8 @@ EXPORTS = [
     "jur_gpu_formod_batch", "jur_gpu_kernel", "kernel_gpu", "jur_ctx_create",
     "jur_ctx_create_wide", "jur_ctx_destroy", "jur_ctx_nd", "jur_ctx_formod_device",
     "jur_ctx_last_timing", "jur_ctx_last_mean_np", "jur_state_map", "jur_ctx_jacobian_device",
-    "jur_gpu_optimal_estimation",
+    "jur_gpu_optimal_estimation", "jur_server_create", "jur_server_destroy", "jur_server_submit",
+    "jur_server_wait", "jur_server_fetch", "jur_server_stats",
 ]
 
 
@@ -88,6 +89,17 @@ def lib():
         L.jur_ctx_jacobian_device.restype = I
         L.jur_gpu_optimal_estimation.argtypes = [P, P, I, P, P, P, P, C.POINTER(D), C.POINTER(I), I, I]
         L.jur_gpu_optimal_estimation.restype = I
+        L.jur_server_create.argtypes = [P, P, C.POINTER(abi.server_opts_t), C.POINTER(P)]
+        L.jur_server_create.restype = I
+        L.jur_server_destroy.argtypes = [P]
+        L.jur_server_submit.argtypes = [P, C.POINTER(abi.request_t), C.POINTER(C.c_uint64)]
+        L.jur_server_submit.restype = I
+        L.jur_server_wait.argtypes = [P, C.c_uint64, D, C.POINTER(abi.result_t)]
+        L.jur_server_wait.restype = I
+        L.jur_server_fetch.argtypes = [P, C.c_uint64, P, P, C.POINTER(D), C.POINTER(D)]
+        L.jur_server_fetch.restype = I
+        L.jur_server_stats.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.jur_server_stats.restype = I
         if L.jur_gpu_abi_check(C.sizeof(abi.ctl_t), C.sizeof(abi.atm_t), C.sizeof(abi.obs_t)):
             raise OSError("ABI mismatch between jurassic_amd.abi and libjurassic_gpu.so")
         _LIB = L
@@ -202,6 +214,74 @@ class Context:
     def close(self):
         if self.ptr:
             lib().jur_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The request server (jur_server_*): forward models, Jacobians and
+    retrievals submitted at any time, batched continuously on the GPU."""
+
+    def __init__(self, ctl, ret, method=abi.JUR_JAC_ADJOINT, precision=abi.JUR_FP32, max_bytes=0,
+                 max_batch=0):
+        self.ptr = C.c_void_p()
+        self.ret = ret
+        opt = abi.server_opts_t(method, precision, max_bytes, max_batch)
+        _check(lib().jur_server_create(C.byref(ctl), C.byref(ret), C.byref(opt), C.byref(self.ptr)))
+
+    @staticmethod
+    def request(kind, atm, obs, rett=(-999.0, -999.0), retq=None, retp=(-999.0, -999.0)):
+        """A request_t; retq maps emitter index -> (zmin, zmax)."""
+        r = abi.request_t()
+        r.kind = kind
+        r.atm = C.addressof(atm)
+        r.obs = C.addressof(obs)
+        r.retp_zmin, r.retp_zmax = retp
+        r.rett_zmin, r.rett_zmax = rett
+        for i in range(abi.NG):
+            r.retq_zmin[i] = r.retq_zmax[i] = -999.0
+        for i in range(abi.NW):
+            r.retk_zmin[i] = r.retk_zmax[i] = -999.0
+        for ig, (a, b) in (retq or {}).items():
+            r.retq_zmin[ig], r.retq_zmax[ig] = a, b
+        return r
+
+    def submit(self, req):
+        i = C.c_uint64()
+        _check(lib().jur_server_submit(self.ptr, C.byref(req), C.byref(i)))
+        return i.value
+
+    def wait(self, rid, timeout=-1.0):
+        res = abi.result_t()
+        _check(lib().jur_server_wait(self.ptr, rid, timeout, C.byref(res)))
+        return res
+
+    def fetch(self, rid, res):
+        """(obs, atm, K or None, chisq history or None) of a finished request;
+        raises if the request failed."""
+        obs, atm = abi.obs_t(), abi.atm_t()
+        k = np.zeros((max(res.m, 1), max(res.n, 1))) if res.kind == abi.JUR_REQ_JACOBIAN else None
+        h = np.zeros(self.ret.conv_itmax + 1) if res.kind == abi.JUR_REQ_RETRIEVAL else None
+        _check(lib().jur_server_fetch(self.ptr, rid, C.byref(obs), C.byref(atm),
+                                      k.ctypes.data_as(C.POINTER(C.c_double)) if k is not None else None,
+                                      h.ctypes.data_as(C.POINTER(C.c_double)) if h is not None else None))
+        if h is not None:
+            h = h[:res.niter + 1]
+        return obs, atm, k, h
+
+    def stats(self):
+        a, b = C.c_long(), C.c_long()
+        _check(lib().jur_server_stats(self.ptr, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.ptr:
+            lib().jur_server_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):
