@@ -25,6 +25,15 @@ oracle kernel() (OpenMP over the 324 columns) on a sample of atmospheres.
 The FP32 adjoint's error is stated against the FP64 adjoint on a subset:
 relative L2 per state block and the brightness temperature of the forward
 part.
+
+Serving (the "serving" object): Poisson arrivals at 2000 requests/s (seeded)
+into the request server for a fixed time; 60% forward, 25% Jacobian, 15%
+Levenberg-Marquardt retrievals (CONV_ITMAX 10, S_a: 2 K and 20% vmr,
+exponential correlation over 5 km; S_eps 0.1 K^2); 1..64 rays per request,
+limb or nadir 50/50; state T plus 0..3 volume mixing ratios on 81 levels
+(n = 81..324); 12 channels, brightness temperatures, FP32 lookups, adjoint
+Jacobians.  Latency is arrival (submission) to result in host memory;
+p50/p99 per kind and overall, with the throughput sustained.
 """
 import argparse
 import ctypes as C
@@ -250,6 +259,148 @@ def jacobian_bench(args, torch, gpu, abi):
     return out
 
 
+def serving_bench(args, gpu, abi):
+    """Poisson stream of mixed requests into the request server."""
+    import threading
+    import numpy as np
+    from jurassic_amd import scenes
+    rng = np.random.default_rng(2024)
+    ctl = scenes.limb_3gas_ctl()
+    ctl.write_bbt = 1
+    nd = ctl.nd
+    ret = abi.ret_t()
+    ret.kernel_recomp, ret.conv_itmax, ret.conv_dmin = 3, 10, 0.1
+    for i in range(nd):
+        ret.err_noise[i] = math.sqrt(0.1)
+    ret.err_temp, ret.err_temp_cz, ret.err_temp_ch = 2.0, 5.0, 500.0
+    for ig in range(3):
+        ret.err_q[ig], ret.err_q_cz[ig], ret.err_q_ch[ig] = 20.0, 5.0, 500.0
+    # geometry templates: (limb | nadir) x 1..64 rays
+    geo = {}
+    for kind in ("limb", "nadir"):
+        for nr in range(1, 65):
+            if kind == "limb":
+                g = scenes.limb_geometry(800.0, np.sort(rng.uniform(5.0, 67.0, nr)))
+            else:
+                g = scenes.nadir_geometry(705.0, np.sort(rng.uniform(0.0, 50.0, nr)))
+            geo[kind, nr] = scenes.make_obs(g)
+    atms = [scenes.make_profile_atm(GASES, iprof=1000 + k) for k in range(64)]
+    apr = scenes.make_profile_atm(GASES, perturb=False)
+    # measurements for the retrievals: brightness temperatures of the pool
+    # atmospheres on every template (one batched GPU forward model)
+    keys = list(geo.keys())
+    ma = (abi.atm_t * len(keys))()
+    mo = (abi.obs_t * len(keys))()
+    for i, key in enumerate(keys):
+        ma[i] = atms[i % len(atms)]
+        mo[i] = geo[key]
+    gpu.formod_batch(ctl, ma, mo, abi.JUR_FP32)
+    meas = {key: abi.copy_struct(mo[i]) for i, key in enumerate(keys)}
+    # the schedule
+    nreq = int(args.serve_rate * args.serve_seconds)
+    t_arr = np.cumsum(rng.exponential(1.0 / args.serve_rate, nreq))
+    kinds = rng.choice([abi.JUR_REQ_FORWARD, abi.JUR_REQ_JACOBIAN, abi.JUR_REQ_RETRIEVAL], nreq,
+                       p=[0.6, 0.25, 0.15])
+    nrays = rng.integers(1, 65, nreq)
+    geos = rng.choice(["limb", "nadir"], nreq)
+    ngas = rng.integers(0, 4, nreq)
+    which = rng.integers(0, len(atms), nreq)
+    reqs = []
+    for i in range(nreq):
+        key = (str(geos[i]), int(nrays[i]))
+        retq = {ig: (-100.0, 100.0) for ig in range(int(ngas[i]))}
+        if kinds[i] == abi.JUR_REQ_RETRIEVAL:
+            reqs.append(gpu.Server.request(int(kinds[i]), apr, meas[key], rett=(-100.0, 100.0), retq=retq))
+        else:
+            reqs.append(gpu.Server.request(int(kinds[i]), atms[which[i]], geo[key], rett=(-100.0, 100.0),
+                                           retq=retq))
+    srv = gpu.Server(ctl, ret, precision=abi.JUR_FP32, lanes=args.serve_lanes)
+    # warm: tables, kernels, allocations
+    for kind in (abi.JUR_REQ_FORWARD, abi.JUR_REQ_JACOBIAN, abi.JUR_REQ_RETRIEVAL):
+        i = srv.submit(gpu.Server.request(kind, apr if kind == abi.JUR_REQ_RETRIEVAL else atms[0],
+                                          meas["limb", 8] if kind == abi.JUR_REQ_RETRIEVAL else geo["limb", 8],
+                                          rett=(-100.0, 100.0)))
+        srv.wait(i, 300.0)
+        srv.release(i)
+    r0 = srv.stats()[0]
+    log("serving: %d requests over %.1f s (%.0f/s), %d forward, %d Jacobian, %d retrieval"
+        % (nreq, t_arr[-1], args.serve_rate, (kinds == 0).sum(), (kinds == 1).sum(), (kinds == 2).sum()))
+    ids = [None] * nreq
+    lag = [0.0]
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            dt = t_arr[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            else:
+                lag[0] = max(lag[0], -dt)
+            ids[i] = srv.submit(reqs[i])
+
+    th = threading.Thread(target=submitter)
+    t_start = time.perf_counter()
+    th.start()
+    lat = np.zeros(nreq)
+    t_sub = np.zeros(nreq)
+    t_done = np.zeros(nreq)
+    status = np.zeros(nreq, dtype=int)
+    niter = np.zeros(nreq, dtype=int)
+    fails = {}
+    done = 0
+    last_log = time.perf_counter()
+    for i in range(nreq):
+        while ids[i] is None:
+            time.sleep(0.0005)
+        res = srv.wait(ids[i], 600.0)
+        status[i] = res.status
+        niter[i] = res.niter
+        if res.status:
+            key = "%d %s" % (int(kinds[i]), res.message.decode(errors="replace"))
+            fails[key] = fails.get(key, 0) + 1
+        t_sub[i], t_done[i] = res.t_submit, res.t_done
+        lat[i] = res.t_done - res.t_submit
+        srv.release(ids[i])
+        done += 1
+        if time.perf_counter() - last_log > 5:
+            log("serving: %d/%d results" % (done, nreq))
+            last_log = time.perf_counter()
+    th.join()
+    wall = time.perf_counter() - t_start
+    rounds, max_active = srv.stats()
+    prof = srv.profile()
+    srv.close()
+    ok = status == 0
+    out = {
+        "workload": "Poisson %.0f requests/s for %.1f s: 60%% forward, 25%% Jacobian, 15%% retrieval; 1..64 rays; "
+                    "limb/nadir 50/50; n = 81..324; 12 channels, BT; FP32 lookups, adjoint Jacobians"
+                    % (args.serve_rate, args.serve_seconds),
+        "requests": nreq, "failed": int((~ok).sum()), "failures": fails,
+        "throughput_req_per_s": nreq / (t_done.max() - t_sub.min()),
+        "offered_req_per_s": nreq / t_arr[-1],
+        "submit_lag_max_s": lag[0],
+        "device_rounds": rounds - r0, "max_in_flight": max_active, "lanes": args.serve_lanes,
+        "latency_ms": {"p50": float(np.percentile(lat[ok], 50) * 1e3), "p99": float(np.percentile(lat[ok], 99) * 1e3),
+                       "max": float(lat[ok].max() * 1e3)},
+        "wall_s": wall,
+        "worker_seconds": {"admission": prof[0], "jacobian_passes": prof[1], "forward_passes": prof[2],
+                           "dense_and_bookkeeping": prof[3]},
+    }
+    for k, name in ((0, "forward"), (1, "jacobian"), (2, "retrieval")):
+        sel = ok & (kinds == k)
+        if sel.any():
+            out["latency_ms_" + name] = {"p50": float(np.percentile(lat[sel], 50) * 1e3),
+                                         "p99": float(np.percentile(lat[sel], 99) * 1e3),
+                                         "count": int(sel.sum())}
+    sel = ok & (kinds == 2)
+    if sel.any():
+        out["retrieval_mean_iterations"] = float(niter[sel].mean())
+    log("serving: p50 %.1f ms, p99 %.1f ms, %.0f req/s, %d device rounds, up to %d in flight"
+        % (out["latency_ms"]["p50"], out["latency_ms"]["p99"], out["throughput_req_per_s"], out["device_rounds"],
+           max_active))
+    return out
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -265,6 +416,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-jac", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-lanes", type=int, default=1)
     ap.add_argument("--jac-only", action="store_true", help="skip the forward-model section")
     ap.add_argument("--jac-nprof", type=int, default=NJAC_PROF)
     ap.add_argument("--jac-steps", type=int, default=2)
@@ -291,6 +447,9 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written")
         del src, dst
+    if args.serve_only:
+        print(json.dumps({"serving": serving_bench(args, gpu, abi)}))
+        return
     if args.jac_only:
         print(json.dumps({"jacobian": jacobian_bench(args, torch, gpu, abi)}))
         return
@@ -389,6 +548,8 @@ def main():
     del rad_d, tau_d, atm_d
     if not args.no_jac:
         out["jacobian"] = jacobian_bench(args, torch, gpu, abi)
+    if not args.no_serve:
+        out["serving"] = serving_bench(args, gpu, abi)
     print(json.dumps(out))
 
 

@@ -276,10 +276,11 @@ int jur_gpu_optimal_estimation(const jur_ret_t *ret, const ctl_t *ctl, int n,
 
 /* ------------------------------------------------------------------
    Request server.  Forward models, Jacobians and retrievals are submitted
-   at any time from any thread; a worker thread batches everything in
-   flight into shared device passes (continuous batching: requests that
-   arrive while a pass runs join the next one, admitted in arrival order
-   while their device memory fits the budget).  No counterpart in the
+   at any time from any thread; worker lanes (threads with their own HIP
+   stream) batch everything in flight into shared device passes (continuous
+   batching: requests that arrive while a pass runs join the next round of
+   the first lane to start one, admitted in arrival order while their
+   device memory fits the budget).  No counterpart in the
    reference, whose programs run one job at a time.
    ------------------------------------------------------------------ */
 #define JUR_REQ_FORWARD 0     /* formod */
@@ -294,6 +295,7 @@ typedef struct {
   int precision;          /* JUR_FP32 or JUR_FP64 */
   size_t max_bytes;       /* device memory for requests in flight (0: 16 GiB) */
   int max_batch;          /* most requests in flight (0: no limit) */
+  int lanes;              /* worker lanes, each with its own stream (0: 1) */
 } jur_server_opts_t;
 
 typedef struct {
@@ -336,6 +338,10 @@ int jur_server_fetch(jur_server_t *srv, uint64_t id, obs_t *obs, atm_t *atm,
                      double *k, double *chisq_hist);
 /* Device passes run so far and the most requests in flight at once. */
 int jur_server_stats(const jur_server_t *srv, long *rounds, long *max_active);
+/* Worker time in seconds, summed over lanes: [0] admission, [1] device
+   passes (with the adjoint, every pass), [2] forward-only passes (the
+   finite-difference method), [3] dense algebra and bookkeeping (n <= 4). */
+int jur_server_profile(const jur_server_t *srv, double *sec, int n);
 
 /* ERRMSG-style wrapper of jur_gpu_kernel(..., JUR_JAC_FD, JUR_FP64). */
 void kernel_gpu(ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m,

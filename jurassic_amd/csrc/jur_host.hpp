@@ -23,9 +23,11 @@ int set_err(int code, const char *fmt, ...);
 #define HIPCHK(x)                                                                    \
   do {                                                                               \
     hipError_t e_ = (x);                                                             \
-    if (e_ != hipSuccess)                                                            \
+    if (e_ != hipSuccess) {                                                          \
+      (void)hipGetLastError(); /* the runtime keeps the error as pending too */     \
       return set_err(JUR_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_),      \
                      __FILE__, __LINE__);                                            \
+    }                                                                                \
   } while (0)
 
 template <typename T>
@@ -43,6 +45,7 @@ struct DevBuf {
     release();
     if (m == 0) m = 1;
     if (hipMalloc((void **)&p, m * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
       p = nullptr;
       return set_err(JUR_ERR_MEM, "device allocation of %zu bytes failed", m * sizeof(T));
     }
@@ -53,6 +56,16 @@ struct DevBuf {
     int rc = reserve(m);
     if (rc) return rc;
     if (m) HIPCHK(hipMemcpy(p, h, m * sizeof(T), hipMemcpyHostToDevice));
+    return JUR_OK;
+  }
+  // Ordered on stream s (complete on return).
+  int upload(const T *h, size_t m, hipStream_t s) {
+    int rc = reserve(m);
+    if (rc) return rc;
+    if (m) {
+      HIPCHK(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
     return JUR_OK;
   }
 };

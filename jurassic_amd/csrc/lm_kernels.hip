@@ -254,7 +254,59 @@ __global__ __launch_bounds__(LM_T) void k_lm_quad(const LmDesc *D, const int *li
   if (threadIdx.x == 0) q[r] = part[0];
 }
 
+// Per-retrieval vectors to and from one packed staging buffer (one copy per
+// phase instead of one per retrieval): which = 0 puts dx (n) then dy (m),
+// which = 3 puts sig (m), which = 1 gets b (n), which = 2 gets step (n).
+__global__ __launch_bounds__(LM_T) void k_lm_stage(const LmDesc *D, const int *list, const size_t *off,
+                                                   double *stage, int which) {
+  const LmDesc d = D[list[blockIdx.x]];
+  double *s = stage + off[blockIdx.x];
+  if (which == 0) {
+    for (int i = threadIdx.x; i < d.n; i += LM_T) d.dx[i] = s[i];
+    for (int i = threadIdx.x; i < d.m; i += LM_T) d.dy[i] = s[d.n + i];
+  } else if (which == 3) {
+    for (int i = threadIdx.x; i < d.m; i += LM_T) d.sig[i] = s[i];
+  } else {
+    const double *v = which == 1 ? d.b : d.step;
+    for (int i = threadIdx.x; i < d.n; i += LM_T) s[i] = v[i];
+  }
+}
+
+// S_a of set_cov_apr (src/retrieval.c:808-920) into d.a from packed
+// per-element inputs in stage + off[b]: [sigma | z | x | y | zc | cz | ch |
+// quantity], 8 n doubles (x, y, zc: geo2cart(0, lon, lat)).
+__global__ __launch_bounds__(LM_T) void k_lm_cov_apr(const LmDesc *D, const int *list, const size_t *off,
+                                                     const double *stage) {
+  const LmDesc d = D[list[blockIdx.x]];
+  const int n = d.n;
+  const double *s = stage + off[blockIdx.x];
+  const double *sg = s, *z = s + n, *px = s + 2 * n, *py = s + 3 * n, *pz = s + 4 * n;
+  const double *cz = s + 5 * n, *ch = s + 6 * n, *q = s + 7 * n;
+  for (long e = threadIdx.x + (long)blockIdx.y * LM_T; e < (long)n * n; e += (long)LM_T * gridDim.y) {
+    const int i = (int)(e / n), j = (int)(e - (long)i * n);
+    double v = 0;
+    if (i == j) {
+      v = sg[i] * sg[i];
+    } else if (q[i] == q[j] && cz[i] > 0 && ch[i] > 0) {
+      const double dx = px[i] - px[j], dy = py[i] - py[j], dz = pz[i] - pz[j];
+      const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+      v = sg[i] * sg[j] * exp(-dist / ch[i] - fabs(z[i] - z[j]) / cz[i]);
+    }
+    d.a[e] = v;
+  }
+}
+
 // ---------------- launchers ----------------
+void launch_lm_cov_apr(const LmDesc *D, const int *list, int cnt, const size_t *off, const double *stage,
+                       hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_lm_cov_apr, dim3(cnt, 16), dim3(LM_T), 0, s, D, list, off, stage);
+}
+void launch_lm_stage(const LmDesc *D, const int *list, int cnt, const size_t *off, double *stage, int which,
+                     hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_lm_stage, dim3(cnt), dim3(LM_T), 0, s, D, list, off, stage, which);
+}
 void launch_lm_gather(const LmDesc *D, const int *list, int cnt, int max_rows, const double *kpass, int nd,
                       int ncol, hipStream_t s) {
   if (cnt == 0 || max_rows == 0) return;

@@ -47,6 +47,43 @@ void launch_lm_cholinv(const LmDesc *, const int *, int, int *, hipStream_t);
 void launch_lm_rhs(const LmDesc *, const int *, int, int, hipStream_t);
 void launch_lm_solve(const LmDesc *, const int *, int, const double *, int *, hipStream_t);
 void launch_lm_quad(const LmDesc *, const int *, int, double *, hipStream_t);
+void launch_lm_stage(const LmDesc *, const int *, int, const size_t *, double *, int, hipStream_t);
+void launch_lm_cov_apr(const LmDesc *, const int *, int, const size_t *, const double *, hipStream_t);
+
+// Device memory for the jobs' matrices and vectors: blocks of 2^k doubles
+// kept on free lists and reused, so that admitting a request allocates
+// nothing once the pool has warmed up.
+struct DevPool {
+  std::vector<std::vector<double *>> free_;
+  std::vector<double *> all_;
+  DevPool() : free_(48) {}
+  ~DevPool() {
+    for (double *p : all_) (void)hipFree(p);
+  }
+  static int cls(size_t n) {
+    int k = 6;
+    while (((size_t)1 << k) < n) k++;
+    return k;
+  }
+  double *get(size_t n, int &c) {
+    c = cls(n);
+    if (!free_[c].empty()) {
+      double *p = free_[c].back();
+      free_[c].pop_back();
+      return p;
+    }
+    double *p = nullptr;
+    if (hipMalloc((void **)&p, ((size_t)1 << c) * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    all_.push_back(p);
+    return p;
+  }
+  void put(double *p, int c) {
+    if (p) free_[c].push_back(p);
+  }
+};
 int kernel_fd_host(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m, size_t n, int precision);
 
 #define POW2(x) ((x) * (x))
@@ -74,8 +111,11 @@ struct Job {
   Stage st = ST_EVAL_FK0;
   int rc = JUR_OK;
   char err[256] = "";
-  // device buffers of a retrieval
-  DevBuf<double> d_sainv, d_cmat, d_a, d_k, d_sig, d_dy, d_dx, d_b, d_step;
+  // device buffers of a retrieval (from the engine's pool):
+  // S_a^-1, C, A, K, sig, dy, dx, b, step
+  double *dp[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int dc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<double> sa_in;  // packed inputs of S_a (8 n)
   int ray0 = 0;
   size_t bytes = 0;        // device footprint of the job's own buffers
   size_t admit_bytes = 0;  // what the server reserved for it
@@ -88,26 +128,30 @@ struct Job {
   std::vector<double> own_hist, own_k;
   int own_niter = 0;
 
-  LmDesc desc() {
-    return LmDesc{n, m, ray0, nr, d_sainv.p, d_cmat.p, d_a.p, d_k.p, d_sig.p, d_dy.p, d_dx.p, d_b.p, d_step.p};
-  }
+  LmDesc desc() { return LmDesc{n, m, ray0, nr, dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], dp[8]}; }
   bool failed() const { return rc != JUR_OK; }
 };
 
-// set_cov_apr (src/retrieval.c:808-920).
-int cov_apr(const jur_ret_t *ret, const ctl_t *ctl, const atm_t *atm, const Job &J, std::vector<double> &sa) {
+// The inputs of set_cov_apr (src/retrieval.c:808-920) per state element,
+// packed for k_lm_cov_apr: standard deviation, altitude, geo2cart(0, lon,
+// lat), correlation lengths in z and horizontally, quantity index.
+int cov_apr_inputs(const jur_ret_t *ret, const ctl_t *ctl, const atm_t *atm, Job &J) {
   const size_t n = J.n;
   std::vector<double> xa(n + 1);
   atm2x_h(ctl, atm, xa.data(), nullptr, nullptr);
   const int ng = ctl->ng, nw = ctl->nw, ncl = ctl->ncl, nsf = ctl->nsf;
+  J.sa_in.assign(8 * n, 0.0);
+  double *sg = J.sa_in.data(), *z = sg + n, *px = sg + 2 * n, *py = sg + 3 * n, *pz = sg + 4 * n;
+  double *czv = sg + 5 * n, *chv = sg + 6 * n, *qv = sg + 7 * n;
   for (size_t i = 0; i < n; i++) {
     const int q = J.iqa[i];
-    if (q == IDXP) xa[i] = ret->err_press / 100 * xa[i];
-    if (q == IDXT) xa[i] = ret->err_temp;
+    double cz = 0, ch = 0;
+    if (q == IDXP) { xa[i] = ret->err_press / 100 * xa[i]; cz = ret->err_press_cz; ch = ret->err_press_ch; }
+    if (q == IDXT) { xa[i] = ret->err_temp; cz = ret->err_temp_cz; ch = ret->err_temp_ch; }
     for (int ig = 0; ig < ng; ig++)
-      if (q == 2 + ig) xa[i] = ret->err_q[ig] / 100 * xa[i];
+      if (q == 2 + ig) { xa[i] = ret->err_q[ig] / 100 * xa[i]; cz = ret->err_q_cz[ig]; ch = ret->err_q_ch[ig]; }
     for (int iw = 0; iw < nw; iw++)
-      if (q == 2 + ng + iw) xa[i] = ret->err_k[iw];
+      if (q == 2 + ng + iw) { xa[i] = ret->err_k[iw]; cz = ret->err_k_cz[iw]; ch = ret->err_k_ch[iw]; }
     if (q == 2 + ng + nw) xa[i] = ret->err_clz;
     if (q == 3 + ng + nw) xa[i] = ret->err_cldz;
     for (int icl = 0; icl < ncl; icl++)
@@ -117,38 +161,41 @@ int cov_apr(const jur_ret_t *ret, const ctl_t *ctl, const atm_t *atm, const Job 
     if (q == 6 + ng + nw + ncl) xa[i] = ret->err_sft;
     for (int isf = 0; isf < nsf; isf++)
       if (q == 7 + ng + nw + ncl + isf) xa[i] = ret->err_sfeps[isf];
-  }
-  for (size_t i = 0; i < n; i++)
     if (POW2(xa[i]) <= 0) return set_err(JUR_ERR_ARG, "Check a priori data (zero standard deviation)!");
-  sa.assign(n * n, 0.0);
-  for (size_t i = 0; i < n; i++) sa[i * n + i] = POW2(xa[i]);
-  for (size_t i = 0; i < n; i++)
-    for (size_t j = 0; j < n; j++)
-      if (i != j && J.iqa[i] == J.iqa[j]) {
-        const int q = J.iqa[i];
-        double cz = 0, ch = 0;
-        if (q == IDXP) { cz = ret->err_press_cz; ch = ret->err_press_ch; }
-        if (q == IDXT) { cz = ret->err_temp_cz; ch = ret->err_temp_ch; }
-        for (int ig = 0; ig < ng; ig++)
-          if (q == 2 + ig) { cz = ret->err_q_cz[ig]; ch = ret->err_q_ch[ig]; }
-        for (int iw = 0; iw < nw; iw++)
-          if (q == 2 + ng + iw) { cz = ret->err_k_cz[iw]; ch = ret->err_k_ch[iw]; }
-        if (cz > 0 && ch > 0) {
-          double x0[3], x1[3];
-          auto geo = [](double lon, double lat, double *x) {
-            const double latr = lat / 180. * JUR_PI, lonr = lon / 180. * JUR_PI;
-            x[0] = JUR_RE * cos(latr) * cos(lonr);
-            x[1] = JUR_RE * cos(latr) * sin(lonr);
-            x[2] = JUR_RE * sin(latr);
-          };
-          geo(atm->lon[J.ipa[i]], atm->lat[J.ipa[i]], x0);
-          geo(atm->lon[J.ipa[j]], atm->lat[J.ipa[j]], x1);
-          const double dist = sqrt(POW2(x0[0] - x1[0]) + POW2(x0[1] - x1[1]) + POW2(x0[2] - x1[2]));
-          const double rho = exp(-dist / ch - fabs(atm->z[J.ipa[i]] - atm->z[J.ipa[j]]) / cz);
-          sa[i * n + j] = xa[i] * xa[j] * rho;
-        }
-      }
+    const int ip = J.ipa[i];
+    const double latr = atm->lat[ip] / 180. * JUR_PI, lonr = atm->lon[ip] / 180. * JUR_PI;
+    sg[i] = xa[i];
+    z[i] = atm->z[ip];
+    px[i] = JUR_RE * cos(latr) * cos(lonr);
+    py[i] = JUR_RE * cos(latr) * sin(lonr);
+    pz[i] = JUR_RE * sin(latr);
+    czv[i] = cz;
+    chv[i] = ch;
+    qv[i] = q;
+  }
   return JUR_OK;
+}
+
+// The rows of an observation that a job uses: geometry of its nr rays and
+// radiance/transmittance of nd channels (an obs_t holds NR x ND of each).
+void copy_obs_rows(obs_t *dst, const obs_t *src, int nd) {
+  const int nr = src->nr;
+  dst->nr = nr;
+  const size_t b = (size_t)nr * sizeof(double);
+  memcpy(dst->time, src->time, b);
+  memcpy(dst->obsz, src->obsz, b);
+  memcpy(dst->obslon, src->obslon, b);
+  memcpy(dst->obslat, src->obslat, b);
+  memcpy(dst->vpz, src->vpz, b);
+  memcpy(dst->vplon, src->vplon, b);
+  memcpy(dst->vplat, src->vplat, b);
+  memcpy(dst->tpz, src->tpz, b);
+  memcpy(dst->tplon, src->tplon, b);
+  memcpy(dst->tplat, src->tplat, b);
+  for (int id = 0; id < nd; id++) {
+    memcpy(dst->rad[id], src->rad[id], b);
+    memcpy(dst->tau[id], src->tau[id], b);
+  }
 }
 
 // The state clamps of optimal_estimation (src/retrieval.c:572-598).
@@ -181,16 +228,45 @@ struct Engine {
   std::vector<Job *> jobs;  // active
   DevBuf<LmDesc> d_desc;
   DevBuf<int> d_list, d_status;
-  DevBuf<double> d_lmpar, d_q;
+  DevBuf<double> d_lmpar, d_q, d_stage;
+  DevBuf<size_t> d_off;
+  DevPool pool;
+  std::vector<Job *> pending;  // retrievals whose device setup is due
+  // S_a^-1 by the content of its inputs: retrievals with the same state
+  // layout, a priori and settings share it (typical for a served model)
+  struct SaEntry {
+    double *p;
+    int cls, n;
+    uint64_t used;
+  };
+  std::unordered_map<std::string, SaEntry> sa_cache;
+  uint64_t sa_clock = 0;
+  static constexpr size_t SA_CACHE_MAX = 64;
+  ~Engine() {
+    for (auto &e : sa_cache) pool.put(e.second.p, e.second.cls);
+  }
   DevBuf<atm_t> p_atm;
   DevBuf<int> p_prof, p_xmap, p_pmap;
   DevBuf<double> p_geom, p_rad, p_tau, p_k;
   DevBuf<unsigned char> p_mask;
   std::vector<double> h_buf;
   hipStream_t st = 0;
+  // seconds spent: [0] admission, [1] Jacobian passes, [2] forward passes,
+  // [3] dense algebra and bookkeeping
+  double prof[4] = {0, 0, 0, 0};
 
-  // Adds a job; on error the job is marked failed (and not added).
+  // Adds a job (host preparation); on error the job is marked failed and
+  // not added.  Retrievals wait in pending for setup().
   int add(Job *J);
+  // Device setup of the pending retrievals, batched: pooled buffers, S_a
+  // built and inverted on the GPU.  Jobs that fail go to done.
+  int setup(std::vector<Job *> &done);
+  void release(Job *J) {
+    for (int b = 0; b < 9; b++) {
+      pool.put(J->dp[b], J->dc[b]);
+      J->dp[b] = nullptr;
+    }
+  }
   // One round over all active jobs; finished jobs are moved to done.
   int round(std::vector<Job *> &done);
 
@@ -202,9 +278,10 @@ struct Engine {
     return rc;
   }
   int upload_desc();
-  int list(const std::vector<int> &ids) { return ids.empty() ? JUR_OK : d_list.upload(ids.data(), ids.size()); }
-  int eval(const std::vector<int> &ids, bool with_k);
+  int list(const std::vector<int> &ids) { return ids.empty() ? JUR_OK : d_list.upload(ids.data(), ids.size(), st); }
+  int eval(const std::vector<int> &ids, const std::vector<char> &wantk);
   int chisq(const std::vector<int> &ids);
+  int stage_get(const std::vector<int> &ids, int which);
 };
 
 int Engine::add(Job *J) {
@@ -216,7 +293,7 @@ int Engine::add(Job *J) {
   J->m = J->nr * nd;
   J->mv = (int)obs2y_h(sc, J->obs_meas, nullptr);
   J->n = (int)atm2x_h(sc, J->atm_apr, nullptr, nullptr, nullptr);
-  memcpy(J->obs_i, J->obs_meas, sizeof(obs_t));
+  copy_obs_rows(J->obs_i, J->obs_meas, nd);
   if (J->kind == JUR_REQ_FORWARD) {
     J->st = ST_EVAL_F_OUT;
     J->bytes = (size_t)J->m * 16;
@@ -260,54 +337,148 @@ int Engine::add(Job *J) {
   J->b.assign(J->n, 0.0);
   J->step.assign(J->n, 0.0);
   memcpy(J->atm_i, J->atm_apr, sizeof(atm_t));
-  const size_t nn = (size_t)J->n * J->n;
   int rc;
-  if ((rc = J->d_sainv.reserve(nn)) || (rc = J->d_cmat.reserve(nn)) || (rc = J->d_a.reserve(nn)) ||
-      (rc = J->d_k.reserve((size_t)J->m * J->n)) || (rc = J->d_sig.upload(J->sig.data(), J->m)) ||
-      (rc = J->d_dy.reserve(J->m)) || (rc = J->d_dx.reserve(J->n)) || (rc = J->d_b.reserve(J->n)) ||
-      (rc = J->d_step.reserve(J->n)))
-    return fail(J, rc);
+  if ((rc = cov_apr_inputs(ret, sc, J->atm_apr, *J))) return fail(J, rc);
+  const size_t nn = (size_t)J->n * J->n;
+  const size_t sz[9] = {nn, nn, nn, (size_t)J->m * J->n, (size_t)J->m, (size_t)J->m,
+                        (size_t)J->n, (size_t)J->n, (size_t)J->n};
+  for (int b = 0; b < 9; b++) {
+    J->dp[b] = pool.get(sz[b], J->dc[b]);
+    if (!J->dp[b]) {
+      release(J);
+      return fail(J, set_err(JUR_ERR_MEM, "device allocation failed"));
+    }
+  }
   J->bytes = (3 * nn + (size_t)J->m * J->n + 2 * J->m + 3 * J->n) * 8;
-  // S_a (host) -> S_a^-1 (device)
-  std::vector<double> sa;
-  if ((rc = cov_apr(ret, sc, J->atm_apr, *J, sa))) return fail(J, rc);
-  if (hipMemcpy(J->d_a.p, sa.data(), nn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(J, set_err(JUR_ERR_HIP, "hipMemcpy failed"));
-  LmDesc d = J->desc();
-  DevBuf<LmDesc> dd;
-  const int zero = 0;
-  DevBuf<int> dl;
-  if ((rc = dd.upload(&d, 1)) || (rc = dl.upload(&zero, 1)) || (rc = d_status.reserve(1))) return fail(J, rc);
-  launch_lm_cholinv(dd.p, dl.p, 1, d_status.p, st);
-  int status = 0;
-  if (hipMemcpy(&status, d_status.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-    return fail(J, set_err(JUR_ERR_HIP, "hipMemcpy failed"));
-  if (status) return fail(J, set_err(JUR_ERR_ARG, "a priori covariance is not positive definite"));
   J->st = ST_EVAL_FK0;
-  jobs.push_back(J);
+  pending.push_back(J);
+  return JUR_OK;
+}
+
+int Engine::setup(std::vector<Job *> &done) {
+  if (pending.empty()) return JUR_OK;
+  // descriptors of the pending jobs at the end of the active list
+  const size_t base = jobs.size();
+  jobs.insert(jobs.end(), pending.begin(), pending.end());
+  int rc;
+  if ((rc = upload_desc())) return rc;
+  // S_a^-1 from the cache where its inputs were seen before
+  std::vector<std::string> key(pending.size());
+  std::vector<int> miss;  // indices into jobs
+  std::vector<size_t> miss_k;
+  for (size_t k = 0; k < pending.size(); k++) {
+    Job &J = *pending[k];
+    key[k].assign((const char *)J.sa_in.data(), J.sa_in.size() * sizeof(double));
+    auto it = sa_cache.find(key[k]);
+    if (it != sa_cache.end() && it->second.n == J.n) {
+      HIPCHK(hipMemcpyAsync(J.dp[0], it->second.p, (size_t)J.n * J.n * sizeof(double), hipMemcpyDeviceToDevice,
+                            st));
+      it->second.used = ++sa_clock;
+    } else {
+      miss.push_back((int)(base + k));
+      miss_k.push_back(k);
+    }
+  }
+  // staging: the S_a inputs of the misses, then sig of every pending job
+  std::vector<size_t> off_sa(miss.size()), off_sig(pending.size());
+  size_t tot = 0;
+  for (size_t q = 0; q < miss.size(); q++) {
+    off_sa[q] = tot;
+    tot += pending[miss_k[q]]->sa_in.size();
+  }
+  for (size_t k = 0; k < pending.size(); k++) {
+    off_sig[k] = tot;
+    tot += pending[k]->m;
+  }
+  h_buf.resize(tot);
+  for (size_t q = 0; q < miss.size(); q++) {
+    const Job &J = *pending[miss_k[q]];
+    memcpy(&h_buf[off_sa[q]], J.sa_in.data(), J.sa_in.size() * sizeof(double));
+  }
+  for (size_t k = 0; k < pending.size(); k++)
+    memcpy(&h_buf[off_sig[k]], pending[k]->sig.data(), pending[k]->m * sizeof(double));
+  if ((rc = d_stage.upload(h_buf.data(), tot, st))) return rc;
+  std::vector<char> bad(pending.size(), 0);  // S_a of a miss not positive definite
+  if (!miss.empty()) {
+    if ((rc = list(miss)) || (rc = d_off.upload(off_sa.data(), off_sa.size(), st))) return rc;
+    launch_lm_cov_apr(d_desc.p, d_list.p, (int)miss.size(), d_off.p, d_stage.p, st);
+    HIPCHK(hipGetLastError());
+    if ((rc = d_status.reserve(jobs.size()))) return rc;
+    launch_lm_cholinv(d_desc.p, d_list.p, (int)miss.size(), d_status.p, st);
+    HIPCHK(hipGetLastError());
+    // only the entries of the misses were written
+    std::vector<int> status(jobs.size(), 0);
+    HIPCHK(hipMemcpyAsync(status.data(), d_status.p, jobs.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t q = 0; q < miss.size(); q++) bad[miss_k[q]] = status[miss[q]] != 0;
+    for (size_t q = 0; q < miss.size(); q++) {
+      const size_t k = miss_k[q];
+      Job &J = *pending[k];
+      if (bad[k] || sa_cache.count(key[k])) continue;
+      if (sa_cache.size() >= SA_CACHE_MAX) {
+        auto old = sa_cache.begin();
+        for (auto it = sa_cache.begin(); it != sa_cache.end(); ++it)
+          if (it->second.used < old->second.used) old = it;
+        pool.put(old->second.p, old->second.cls);
+        sa_cache.erase(old);
+      }
+      SaEntry e{nullptr, 0, J.n, ++sa_clock};
+      e.p = pool.get((size_t)J.n * J.n, e.cls);
+      if (!e.p) continue;
+      HIPCHK(hipMemcpyAsync(e.p, J.dp[0], (size_t)J.n * J.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      sa_cache.emplace(key[k], e);
+    }
+  }
+  std::vector<int> all(pending.size());
+  for (size_t k = 0; k < pending.size(); k++) all[k] = (int)(base + k);
+  if ((rc = list(all)) || (rc = d_off.upload(off_sig.data(), off_sig.size(), st))) return rc;
+  launch_lm_stage(d_desc.p, d_list.p, (int)all.size(), d_off.p, d_stage.p, 3, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<Job *> keep(jobs.begin(), jobs.begin() + base);
+  for (size_t k = 0; k < pending.size(); k++) {
+    Job *J = pending[k];
+    J->sa_in.clear();
+    if (bad[k]) {
+      release(J);
+      fail(J, set_err(JUR_ERR_ARG, "a priori covariance is not positive definite"));
+      done.push_back(J);
+    } else {
+      keep.push_back(J);
+    }
+  }
+  jobs.swap(keep);
+  pending.clear();
   return JUR_OK;
 }
 
 int Engine::upload_desc() {
   std::vector<LmDesc> desc(jobs.size());
   for (size_t r = 0; r < jobs.size(); r++) desc[r] = jobs[r]->desc();
-  return d_desc.upload(desc.data(), std::max<size_t>(desc.size(), 1));
+  return d_desc.upload(desc.data(), std::max<size_t>(desc.size(), 1), st);
 }
 
-// One forward (and, with with_k, Jacobian) evaluation of the jobs ids at
-// their current state: fills obs_i, and for retrievals the scaled compact K,
-// for Jacobian requests the output K.
-int Engine::eval(const std::vector<int> &ids, bool with_k) {
+// One forward evaluation of the jobs ids at their current state, with the
+// Jacobian for those with wantk set: fills obs_i, and for retrievals the
+// scaled compact K, for Jacobian requests the output K.  With the adjoint
+// every evaluation is a Jacobian-mode pass (jobs without wantk have no state
+// map and skip the reverse sweep), so a job's forward model goes through the
+// same arithmetic whatever else shares its pass.
+int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
   if (ids.empty()) return JUR_OK;
+  bool with_k = false;
+  for (char w : wantk) with_k |= w != 0;
   const int nd = ctl->nd;
   int rc;
   if (with_k && method == JUR_JAC_FD) {
+    // (the finite-difference path is called with all or none of wantk set)
     // kernel() per job (its n + 1 forward models in one device pass)
     for (int r : ids) {
       Job &J = *jobs[r];
-      memcpy(J.obs_i, J.obs_meas, sizeof(obs_t));
+      copy_obs_rows(J.obs_i, J.obs_meas, nd);
       std::vector<double> kh((size_t)J.mv * J.n + 1);
-      if ((rc = kernel_fd_host(J.sctl, J.atm_i, J.obs_i, kh.data(), J.mv, J.n, precision))) {
+      if ((rc = kernel_fd_host(J.sctl, J.kind == JUR_REQ_RETRIEVAL ? J.atm_i : J.atm_apr, J.obs_i, kh.data(),
+                               J.mv, J.n, precision))) {
         fail(&J, rc);
         continue;
       }
@@ -322,7 +493,8 @@ int Engine::eval(const std::vector<int> &ids, bool with_k) {
           for (int j = 0; j < J.n; j++) kc[(size_t)i * J.n + j] = kh[v * J.n + j] * J.sig[i];
           v++;
         }
-      HIPCHK(hipMemcpy(J.d_k.p, kc.data(), kc.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpyAsync(J.dp[3], kc.data(), kc.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
     }
     return JUR_OK;
   }
@@ -336,12 +508,14 @@ int Engine::eval(const std::vector<int> &ids, bool with_k) {
   const int np = (int)ids.size();
   std::vector<atm_t> atms(np);
   std::vector<int> prof(nray), xm, pmap(np);
+  int nmap = 0;
+  const bool jac_pass = with_k || method == JUR_JAC_ADJOINT;
   std::vector<double> geom((size_t)7 * nray);
   std::vector<unsigned char> mask((size_t)nray * nd);
   for (int k = 0; k < np; k++) {
     Job &J = *jobs[ids[k]];
     memcpy(&atms[k], J.kind == JUR_REQ_RETRIEVAL ? J.atm_i : J.atm_apr, sizeof(atm_t));
-    pmap[k] = k;
+    pmap[k] = wantk[k] ? nmap++ : -1;
     for (int ir = 0; ir < J.nr; ir++) {
       const int i = J.ray0 + ir;
       prof[i] = k;
@@ -355,22 +529,24 @@ int Engine::eval(const std::vector<int> &ids, bool with_k) {
       g[6] = J.obs_meas->vplat[ir];
       for (int id = 0; id < nd; id++) mask[(size_t)i * nd + id] = !isfinite(J.obs_meas->rad[id][ir]);
     }
-    if (with_k) xm.insert(xm.end(), J.xmap.begin(), J.xmap.end());
+    if (wantk[k]) xm.insert(xm.end(), J.xmap.begin(), J.xmap.end());
   }
-  if ((rc = p_atm.upload(atms.data(), np))) return rc;
-  if ((rc = p_prof.upload(prof.data(), nray))) return rc;
-  if ((rc = p_geom.upload(geom.data(), geom.size()))) return rc;
-  if ((rc = p_mask.upload(mask.data(), mask.size()))) return rc;
+  if ((rc = p_atm.upload(atms.data(), np, st))) return rc;
+  if ((rc = p_prof.upload(prof.data(), nray, st))) return rc;
+  if ((rc = p_geom.upload(geom.data(), geom.size(), st))) return rc;
+  if ((rc = p_mask.upload(mask.data(), mask.size(), st))) return rc;
   if ((rc = p_rad.reserve((size_t)nray * nd))) return rc;
   if ((rc = p_tau.reserve((size_t)nray * nd))) return rc;
   BatchView B{np, p_atm.p, nray, p_prof.p, p_geom.p};
-  if (with_k) {
-    if ((rc = p_pmap.upload(pmap.data(), np))) return rc;
-    if ((rc = p_xmap.upload(xm.data(), xm.size()))) return rc;
+  if (!with_k) maxn = 1;
+  if (jac_pass) {
+    if ((rc = p_pmap.upload(pmap.data(), np, st))) return rc;
+    if (xm.empty()) xm.assign(JUR_XMAP_LEN, -1);  // no job wants K: one unused map
+    if ((rc = p_xmap.upload(xm.data(), xm.size(), st))) return rc;
     if ((rc = p_k.reserve((size_t)nray * nd * maxn))) return rc;
     JacArgs ja;
     ja.xmap = p_xmap.p;
-    ja.nmap = np;
+    ja.nmap = nmap;
     ja.prof_map = p_pmap.p;
     ja.ncol = maxn;
     ja.k = p_k.p;
@@ -386,17 +562,19 @@ int Engine::eval(const std::vector<int> &ids, bool with_k) {
       fail(jobs[ids[0]], rc);
       return JUR_OK;
     }
-    for (int r : ids)
-      if ((rc = eval(std::vector<int>{r}, with_k))) return rc;
+    for (size_t k = 0; k < ids.size(); k++)
+      if ((rc = eval(std::vector<int>{ids[k]}, std::vector<char>{wantk[k]}))) return rc;
     return JUR_OK;
   }
   h_buf.resize((size_t)nray * nd * 2);
-  HIPCHK(hipMemcpy(h_buf.data(), p_rad.p, (size_t)nray * nd * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(h_buf.data() + (size_t)nray * nd, p_tau.p, (size_t)nray * nd * sizeof(double),
-                   hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(h_buf.data(), p_rad.p, (size_t)nray * nd * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpyAsync(h_buf.data() + (size_t)nray * nd, p_tau.p, (size_t)nray * nd * sizeof(double),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   for (int r : ids) {
     Job &J = *jobs[r];
-    memcpy(J.obs_i, J.obs_meas, sizeof(obs_t));
+    copy_obs_rows(J.obs_i, J.obs_meas, nd);
     for (int ir = 0; ir < J.nr; ir++)
       for (int id = 0; id < nd; id++) {
         const size_t i = (size_t)(J.ray0 + ir) * nd + id;
@@ -407,12 +585,13 @@ int Engine::eval(const std::vector<int> &ids, bool with_k) {
   if (!with_k) return JUR_OK;
   // Jacobian requests: copy their finite rows out
   std::vector<double> rows;
-  for (int r : ids) {
-    Job &J = *jobs[r];
-    if (J.kind != JUR_REQ_JACOBIAN) continue;
+  for (size_t q = 0; q < ids.size(); q++) {
+    Job &J = *jobs[ids[q]];
+    if (J.kind != JUR_REQ_JACOBIAN || !wantk[q]) continue;
     rows.resize((size_t)J.m * maxn);
-    HIPCHK(hipMemcpy(rows.data(), p_k.p + (size_t)J.ray0 * nd * maxn, rows.size() * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(rows.data(), p_k.p + (size_t)J.ray0 * nd * maxn, rows.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     size_t v = 0;
     for (int ir = 0; ir < J.nr; ir++)
       for (int id = 0; id < nd; id++)
@@ -424,10 +603,10 @@ int Engine::eval(const std::vector<int> &ids, bool with_k) {
   // retrievals: scaled compact K
   std::vector<int> lm;
   int maxrows = 0;
-  for (int r : ids)
-    if (jobs[r]->kind == JUR_REQ_RETRIEVAL) {
-      lm.push_back(r);
-      maxrows = std::max(maxrows, jobs[r]->m);
+  for (size_t q = 0; q < ids.size(); q++)
+    if (wantk[q] && jobs[ids[q]]->kind == JUR_REQ_RETRIEVAL) {
+      lm.push_back(ids[q]);
+      maxrows = std::max(maxrows, jobs[ids[q]]->m);
     }
   if (lm.empty()) return JUR_OK;
   if ((rc = upload_desc())) return rc;
@@ -446,29 +625,66 @@ int Engine::chisq(const std::vector<int> &ids) {
   if (live.empty()) return JUR_OK;
   const int nd = ctl->nd;
   int rc;
-  for (int r : live) {
-    Job &J = *jobs[r];
+  std::vector<size_t> off(live.size());
+  size_t tot = 0;
+  for (size_t k = 0; k < live.size(); k++) {
+    off[k] = tot;
+    tot += jobs[live[k]]->n + jobs[live[k]]->m;
+  }
+  h_buf.resize(tot);
+  for (size_t k = 0; k < live.size(); k++) {
+    Job &J = *jobs[live[k]];
     for (int i = 0; i < J.n; i++) J.dx[i] = J.x_i[i] - J.x_a[i];
     for (int ir = 0; ir < J.nr; ir++)
       for (int id = 0; id < nd; id++) {
         const int i = ir * nd + id;
         J.dy[i] = J.sig[i] != 0 ? J.y_m[i] - J.obs_i->rad[id][ir] : 0.0;
       }
-    HIPCHK(hipMemcpy(J.d_dx.p, J.dx.data(), J.n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(J.d_dy.p, J.dy.data(), J.m * sizeof(double), hipMemcpyHostToDevice));
+    memcpy(&h_buf[off[k]], J.dx.data(), J.n * sizeof(double));
+    memcpy(&h_buf[off[k] + J.n], J.dy.data(), J.m * sizeof(double));
   }
   if ((rc = upload_desc())) return rc;
   if ((rc = list(live))) return rc;
+  if ((rc = d_stage.upload(h_buf.data(), tot, st))) return rc;
+  if ((rc = d_off.upload(off.data(), off.size(), st))) return rc;
+  launch_lm_stage(d_desc.p, d_list.p, (int)live.size(), d_off.p, d_stage.p, 0, st);
+  HIPCHK(hipGetLastError());
   if ((rc = d_q.reserve(jobs.size()))) return rc;
   launch_lm_quad(d_desc.p, d_list.p, (int)live.size(), d_q.p, st);
   HIPCHK(hipGetLastError());
   std::vector<double> q(jobs.size());
-  HIPCHK(hipMemcpy(q.data(), d_q.p, jobs.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(q.data(), d_q.p, jobs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   for (int r : live) {
     Job &J = *jobs[r];
     double cm = 0;
     for (int i = 0; i < J.m; i++) cm += POW2(J.dy[i] * J.sig[i]);
     J.chisq = (cm + q[r]) / (double)J.mv;
+  }
+  return JUR_OK;
+}
+
+// b (which = 1) or step (which = 2) of the jobs ids to the host.
+int Engine::stage_get(const std::vector<int> &ids, int which) {
+  if (ids.empty()) return JUR_OK;
+  std::vector<size_t> off(ids.size());
+  size_t tot = 0;
+  for (size_t k = 0; k < ids.size(); k++) {
+    off[k] = tot;
+    tot += jobs[ids[k]]->n;
+  }
+  int rc;
+  if ((rc = d_stage.reserve(tot))) return rc;
+  if ((rc = d_off.upload(off.data(), off.size(), st))) return rc;
+  if ((rc = list(ids))) return rc;
+  launch_lm_stage(d_desc.p, d_list.p, (int)ids.size(), d_off.p, d_stage.p, which, st);
+  HIPCHK(hipGetLastError());
+  h_buf.resize(tot);
+  HIPCHK(hipMemcpyAsync(h_buf.data(), d_stage.p, tot * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (size_t k = 0; k < ids.size(); k++) {
+    Job &J = *jobs[ids[k]];
+    memcpy(which == 1 ? J.b.data() : J.step.data(), &h_buf[off[k]], J.n * sizeof(double));
   }
   return JUR_OK;
 }
@@ -485,8 +701,25 @@ int Engine::round(std::vector<Job *> &done) {
     if (s == ST_EVAL_F || s == ST_EVAL_F_OUT) need_f.push_back((int)r);
     if (s == ST_EVAL_FK0 || s == ST_EVAL_FK || s == ST_EVAL_FK_OUT) need_fk.push_back((int)r);
   }
-  if ((rc = eval(need_fk, true))) return rc;
-  if ((rc = eval(need_f, false))) return rc;
+  auto tnow = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+  const double t0 = tnow();
+  double t1;
+  if (method == JUR_JAC_FD) {
+    if ((rc = eval(need_fk, std::vector<char>(need_fk.size(), 1)))) return rc;
+    t1 = tnow();
+    if ((rc = eval(need_f, std::vector<char>(need_f.size(), 0)))) return rc;
+  } else {
+    // one pass for every evaluation of the round
+    std::vector<int> all(need_fk);
+    all.insert(all.end(), need_f.begin(), need_f.end());
+    std::vector<char> wk(all.size(), 0);
+    for (size_t q = 0; q < need_fk.size(); q++) wk[q] = 1;
+    if ((rc = eval(all, wk))) return rc;
+    t1 = tnow();
+  }
+  const double t2 = tnow();
+  prof[1] += t1 - t0;
+  prof[2] += t2 - t1;
   for (int r : need_f)
     if (jobs[r]->st == ST_EVAL_F_OUT) jobs[r]->st = ST_DONE;
   for (int r : need_fk)
@@ -576,8 +809,7 @@ int Engine::round(std::vector<Job *> &done) {
     if ((rc = list(need_rhs))) return rc;
     launch_lm_rhs(d_desc.p, d_list.p, (int)need_rhs.size(), maxn_rhs, st);
     HIPCHK(hipGetLastError());
-    for (int r : need_rhs)
-      HIPCHK(hipMemcpy(jobs[r]->b.data(), jobs[r]->d_b.p, jobs[r]->n * sizeof(double), hipMemcpyDeviceToHost));
+    if ((rc = stage_get(need_rhs, 1))) return rc;
   }
   // ---- trial steps ----
   std::vector<int> trial;
@@ -586,21 +818,22 @@ int Engine::round(std::vector<Job *> &done) {
   if (!trial.empty()) {
     std::vector<double> lm(nj);
     for (size_t r = 0; r < nj; r++) lm[r] = jobs[r]->lmpar;
-    if ((rc = d_lmpar.upload(lm.data(), nj))) return rc;
+    if ((rc = d_lmpar.upload(lm.data(), nj, st))) return rc;
     if ((rc = d_status.reserve(nj))) return rc;
     if ((rc = upload_desc())) return rc;
     if ((rc = list(trial))) return rc;
     launch_lm_solve(d_desc.p, d_list.p, (int)trial.size(), d_lmpar.p, d_status.p, st);
     HIPCHK(hipGetLastError());
     std::vector<int> status(nj);
-    HIPCHK(hipMemcpy(status.data(), d_status.p, nj * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(status.data(), d_status.p, nj * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if ((rc = stage_get(trial, 2))) return rc;
     for (int r : trial) {
       Job &J = *jobs[r];
       if (status[r]) {
         fail(&J, set_err(JUR_ERR_ARG, "normal matrix is not positive definite"));
         continue;
       }
-      HIPCHK(hipMemcpy(J.step.data(), J.d_step.p, J.n * sizeof(double), hipMemcpyDeviceToHost));
       for (int i = 0; i < J.n; i++) J.x_i[i] += J.step[i];
       memcpy(J.atm_i, J.atm_apr, sizeof(atm_t));
       x2atm_h(J.sctl, J.x_i.data(), J.atm_i);
@@ -608,11 +841,13 @@ int Engine::round(std::vector<Job *> &done) {
       J.st = ST_EVAL_F;
     }
   }
+  prof[3] += tnow() - t2;
   // ---- finished jobs leave ----
   std::vector<Job *> keep;
   for (Job *J : jobs) {
     if (J->st == ST_DONE) {
       if (J->niter) *J->niter = J->it;
+      release(J);
       done.push_back(J);
     } else {
       keep.push_back(J);
@@ -654,6 +889,7 @@ int optimal_estimation_batch(const jur_ret_t *ret, const ctl_t *ctl, int n, obs_
     if ((rc = E.add(&J))) return set_err(rc, "retrieval %d: %s", r, J.err);
   }
   std::vector<Job *> done;
+  if ((rc = E.setup(done))) return rc;
   while (!E.jobs.empty())
     if ((rc = E.round(done))) return rc;
   for (int r = 0; r < n; r++)
@@ -666,11 +902,20 @@ int optimal_estimation_batch(const jur_ret_t *ret, const ctl_t *ctl, int n, obs_
 // ------------------------------------------------------------------
 // The request server.
 // ------------------------------------------------------------------
-struct jur_server {
+// One worker of the server: its own engine, device context (work buffers)
+// and HIP stream, so that lanes run their passes concurrently.
+struct jur_lane {
   jur::Engine E;
   std::unique_ptr<jur_ctx> ctx;
+  hipStream_t st = nullptr;
+  std::thread th;
+};
+
+struct jur_server {
+  std::vector<std::unique_ptr<jur_lane>> lanes;
   ctl_t ctl;
   jur_ret_t ret;
+  int device = 0;
   size_t budget = 0;
   int max_batch = 0;
   std::mutex mu;
@@ -679,55 +924,61 @@ struct jur_server {
   std::unordered_map<uint64_t, std::unique_ptr<jur::Job>> all;
   uint64_t next_id = 1;
   size_t in_use = 0;
+  long in_flight = 0;
   bool stop = false;
   int fatal = JUR_OK;
   char fatal_msg[256] = "";
-  std::thread worker;
   // statistics
   std::atomic<long> rounds{0}, max_active{0};
 
-  void loop();
+  void loop(jur_lane *L);
 };
 
-void jur_server::loop() {
+void jur_server::loop(jur_lane *L) {
   using namespace jur;
-  if (hipSetDevice(ctx->device) != hipSuccess) {
+  auto now = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+  if (hipSetDevice(device) != hipSuccess) {
     std::lock_guard<std::mutex> lk(mu);
     fatal = JUR_ERR_HIP;
+    snprintf(fatal_msg, sizeof fatal_msg, "hipSetDevice failed");
     cv_out.notify_all();
     return;
   }
+  Engine &E = L->E;
   for (;;) {
     std::vector<Job *> admit;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_in.wait(lk, [&] { return stop || !queue.empty() || !E.jobs.empty(); });
-      if (stop) return;
-      // admission: in arrival order while the device footprint fits
+      cv_in.wait(lk, [&] { return stop || fatal || !queue.empty() || !E.jobs.empty(); });
+      if (stop || fatal) return;
+      // admission: in arrival order while the device footprint fits (an
+      // idle lane always takes one request)
       while (!queue.empty()) {
         Job *J = queue.front();
         const size_t need = (size_t)J->obs_meas->nr * ctl.nd * 8 * 4 +
                             (J->kind == JUR_REQ_FORWARD ? 0 : (size_t)J->obs_meas->nr * ctl.nd * NP * 8 * 2);
-        if (!E.jobs.empty() && (in_use + need > budget ||
-                                (max_batch > 0 && (int)(E.jobs.size() + admit.size()) >= max_batch)))
-          break;
+        const bool idle = E.jobs.empty() && admit.empty();
+        if (!idle && (in_use + need > budget || (max_batch > 0 && in_flight >= max_batch))) break;
         queue.pop_front();
         in_use += need;
+        in_flight++;
         J->admit_bytes = need;
         admit.push_back(J);
       }
     }
-    const double t0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    const double t0 = now();
     std::vector<Job *> done;
     for (Job *J : admit) {
       J->t_start = t0;
       if (E.add(J)) done.push_back(J);  // failed at admission
     }
-    int rc = E.jobs.empty() ? JUR_OK : E.round(done);
+    int rc = E.setup(done);
+    E.prof[0] += now() - t0;
+    if (!rc && !E.jobs.empty()) rc = E.round(done);
     rounds++;
-    max_active = std::max<long>(max_active, (long)E.jobs.size() + (long)done.size());
-    const double t1 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    const double t1 = now();
     std::lock_guard<std::mutex> lk(mu);
+    max_active = std::max<long>(max_active, in_flight);
     if (rc) {
       fatal = rc;
       snprintf(fatal_msg, sizeof fatal_msg, "%s", jur::g_err);
@@ -737,23 +988,24 @@ void jur_server::loop() {
         done.push_back(J);
       }
       E.jobs.clear();
+      for (Job *J : queue) {
+        J->rc = rc;
+        snprintf(J->err, sizeof J->err, "%s", jur::g_err);
+        J->t_done = t1;
+      }
+      queue.clear();
     }
     for (Job *J : done) {
-      // the job's device buffers go back now; results stay on the host
-      J->d_sainv.release();
-      J->d_cmat.release();
-      J->d_a.release();
-      J->d_k.release();
-      J->d_sig.release();
-      J->d_dy.release();
-      J->d_dx.release();
-      J->d_b.release();
-      J->d_step.release();
       J->t_done = t1;
       in_use -= std::min(in_use, J->admit_bytes);
+      in_flight--;
       J->st = ST_DONE;
     }
-    if (!done.empty()) cv_out.notify_all();
+    if (!done.empty() || rc) cv_out.notify_all();
+    if (rc) {
+      cv_in.notify_all();
+      return;  // the device state is unknown: stop serving
+    }
   }
 }
 
@@ -769,27 +1021,41 @@ int jur_gpu_optimal_estimation(const jur_ret_t *ret, const ctl_t *ctl, int n, ob
 int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret, const jur_server_opts_t *opt, jur_server_t **out) {
   using namespace jur;
   if (!ctl || !ret || !out) return set_err(JUR_ERR_ARG, "NULL argument");
+  const int method = opt ? opt->method : JUR_JAC_ADJOINT;
+  const int precision = opt ? opt->precision : JUR_FP32;
+  const int nlanes = (opt && opt->lanes > 0) ? opt->lanes : 1;
+  if (method != JUR_JAC_FD && method != JUR_JAC_ADJOINT) return set_err(JUR_ERR_ARG, "invalid method");
+  if (precision != JUR_FP64 && precision != JUR_FP32) return set_err(JUR_ERR_ARG, "invalid precision");
+  if (nlanes > 16) return set_err(JUR_ERR_ARG, "at most 16 lanes");
   std::unique_ptr<jur_server> s(new jur_server());
   memcpy(&s->ctl, ctl, sizeof(ctl_t));
   memcpy(&s->ret, ret, sizeof(jur_ret_t));
-  jur_ctx_t *c = nullptr;
-  int rc = jur_ctx_create(ctl, &c);
-  if (rc) return rc;
-  s->ctx.reset(c);
-  s->E.ret = &s->ret;
-  s->E.ctl = &s->ctl;
-  s->E.c = c;
-  s->E.method = opt ? opt->method : JUR_JAC_ADJOINT;
-  s->E.precision = opt ? opt->precision : JUR_FP32;
   s->budget = (opt && opt->max_bytes) ? opt->max_bytes : ((size_t)16 << 30);
   s->max_batch = opt ? opt->max_batch : 0;
-  if (s->E.method != JUR_JAC_FD && s->E.method != JUR_JAC_ADJOINT) return set_err(JUR_ERR_ARG, "invalid method");
-  if (s->E.precision != JUR_FP64 && s->E.precision != JUR_FP32) return set_err(JUR_ERR_ARG, "invalid precision");
-  if (s->E.method == JUR_JAC_ADJOINT && (rc = check_jac_ctl(c))) return rc;
-  if (c->fov_dz.size()) return set_err(JUR_ERR_UNSUPPORTED, "the server does not apply a FOV");
-  if (hipGetDevice(&c->device) != hipSuccess) return set_err(JUR_ERR_HIP, "hipGetDevice failed");
+  if (hipGetDevice(&s->device) != hipSuccess) return set_err(JUR_ERR_HIP, "hipGetDevice failed");
+  int rc;
+  for (int l = 0; l < nlanes; l++) {
+    std::unique_ptr<jur_lane> L(new jur_lane());
+    jur_ctx_t *c = nullptr;
+    if ((rc = jur_ctx_create(ctl, &c))) return rc;
+    L->ctx.reset(c);
+    if (method == JUR_JAC_ADJOINT && (rc = check_jac_ctl(c))) return rc;
+    if (c->fov_dz.size()) return set_err(JUR_ERR_UNSUPPORTED, "the server does not apply a FOV");
+    // default priority, non-blocking: lanes overlap on the device
+    HIPCHK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+    L->E.ret = &s->ret;
+    L->E.ctl = &s->ctl;
+    L->E.c = c;
+    L->E.method = method;
+    L->E.precision = precision;
+    L->E.st = L->st;
+    s->lanes.push_back(std::move(L));
+  }
   jur_server *raw = s.get();
-  raw->worker = std::thread([raw] { raw->loop(); });
+  for (auto &L : raw->lanes) {
+    jur_lane *lp = L.get();
+    L->th = std::thread([raw, lp] { raw->loop(lp); });
+  }
   *out = s.release();
   return JUR_OK;
 }
@@ -801,7 +1067,12 @@ void jur_server_destroy(jur_server_t *s) {
     s->stop = true;
   }
   s->cv_in.notify_all();
-  if (s->worker.joinable()) s->worker.join();
+  for (auto &L : s->lanes)
+    if (L->th.joinable()) L->th.join();
+  for (auto &L : s->lanes) {
+    L->E.jobs.clear();
+    if (L->st) (void)hipStreamDestroy(L->st);
+  }
   delete s;
 }
 
@@ -810,11 +1081,28 @@ int jur_server_submit(jur_server_t *s, const jur_request_t *req, uint64_t *id) {
   if (!s || !req || !id || !req->atm || !req->obs) return set_err(JUR_ERR_ARG, "NULL argument");
   if (req->kind != JUR_REQ_FORWARD && req->kind != JUR_REQ_JACOBIAN && req->kind != JUR_REQ_RETRIEVAL)
     return set_err(JUR_ERR_ARG, "invalid request kind");
+  const double t_submit = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  if (req->obs->nr < 1 || req->obs->nr > NR) return set_err(JUR_ERR_ARG, "Too many rays!");
   std::unique_ptr<Job> J(new Job());
   J->kind = req->kind;
+  // The job's ctl only defines its state vector (atm2x and friends read the
+  // counts and the RET* fields); the forward model is the server's.  The
+  // finite-difference Jacobian runs formod on it and needs all of it.
   J->own_ctl.reset(new ctl_t);
-  memcpy(J->own_ctl.get(), &s->ctl, sizeof(ctl_t));
   ctl_t *sc = J->own_ctl.get();
+  if (s->lanes[0]->E.method == JUR_JAC_FD) {
+    memcpy(sc, &s->ctl, sizeof(ctl_t));
+  } else {
+    sc->ng = s->ctl.ng;
+    sc->nd = s->ctl.nd;
+    sc->nw = s->ctl.nw;
+    sc->ncl = s->ctl.ncl;
+    sc->nsf = s->ctl.nsf;
+    sc->retp_zmin = sc->retp_zmax = sc->rett_zmin = sc->rett_zmax = -999;
+    for (int ig = 0; ig < NG; ig++) sc->retq_zmin[ig] = sc->retq_zmax[ig] = -999;
+    for (int iw = 0; iw < NW; iw++) sc->retk_zmin[iw] = sc->retk_zmax[iw] = -999;
+    sc->ret_clz = sc->ret_cldz = sc->ret_clk = sc->ret_sfz = sc->ret_sfp = sc->ret_sft = sc->ret_sfeps = 0;
+  }
   if (req->kind != JUR_REQ_FORWARD) {
     // state windows of the request
     sc->retp_zmin = req->retp_zmin;
@@ -836,7 +1124,7 @@ int jur_server_submit(jur_server_t *s, const jur_request_t *req, uint64_t *id) {
   memcpy(J->own_apr.get(), req->atm, sizeof(atm_t));
   J->own_atm_i.reset(new atm_t);
   J->own_meas.reset(new obs_t);
-  memcpy(J->own_meas.get(), req->obs, sizeof(obs_t));
+  copy_obs_rows(J->own_meas.get(), req->obs, s->ctl.nd);
   J->own_obs_i.reset(new obs_t);
   J->atm_apr = J->own_apr.get();
   J->atm_i = J->own_atm_i.get();
@@ -853,7 +1141,7 @@ int jur_server_submit(jur_server_t *s, const jur_request_t *req, uint64_t *id) {
     J->hist = J->own_hist.data();
     J->niter = &J->own_niter;
   }
-  J->t_submit = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  J->t_submit = t_submit;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->fatal) return set_err(s->fatal, "server stopped: %s", s->fatal_msg);
@@ -907,7 +1195,7 @@ int jur_server_fetch(jur_server_t *s, uint64_t id, obs_t *obs, atm_t *atm, doubl
     J = std::move(it->second);
     s->all.erase(it);
   }
-  if (obs) memcpy(obs, J->obs_i, sizeof(obs_t));
+  if (obs) copy_obs_rows(obs, J->obs_i, s->ctl.nd);
   if (atm) memcpy(atm, J->kind == JUR_REQ_RETRIEVAL ? J->atm_i : J->atm_apr, sizeof(atm_t));
   if (k && J->kind == JUR_REQ_JACOBIAN) memcpy(k, J->own_k.data(), (size_t)J->n * J->mv * sizeof(double));
   if (chisq_hist && J->kind == JUR_REQ_RETRIEVAL)
@@ -919,6 +1207,15 @@ int jur_server_stats(const jur_server_t *s, long *rounds, long *max_active) {
   if (!s) return jur::set_err(JUR_ERR_ARG, "NULL argument");
   if (rounds) *rounds = s->rounds;
   if (max_active) *max_active = s->max_active;
+  return JUR_OK;
+}
+
+int jur_server_profile(const jur_server_t *s, double *sec, int n) {
+  if (!s || !sec) return jur::set_err(JUR_ERR_ARG, "NULL argument");
+  for (int i = 0; i < n && i < 4; i++) {
+    sec[i] = 0;
+    for (auto &L : s->lanes) sec[i] += L->E.prof[i];
+  }
   return JUR_OK;
 }
 

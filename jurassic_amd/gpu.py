@@ -43,7 +43,7 @@ EXPORTS = [
     "jur_ctx_create_wide", "jur_ctx_destroy", "jur_ctx_nd", "jur_ctx_formod_device",
     "jur_ctx_last_timing", "jur_ctx_last_mean_np", "jur_state_map", "jur_ctx_jacobian_device",
     "jur_gpu_optimal_estimation", "jur_server_create", "jur_server_destroy", "jur_server_submit",
-    "jur_server_wait", "jur_server_fetch", "jur_server_stats",
+    "jur_server_wait", "jur_server_fetch", "jur_server_stats", "jur_server_profile",
 ]
 
 
@@ -100,6 +100,8 @@ def lib():
         L.jur_server_fetch.restype = I
         L.jur_server_stats.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.jur_server_stats.restype = I
+        L.jur_server_profile.argtypes = [P, C.POINTER(D), I]
+        L.jur_server_profile.restype = I
         if L.jur_gpu_abi_check(C.sizeof(abi.ctl_t), C.sizeof(abi.atm_t), C.sizeof(abi.obs_t)):
             raise OSError("ABI mismatch between jurassic_amd.abi and libjurassic_gpu.so")
         _LIB = L
@@ -228,10 +230,10 @@ class Server:
     retrievals submitted at any time, batched continuously on the GPU."""
 
     def __init__(self, ctl, ret, method=abi.JUR_JAC_ADJOINT, precision=abi.JUR_FP32, max_bytes=0,
-                 max_batch=0):
+                 max_batch=0, lanes=0):
         self.ptr = C.c_void_p()
         self.ret = ret
-        opt = abi.server_opts_t(method, precision, max_bytes, max_batch)
+        opt = abi.server_opts_t(method, precision, max_bytes, max_batch, lanes)
         _check(lib().jur_server_create(C.byref(ctl), C.byref(ret), C.byref(opt), C.byref(self.ptr)))
 
     @staticmethod
@@ -274,10 +276,20 @@ class Server:
             h = h[:res.niter + 1]
         return obs, atm, k, h
 
+    def release(self, rid):
+        """Drop a finished request without copying its outputs; returns its status."""
+        return lib().jur_server_fetch(self.ptr, rid, None, None, None, None)
+
     def stats(self):
         a, b = C.c_long(), C.c_long()
         _check(lib().jur_server_stats(self.ptr, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def profile(self):
+        """Worker seconds: admission, Jacobian passes, forward passes, dense algebra."""
+        t = (C.c_double * 4)()
+        _check(lib().jur_server_profile(self.ptr, t, 4))
+        return list(t)
 
     def close(self):
         if self.ptr:
