@@ -48,6 +48,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "radiances/s fwd; Jacobians/s adjoint vs FD; p50/p99 retrieval latency"
 HBM_PEAK = 8.0e12
+# Measured HBM bytes per k_lookup_f32 launch at the default workload (4096
+# profiles): rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a
+# 1 GiB device copy under the same counters), in
+# profiles/k_lookup_f32_limb_4096x32x12_hbm_traffic.md.  Other workload sizes
+# report null.
+LOOKUP_F32_TRAFFIC = (4096, 53.67e9)
 NPROF, NRAY_PER, GASES = 4096, 32, ["CO2", "H2O", "O3"]
 
 
@@ -529,7 +535,9 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": LOOKUP_F32_TRAFFIC[1] if nprof == LOOKUP_F32_TRAFFIC[0] else None,
+            "traffic_hbm_rate_GBps": (LOOKUP_F32_TRAFFIC[1] / (k32[1] * 1e-3) / 1e9
+                                      if nprof == LOOKUP_F32_TRAFFIC[0] else None),
             "kernel": "k_lookup_f32<false>",
             "kernel_ms": k32[1],
             "bytes_per_launch_step": lookup_bytes,

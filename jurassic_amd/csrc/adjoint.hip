@@ -273,9 +273,9 @@ __device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix,
     return d;
   }
   const double *tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
-  int j0 = (int)((tq - (float)tg0[0]) * (float)(1.0 / (tg0[1] - tg0[0])));
+  int j0 = (int)((tq - (float)tg0[0]) * frcp((float)(tg0[1] - tg0[0])));
   j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
-  int j1 = (int)((tq - (float)tg0[T.TT]) * (float)(1.0 / (tg0[T.TT + 1] - tg0[T.TT])));
+  int j1 = (int)((tq - (float)tg0[T.TT]) * frcp((float)(tg0[T.TT + 1] - tg0[T.TT])));
   j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
   const size_t c0 = (size_t)tix * T.TP + ipr;
   int nc[4];
@@ -287,8 +287,8 @@ __device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix,
     d.dep = 1;
     return d;
   }
-  const float ta0 = (float)tg0[j0], dt0 = (float)(1.0 / (tg0[j0 + 1] - tg0[j0]));
-  const float ta1 = (float)tg0[T.TT + j1], dt1 = (float)(1.0 / (tg0[T.TT + j1 + 1] - tg0[T.TT + j1]));
+  const float ta0 = (float)tg0[j0], dt0 = frcp((float)(tg0[j0 + 1] - tg0[j0]));
+  const float ta1 = (float)tg0[T.TT + j1], dt1 = frcp((float)(tg0[T.TT + j1 + 1] - tg0[T.TT + j1]));
   const float w0 = (tq - ta0) * dt0, w1 = (tq - ta1) * dt1;
   float wp, dwp;
   if (CGA && pq / pa > 0 && pb / pa > 0) {
@@ -296,7 +296,7 @@ __device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix,
     wp = __logf((float)(pq / pa)) * lpr;
     dwp = lpr / (float)pq;
   } else {
-    dwp = (float)(1.0 / (pb - pa));
+    dwp = frcp((float)(pb - pa));
     wp = (float)(pq - pa) * dwp;
   }
   const float q0 = 1.f - wp, q1 = wp;
@@ -337,12 +337,12 @@ __device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix,
         e = eps_rec_f_vs(r2, arg, se);
         de = se * su;
       } else if (ep > r.w) {
+        // e = 1 - (1 - ep) exp(a u_seg), as ega_step_slow evaluates it
         const float a = fdiv(__logf(1.f - r.w), r.z);
-        const double arg = (double)fdiv(log1pf(-ep), a) + x;
-        const float ex = __expf(a * (float)arg);
-        e = 1.f - ex;
-        se = ex > 0.f ? -a * ex : 0.f;
-        de = ex > 0.f ? ex * a * frcp(a) / (1.f - ep) : 0.f;
+        const float ex = __expf(a * (float)x);
+        e = 1.f - (1.f - ep) * ex;
+        se = ex > 0.f ? -a * (1.f - ep) * ex : 0.f;
+        de = ex;
       } else {
         const float dur = r.z - r.x, der = r.w - r.y;
         su = fdiv(dur, der);

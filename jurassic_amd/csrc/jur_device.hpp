@@ -273,6 +273,103 @@ __device__ __forceinline__ float eps_from_rec_f(const float4 &r, double x) {
   return r.y + fdiv(r.w - r.y, r.z - r.x) * (float)(x - (double)r.x);
 }
 
+// col_walk_f with the record at the (clamped) start index already in hand.
+template <int C>
+__device__ __forceinline__ int col_walk_f_r(const float4 *col, int n, float x, int g, const float4 &rg,
+                                            float4 &r) {
+  const bool left = g > 0 && lo_of<C>(rg) > x;
+  const bool right = g < n - 2 && hi_of<C>(rg) <= x;
+  if (!left && !right) {
+    r = rg;
+    return g;
+  }
+  return col_walk_f<C>(col, n, x, g, r);
+}
+
+// One EGA growth step on one table column, FP32 path: from the path
+// emissivity ep, u_eq = u(ep) (intpol_tbl_u, src/jurassic.c:3933-3967), then
+// eps(u_eq + u_seg) (intpol_tbl_eps, :3895-3929), both evaluated relative to
+// the knot of the inverse lookup's bracket r.
+//
+// ega_step_fast covers the common case without branches: ep inside the eps
+// bracket of the record at the previous index (no walk, not below the first
+// or above the last knot) and u_eq + u_seg still inside its u bracket.
+// Otherwise slow is set and ega_step_slow gives the result.
+__device__ __forceinline__ void ega_step_fast(const float4 &r, float ep, float us, bool gt0, bool ltn2,
+                                              float &ec, bool &slow) {
+  const float du = r.z - r.x, de = r.w - r.y;
+  const float off = (ep - r.y) * fdiv(du, de) + us;
+  const bool walk = (gt0 && r.y > ep) || (ltn2 && r.w <= ep);
+  const bool outside = ep < r.y || ep > r.w;
+  slow = walk || outside || !(off < du);
+  ec = r.y + off * fdiv(de, du);
+}
+
+// The fast step with the records at g and g + 1 in hand (r0 = col[g], r1 =
+// col[min(g + 1, n - 2)]): it also covers the inverse bracket moving up by one
+// knot and the grown column crossing into the next u bracket, which is what a
+// growing path emissivity does every few segments.  Results are those of
+// ega_step_slow for the same inputs (same brackets, same formulas); slow is
+// set for everything else (walks of more than one knot, the extrapolation
+// branches below the first and above the last knot).  g returns the inverse
+// lookup's bracket index.
+__device__ __forceinline__ void ega_step_fast2(const float4 &r0, const float4 &r1, int n, float ep, float us,
+                                               double useg, int &g, float &ec, bool &slow) {
+  const bool top = g >= n - 2;
+  // inverse bracket: stay at g, or move to g + 1
+  const bool stay = (g == 0 || r0.y <= ep) && (top || r0.w > ep);
+  const bool up = !top && r0.w <= ep && (g + 1 >= n - 2 || r1.w > ep);
+  const float4 rb = stay ? r0 : r1;
+  const int b = stay ? g : g + 1;
+  bool ok = (stay || up) && !(ep < rb.y) && !(ep > rb.w);
+  const float du = rb.z - rb.x, de = rb.w - rb.y;
+  const float off = (ep - rb.y) * fdiv(du, de) + us;
+  const float e_in = rb.y + off * fdiv(de, du);
+  // forward bracket when off leaves the inverse bracket: the record after b
+  // (only r1 is in hand, so only from b == g), or b itself where rounding of
+  // du left arg below its upper knot
+  const double arg = (double)rb.x + (double)off;
+  const bool cross = !(off < du);
+  const bool b_top = b >= n - 2;
+  const bool inb = b_top ? !(arg > (double)rb.z) : !((double)rb.z <= arg);
+  const bool nxt = stay && !top && (double)r0.z <= arg && (g + 1 >= n - 2 ? !(arg > (double)r1.z) : (double)r1.z > arg);
+  const float4 rf = inb ? rb : r1;
+  const float e_x = rf.y + fdiv(rf.w - rf.y, rf.z - rf.x) * (float)(arg - (double)rf.x);
+  ok = ok && (!cross || inb || nxt);
+  ec = cross ? e_x : e_in;
+  slow = !ok;
+  g = ok ? b : g;
+}
+
+// The general step, starting the inverse search at g (record rg = col[g] in
+// hand); g returns the inverse lookup's bracket index.
+__device__ __forceinline__ float ega_step_slow(const float4 *col, int n, float ep, double useg, int &g,
+                                               const float4 &rg) {
+  float4 r;
+  const int i = col_walk_f_r<1>(col, n, ep, g, rg, r);
+  g = i;
+  if (ep < r.y) {
+    // below the first knot: u_eq = ep u_0 / eps_0 (small), absolute
+    const double arg = (double)(fdiv(r.x, r.y) * ep) + useg;
+    float4 r2;
+    col_walk_r<0>(col, n, arg, i, r, r2);
+    return eps_from_rec_f(r2, arg);
+  }
+  if (ep > r.w) {
+    // above the last knot: eps(u) = 1 - exp(a u), a = log(1 - eps_last) /
+    // u_last, so eps(u_eq + u_seg) = 1 - (1 - ep) exp(a u_seg)
+    const float a = fdiv(__logf(1.f - r.w), r.z);
+    return 1.f - (1.f - ep) * __expf(a * (float)useg);
+  }
+  const float du = r.z - r.x, de = r.w - r.y;
+  const float off = (ep - r.y) * fdiv(du, de) + (float)useg;
+  if (off < du) return r.y + off * fdiv(de, du);
+  const double arg = (double)r.x + (double)off;
+  float4 r2;
+  col_walk_r<0>(col, n, arg, i, r, r2);
+  return eps_from_rec_f(r2, arg);
+}
+
 // Reference-order lookups for columns that are not monotone (bisection).
 __device__ __forceinline__ float col_eps_bis(const float4 *col, int n, double x) {
   const float4 r0 = col[0];

@@ -64,11 +64,18 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
   int nt0 = 0, nt1 = 0, n00 = 0, n01 = 0, n10 = 0, n11 = 0;
   const double *tg0 = nullptr;
 
+  // LOS values of the current point; the next point's are loaded after the
+  // corner records of the current one, so that waiting for the records does
+  // not wait for the (HBM) LOS loads as well
+  double p = 0, t = 0, useg = 0;
+  if (R.np > 0) {
+    p = LOSR(F_P, 0);
+    t = LOSR(F_T, 0);
+    useg = LOSR(L.f_u + ig, 0);
+  }
   int ip = 0;
   for (; ip < R.np; ip++) {
-    const double p = LOSR(F_P, ip);
-    const double t = LOSR(F_T, ip);
-    const double useg = LOSR(L.f_u + ig, ip);
+    const int ipn = ip + 1 < R.np ? ip + 1 : ip;
     if (CGA) {
       cu += useg;
       cp += useg * p;
@@ -87,17 +94,18 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, pq, ipr, pdir) : locate_irr(pg, npt, pq);
       pa = pg[ipr];
       pb = pg[ipr + 1];
-      wp_scale = (float)(1.0 / (pb - pa));
-      lpr = (float)(1.0 / log(pb / pa));
+      wp_scale = frcp((float)(pb - pa));
+      if (CGA) lpr = (float)(1.0 / log(pb / pa));
       nt0 = ntp[ipr];
       nt1 = ntp[ipr + 1];
       tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
       g00 = (float)tg0[0];
-      gi0 = (float)(1.0 / (tg0[1] - tg0[0]));
+      gi0 = frcp((float)(tg0[1] - tg0[0]));
       g10 = (float)tg0[T.TT];
-      gi1 = (float)(1.0 / (tg0[T.TT + 1] - tg0[T.TT]));
+      gi1 = frcp((float)(tg0[T.TT + 1] - tg0[T.TT]));
       it0 = it1 = -1;
     }
+    double pn = p, tn = t, un = useg;
     if (nt0 >= 2 && nt1 >= 2) {
       int j0 = (int)((tq - g00) * gi0);
       j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
@@ -107,14 +115,14 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       if (j0 != it0) {
         it0 = j0;
         ta0 = (float)tg0[it0];
-        dt0 = (float)(1.0 / (tg0[it0 + 1] - tg0[it0]));
+        dt0 = frcp((float)(tg0[it0 + 1] - tg0[it0]));
         n00 = T.nu[c0 * T.TT + it0];
         n01 = T.nu[c0 * T.TT + it0 + 1];
       }
       if (j1 != it1) {
         it1 = j1;
         ta1 = (float)tg0[T.TT + it1];
-        dt1 = (float)(1.0 / (tg0[T.TT + it1 + 1] - tg0[T.TT + it1]));
+        dt1 = frcp((float)(tg0[T.TT + it1 + 1] - tg0[T.TT + it1]));
         n10 = T.nu[(c0 + 1) * T.TT + it1];
         n11 = T.nu[(c0 + 1) * T.TT + it1 + 1];
       }
@@ -127,46 +135,47 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         const int nc[4] = {n00, n01, n10, n11};
         const float us = (float)useg;
         float ec[4];
-        if (!mono) {
+        if (CGA || !mono) {
+          if (!mono) {
 #pragma unroll 1
-          for (int c = 0; c < 4; c++)
-            ec[c] = CGA ? col_eps_bis(kc[c], nc[c], cu)
-                        : col_eps_bis(kc[c], nc[c], col_u_bis(kc[c], nc[c], ep) + useg);
-        } else {
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          float4 r;
-          if (CGA) {
-            gi[c] = col_walk<0>(kc[c], nc[c], cu, gi[c], r);
-            ec[c] = eps_from_rec_f(r, cu);
-            continue;
-          }
-          const int i = col_walk_f<1>(kc[c], nc[c], ep, gi[c], r);
-          gi[c] = i;
-          if (ep < r.y) {
-            // below the first knot: u = ep u_0 / eps_0 (small), absolute
-            const double arg = (double)(fdiv(r.x, r.y) * ep) + useg;
-            float4 r2;
-            col_walk_r<0>(kc[c], nc[c], arg, i, r, r2);
-            ec[c] = eps_from_rec_f(r2, arg);
-          } else if (ep > r.w) {
-            // above the last knot: u = log(1 - ep)/a, a = log(1 - eps_last)/u_last
-            const float a = fdiv(__logf(1.f - r.w), r.z);
-            const double arg = (double)fdiv(log1pf(-ep), a) + useg;
-            ec[c] = 1.f - __expf(a * (float)arg);
+            for (int c = 0; c < 4; c++)
+              ec[c] = CGA ? col_eps_bis(kc[c], nc[c], cu)
+                          : col_eps_bis(kc[c], nc[c], col_u_bis(kc[c], nc[c], ep) + useg);
           } else {
-            const float du = r.z - r.x, de = r.w - r.y;
-            const float off = (ep - r.y) * fdiv(du, de) + us;
-            if (off < du) {
-              ec[c] = r.y + off * fdiv(de, du);
-            } else {
-              const double arg = (double)r.x + (double)off;
-              float4 r2;
-              col_walk_r<0>(kc[c], nc[c], arg, i, r, r2);
-              ec[c] = eps_from_rec_f(r2, arg);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              float4 r;
+              gi[c] = col_walk<0>(kc[c], nc[c], cu, gi[c], r);
+              ec[c] = eps_from_rec_f(r, cu);
             }
           }
-        }
+          pn = LOSR(F_P, ipn);
+          tn = LOSR(F_T, ipn);
+          un = LOSR(L.f_u + ig, ipn);
+        } else {
+          // the bracket records at the previous segment's index and the next
+          float4 r[4], rn[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            int g = gi[c];
+            g = g < 0 ? 0 : (g > nc[c] - 2 ? nc[c] - 2 : g);
+            gi[c] = g;
+            r[c] = kc[c][g];
+            rn[c] = kc[c][g + 1 < nc[c] - 2 ? g + 1 : nc[c] - 2];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          pn = LOSR(F_P, ipn);
+          tn = LOSR(F_T, ipn);
+          un = LOSR(L.f_u + ig, ipn);
+          __builtin_amdgcn_sched_barrier(0);
+          // common case: the inverse and forward brackets are those records
+          bool slow[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) ega_step_fast2(r[c], rn[c], nc[c], ep, us, useg, gi[c], ec[c], slow[c]);
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            if (slow[c]) ec[c] = ega_step_slow(kc[c], nc[c], ep, useg, gi[c], r[c]);
+          }
         }
         const float w0 = (tq - ta0) * dt0;
         const float w1 = (tq - ta1) * dt1;
@@ -181,7 +190,15 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         en = (en < 1.f) ? en : 1.f;
         en = (en > 0.f) ? en : 0.f;
         ep = en;
+      } else {
+        pn = LOSR(F_P, ipn);
+        tn = LOSR(F_T, ipn);
+        un = LOSR(L.f_u + ig, ipn);
       }
+    } else {
+      pn = LOSR(F_P, ipn);
+      tn = LOSR(F_T, ipn);
+      un = LOSR(L.f_u + ig, ipn);
     }
     out[(size_t)ip * WAVE] = ep;
     if (ep >= 1.f) {
@@ -189,6 +206,9 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       for (ip++; ip < R.np; ip++) out[(size_t)ip * WAVE] = 1.f;
       break;
     }
+    p = pn;
+    t = tn;
+    useg = un;
   }
 }
 
