@@ -258,6 +258,47 @@ __device__ __forceinline__ LkPart lookup_partials(const TabView &T, int tix, int
 // FP32 path: partials of one chain step of k_lookup_f32, with its float
 // cell weights, its knot-relative update inside a bracket and its
 // reciprocal-multiply slopes.
+// General FP32 EGA step on one monotone column (ega_step_slow) with its
+// partials de = d eps / d ep and se = d eps / d u_seg; the inverse search
+// starts at g and g returns its bracket.
+__device__ __forceinline__ float ega_corner_f32_d(const float4 *col, int n, float ep, double x, int &g,
+                                                  float &de, float &se) {
+  float4 r;
+  const int i = col_walk_f<1>(col, n, ep, g, r);
+  g = i;
+  float e, su;
+  if (ep < r.y) {
+    su = fdiv(r.x, r.y);
+    const double arg = (double)(su * ep) + x;
+    float4 r2;
+    col_walk_r<0>(col, n, arg, i, r, r2);
+    e = eps_rec_f_vs(r2, arg, se);
+    de = se * su;
+  } else if (ep > r.w) {
+    // e = 1 - (1 - ep) exp(a u_seg), as ega_step_slow evaluates it
+    const float a = fdiv(__logf(1.f - r.w), r.z);
+    const float ex = __expf(a * (float)x);
+    e = 1.f - (1.f - ep) * ex;
+    se = ex > 0.f ? -a * (1.f - ep) * ex : 0.f;
+    de = ex;
+  } else {
+    const float dur = r.z - r.x, der = r.w - r.y;
+    su = fdiv(dur, der);
+    const float off = (ep - r.y) * su + (float)x;
+    if (off < dur) {
+      se = fdiv(der, dur);
+      e = r.y + off * se;
+    } else {
+      const double arg = (double)r.x + (double)off;
+      float4 r2;
+      col_walk_r<0>(col, n, arg, i, r, r2);
+      e = eps_rec_f_vs(r2, arg, se);
+    }
+    de = se * su;
+  }
+  return e;
+}
+
 template <bool CGA>
 __device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix, int npt, int mono, int pdir,
                                                       float ep, double pq, float tq, double x, int &ipr,
@@ -326,38 +367,7 @@ __device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix,
       e = col_eps_bis_vs(col, n, u + x, se);
       de = se * su;
     } else {
-      float4 r;
-      const int i = col_walk_f<1>(col, n, ep, gi[c], r);
-      gi[c] = i;
-      if (ep < r.y) {
-        su = fdiv(r.x, r.y);
-        const double arg = (double)(su * ep) + x;
-        float4 r2;
-        col_walk_r<0>(col, n, arg, i, r, r2);
-        e = eps_rec_f_vs(r2, arg, se);
-        de = se * su;
-      } else if (ep > r.w) {
-        // e = 1 - (1 - ep) exp(a u_seg), as ega_step_slow evaluates it
-        const float a = fdiv(__logf(1.f - r.w), r.z);
-        const float ex = __expf(a * (float)x);
-        e = 1.f - (1.f - ep) * ex;
-        se = ex > 0.f ? -a * (1.f - ep) * ex : 0.f;
-        de = ex;
-      } else {
-        const float dur = r.z - r.x, der = r.w - r.y;
-        su = fdiv(dur, der);
-        const float off = (ep - r.y) * su + (float)x;
-        if (off < dur) {
-          se = fdiv(der, dur);
-          e = r.y + off * se;
-        } else {
-          const double arg = (double)r.x + (double)off;
-          float4 r2;
-          col_walk_r<0>(col, n, arg, i, r, r2);
-          e = eps_rec_f_vs(r2, arg, se);
-        }
-        de = se * su;
-      }
+      e = ega_corner_f32_d(col, n, ep, x, gi[c], de, se);
     }
     ec[c] = e;
     dep += cw[c] * de;
@@ -468,9 +478,13 @@ __device__ __forceinline__ int level_of(const atm_t *atm, double z, int lev, dou
 // lane's tile row when the pair changes.
 // Quantities in the mask ATOM are added atomically (columns that other
 // wavefronts of the same row also update).
+// The pair's altitudes and pressures and the reciprocals the weights need are
+// kept while the pair lasts (a pair spans ~10 LOS points at RAYDZ = 0.1 km).
 template <int NQ, unsigned ATOM = 0>
 struct LevelWindow {
   int wa = -1;
+  double z0 = 0, z1 = 0, rz = 0, rp0 = 0, rp1 = 0;
+  bool plog = false;
   double acc[NQ][2];
   __device__ __forceinline__ void flush(const int *xm, const int *qi, double *kt) {
     if (wa < 0) return;
@@ -486,24 +500,33 @@ struct LevelWindow {
       }
   }
   // Point adjoints v[q] at altitude z; quantity q == qp is pressure (LOGY
-  // weights p/p_level), the others use the LIN weights.
+  // weights p/p_level, src/jurassic.c:3685-3706), the others use the LIN
+  // weights.
   __device__ __forceinline__ void add(const atm_t *atm, double z, int &lev, const double *v, int qp, double p,
                                       const int *xm, const int *qi, double *kt) {
-    double w;
-    lev = level_of(atm, z, lev, w);
-    if (lev != wa) {
-      flush(xm, qi, kt);
-      wa = lev;
+    if (!(wa >= 0 && z0 <= z && z < z1)) {
+      double w;
+      lev = level_of(atm, z, lev, w);
+      if (lev != wa) {
+        flush(xm, qi, kt);
+        wa = lev;
 #pragma unroll
-      for (int q = 0; q < NQ; q++) acc[q][0] = acc[q][1] = 0;
+        for (int q = 0; q < NQ; q++) acc[q][0] = acc[q][1] = 0;
+      }
+      z0 = atm->z[lev];
+      z1 = atm->z[lev + 1];
+      rz = 1 / (z1 - z0);
+      const double P0 = atm->p[lev], P1 = atm->p[lev + 1];
+      plog = P1 / P0 > 0;
+      rp0 = 1 / P0;
+      rp1 = 1 / P1;
     }
-    const double P0 = atm->p[lev], P1 = atm->p[lev + 1];
-    const bool plog = P1 / P0 > 0;
+    const double w = (z - z0) * rz;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       if (q == qp && plog) {
-        acc[q][0] += (1 - w) * p / P0 * v[q];
-        acc[q][1] += w * p / P1 * v[q];
+        acc[q][0] += (1 - w) * (p * rp0) * v[q];
+        acc[q][1] += w * (p * rp1) * v[q];
       } else {
         acc[q][0] += (1 - w) * v[q];
         acc[q][1] += w * v[q];
@@ -725,7 +748,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
 // Reverse sweep of one emitter chain per wavefront: L_g,i from H_i and the
 // recomputed table partials; T and p entries are shared with the other
 // emitters of the row (atomic adds), q_g entries belong to this chain.
-template <bool CGA, typename SF>
+template <bool CGA, typename SF, bool FAST>
 __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, LosView L, EpsView E,
                                                     const RayInfo *info, int blk0, int nblk, JacView J) {
   const int nd = M.nd, ng = M.ng;
@@ -773,6 +796,200 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
   double carry = 0, Hn = 0, sU = 0, sP = 0, sT = 0;
   double trc = nef > 0 ? trans_of(EPS_AT(g, nef - 1)) : 1;
   constexpr bool F32 = sizeof(SF) == sizeof(float);
+  if constexpr (FAST) {
+    static_assert(F32 && !CGA, "the fast chain is the FP32 EGA one");
+    {
+      // FP32 EGA on monotone columns: the cell and its T brackets are kept
+      // from point to point as in k_lookup_f32, and each corner starts from
+      // the record pair below the previous point's inverse bracket (walking
+      // backwards the path emissivity falls), which covers the common case
+      // without a search.
+      const TabView &T = M.tab;
+      const double *pg = T.p + (size_t)tix * T.TP;
+      const int *ntp = T.nt + (size_t)tix * T.TP;
+      int it0 = -1, it1 = -1, nt0 = 0, nt1 = 0, n00 = 0, n01 = 0, n10 = 0, n11 = 0;
+      double pa = 0, pb = 0;
+      float wp_scale = 0, g00 = 0, gi0 = 0, g10 = 0, gi1 = 0, ta0 = 0, dt0 = 0, ta1 = 0, dt1 = 0;
+      const double *tg0 = nullptr;
+      double p = 0, t = 0, useg = 0;
+      float epf = 0.f;
+      if (nef > 0) {
+        p = LOSR(F_P, nef - 1);
+        t = LOSR(F_T, nef - 1);
+        useg = LOSR(fu, nef - 1);
+        epf = nef > 1 ? (float)EPS_AT(g, nef - 2) : 0.f;
+      }
+      for (int ip = nef - 1; ip >= 0; ip--) {
+        const int ipn = ip > 0 ? ip - 1 : 0;
+        const float tq = (float)t;
+        bool keep = false;
+        if (ipr >= 0 && pdir != 0) {
+          if (pdir > 0)
+            keep = (ipr == 0 || !(pa > p)) && (ipr == npt - 2 || pb > p);
+          else
+            keep = (ipr == 0 || !(pa <= p)) && (ipr == npt - 2 || pb <= p);
+        }
+        if (!keep) {
+          const int iprev = ipr;
+          ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, p, ipr, pdir) : locate_irr(pg, npt, p);
+          if (ipr == iprev + 1) {
+            const int a0 = gi[0], a1 = gi[1];
+            gi[0] = gi[2];
+            gi[1] = gi[3];
+            gi[2] = 2 * gi[2] - a0;
+            gi[3] = 2 * gi[3] - a1;
+          } else if (ipr == iprev - 1) {
+            const int a2 = gi[2], a3 = gi[3];
+            gi[2] = gi[0];
+            gi[3] = gi[1];
+            gi[0] = 2 * gi[0] - a2;
+            gi[1] = 2 * gi[1] - a3;
+          }
+          pa = pg[ipr];
+          pb = pg[ipr + 1];
+          wp_scale = frcp((float)(pb - pa));
+          nt0 = ntp[ipr];
+          nt1 = ntp[ipr + 1];
+          tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
+          g00 = (float)tg0[0];
+          gi0 = frcp((float)(tg0[1] - tg0[0]));
+          g10 = (float)tg0[T.TT];
+          gi1 = frcp((float)(tg0[T.TT + 1] - tg0[T.TT]));
+          it0 = it1 = -1;
+        }
+        LkPart d{1, 0, 0, 0};
+        double pn = p, tn = t, un = useg;
+        float en_f = epf;
+        bool loaded = false;
+        if (nt0 >= 2 && nt1 >= 2) {
+          int j0 = (int)((tq - g00) * gi0);
+          j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
+          int j1 = (int)((tq - g10) * gi1);
+          j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
+          const size_t c0 = (size_t)tix * T.TP + ipr;
+          if (j0 != it0) {
+            if (j0 == it0 + 1) {
+              const int a0 = gi[0];
+              gi[0] = gi[1];
+              gi[1] = 2 * gi[1] - a0;
+            } else if (j0 == it0 - 1) {
+              const int a1 = gi[1];
+              gi[1] = gi[0];
+              gi[0] = 2 * gi[0] - a1;
+            }
+            it0 = j0;
+            ta0 = (float)tg0[it0];
+            dt0 = frcp((float)(tg0[it0 + 1] - tg0[it0]));
+            n00 = T.nu[c0 * T.TT + it0];
+            n01 = T.nu[c0 * T.TT + it0 + 1];
+          }
+          if (j1 != it1) {
+            if (j1 == it1 + 1) {
+              const int a2 = gi[2];
+              gi[2] = gi[3];
+              gi[3] = 2 * gi[3] - a2;
+            } else if (j1 == it1 - 1) {
+              const int a3 = gi[3];
+              gi[3] = gi[2];
+              gi[2] = 2 * gi[2] - a3;
+            }
+            it1 = j1;
+            ta1 = (float)tg0[T.TT + it1];
+            dt1 = frcp((float)(tg0[T.TT + it1 + 1] - tg0[T.TT + it1]));
+            n10 = T.nu[(c0 + 1) * T.TT + it1];
+            n11 = T.nu[(c0 + 1) * T.TT + it1 + 1];
+          }
+          if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
+            const float4 *kc[4];
+            kc[0] = T.rec + (c0 * T.TT + it0) * T.TU;
+            kc[1] = kc[0] + T.TU;
+            kc[2] = T.rec + ((c0 + 1) * T.TT + it1) * T.TU;
+            kc[3] = kc[2] + T.TU;
+            const int nc[4] = {n00, n01, n10, n11};
+            float4 r[4], rn[4];
+            int gs[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              int g0 = gi[c] - 1;
+              g0 = g0 < 0 ? 0 : (g0 > nc[c] - 2 ? nc[c] - 2 : g0);
+              gs[c] = g0;
+              r[c] = kc[c][g0];
+              rn[c] = kc[c][g0 + 1 < nc[c] - 2 ? g0 + 1 : nc[c] - 2];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            pn = LOSR(F_P, ipn);
+            tn = LOSR(F_T, ipn);
+            un = LOSR(fu, ipn);
+            en_f = ipn > 0 ? (float)EPS_AT(g, ipn - 1) : 0.f;
+            loaded = true;
+            __builtin_amdgcn_sched_barrier(0);
+            const float us = (float)useg;
+            float ec[4], dep[4], deu[4];
+            bool slow[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              int gg = gs[c];
+              ega_step_fast2_d(r[c], rn[c], nc[c], epf, us, gg, ec[c], dep[c], deu[c], slow[c]);
+              gi[c] = slow[c] ? gi[c] : gg;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              if (slow[c]) ec[c] = ega_corner_f32_d(kc[c], nc[c], epf, useg, gi[c], dep[c], deu[c]);
+            }
+            const float w0 = (tq - ta0) * dt0, w1 = (tq - ta1) * dt1;
+            const float dwp = wp_scale;
+            const float wp = (float)(p - pa) * dwp;
+            const float q0 = 1.f - wp, q1 = wp;
+            const float cw[4] = {q0 * (1.f - w0), q0 * w0, q1 * (1.f - w1), q1 * w1};
+            const float tw[4] = {-q0 * dt0, q0 * dt0, -q1 * dt1, q1 * dt1};
+            const float pw[4] = {-(1.f - w0) * dwp, -w0 * dwp, (1.f - w1) * dwp, w1 * dwp};
+            float sdep = 0.f, sdu = 0.f, sdt = 0.f, sdp = 0.f;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              sdep += cw[c] * dep[c];
+              sdu += cw[c] * deu[c];
+              sdt += tw[c] * ec[c];
+              sdp += pw[c] * ec[c];
+            }
+            const float a = ec[0] + (ec[1] - ec[0]) * w0;
+            const float b = ec[2] + (ec[3] - ec[2]) * w1;
+            const float en = a + (b - a) * wp;
+            if (en < 0.f || en > 1.f) {
+              d = LkPart{0, 0, 0, 0};  // clamped: constant
+            } else {
+              d = LkPart{sdep, sdp, sdt, sdu};
+            }
+          }
+        }
+        if (!loaded) {
+          pn = LOSR(F_P, ipn);
+          tn = LOSR(F_T, ipn);
+          un = LOSR(fu, ipn);
+          en_f = ipn > 0 ? (float)EPS_AT(g, ipn - 1) : 0.f;
+        }
+        const double H = SCR(0, ip);
+        const double z = LOSR(F_Z, ip), ds = LOSR(F_DS, ip);
+        const double trp = 1.0 - (double)epf;
+        const double lam = (Hn - H) / trc + carry;
+        const double u_bar = lam * d.du, p_bar = lam * d.dp, t_bar = lam * d.dt;
+        carry = lam * d.dep;
+        const double ucoef = 10 * p / (JUR_KB * t) * ds;
+        double v[3];
+        v[0] = t_bar - u_bar * useg / t;
+        v[1] = p_bar + u_bar * useg / p;
+        v[2] = u_bar * ucoef;
+        W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+        Hn = H;
+        trc = ip > 0 ? trp : 1.0;
+        p = pn;
+        t = tn;
+        useg = un;
+        epf = en_f;
+      }
+      W.flush(xm, qi, kt);
+      return;
+    }
+  } else {
   for (int ip = nef - 1; ip >= 0; ip--) {
     const double H = SCR(0, ip);
     double epp = 0, trp = 1;
@@ -825,6 +1042,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
     trc = trp;
   }
   W.flush(xm, qi, kt);
+  }
 #undef EPS_AT
 #undef SCR
 }
@@ -875,9 +1093,15 @@ static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L,
   const int nb8 = padded_blocks(nblk);
   hipLaunchKernelGGL((k_adj_acc<SF>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0, nblk, J,
                      rad, tau);
-  if (M.ng > 0)
-    hipLaunchKernelGGL((k_adj_chain<CGA, SF>), dim3(nb8 * M.nd * M.ng), dim3(WAVE), 0, s, M, B, L, E, info,
-                       blk0, nblk, J);
+  if (M.ng == 0) return;
+  const dim3 grid(nb8 * M.nd * M.ng), blk(WAVE);
+  if constexpr (!CGA && sizeof(SF) == sizeof(float)) {
+    if (M.tab.all_mono) {
+      hipLaunchKernelGGL((k_adj_chain<false, float, true>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_adj_chain<CGA, SF, false>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
 }
 
 void launch_adjoint(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,

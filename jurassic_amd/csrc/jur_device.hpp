@@ -337,6 +337,52 @@ __device__ __forceinline__ void ega_step_fast2(const float4 &r0, const float4 &r
   const float e_x = rf.y + fdiv(rf.w - rf.y, rf.z - rf.x) * (float)(arg - (double)rf.x);
   ok = ok && (!cross || inb || nxt);
   ec = cross ? e_x : e_in;
+  // above the last knot (paths that have grown past a column's last
+  // emissivity, e.g. a limb path climbing into thin layers)
+  const bool above = (stay || up) && b_top && ep > rb.w;
+  if (above) ec = 1.f - (1.f - ep) * __expf(fdiv(__logf(1.f - rb.w), rb.z) * us);
+  ok = ok || above;
+  slow = !ok;
+  g = ok ? b : g;
+}
+
+// ega_step_fast2 with the partials of the step: dep = d eps / d ep and
+// deu = d eps / d u_seg, the slopes of the brackets used (as the FP32
+// lookup evaluates them: reciprocal-multiply float slopes).
+__device__ __forceinline__ void ega_step_fast2_d(const float4 &r0, const float4 &r1, int n, float ep, float us,
+                                                 int &g, float &ec, float &dep, float &deu, bool &slow) {
+  const bool top = g >= n - 2;
+  const bool stay = (g == 0 || r0.y <= ep) && (top || r0.w > ep);
+  const bool up = !top && r0.w <= ep && (g + 1 >= n - 2 || r1.w > ep);
+  const float4 rb = stay ? r0 : r1;
+  const int b = stay ? g : g + 1;
+  bool ok = (stay || up) && !(ep < rb.y) && !(ep > rb.w);
+  const float du = rb.z - rb.x, de = rb.w - rb.y;
+  const float su = fdiv(du, de);
+  const float off = (ep - rb.y) * su + us;
+  const float se_in = fdiv(de, du);
+  const float e_in = rb.y + off * se_in;
+  const double arg = (double)rb.x + (double)off;
+  const bool cross = !(off < du);
+  const bool b_top = b >= n - 2;
+  const bool inb = b_top ? !(arg > (double)rb.z) : !((double)rb.z <= arg);
+  const bool nxt = stay && !top && (double)r0.z <= arg && (g + 1 >= n - 2 ? !(arg > (double)r1.z) : (double)r1.z > arg);
+  const float4 rf = inb ? rb : r1;
+  const float sf = fdiv(rf.w - rf.y, rf.z - rf.x);
+  const float e_x = rf.y + sf * (float)(arg - (double)rf.x);
+  ok = ok && (!cross || inb || nxt);
+  ec = cross ? e_x : e_in;
+  deu = cross ? sf : se_in;
+  dep = deu * su;
+  const bool above = (stay || up) && b_top && ep > rb.w;
+  if (above) {
+    const float a = fdiv(__logf(1.f - rb.w), rb.z);
+    const float ex = __expf(a * us);
+    ec = 1.f - (1.f - ep) * ex;
+    deu = ex > 0.f ? -a * (1.f - ep) * ex : 0.f;
+    dep = ex;
+  }
+  ok = ok || above;
   slow = !ok;
   g = ok ? b : g;
 }

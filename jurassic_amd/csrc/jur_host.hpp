@@ -118,7 +118,7 @@ struct jur_ctx {
   jur::DevBuf<double> d_p, d_t;
   jur::DevBuf<float4> d_rec;
   int TP = 2, TT = 2, TU = 2;
-  int shared = 0, ref_tix = 0;
+  int shared = 0, ref_tix = 0, all_mono = 1;
   size_t table_bytes = 0;
   // model constants
   jur::DevBuf<double> d_st, d_sr, d_clnu, d_sfnu, d_ctm;

@@ -43,6 +43,7 @@ struct TabView {
                            // (search by the reference bisection)
   const int *pdir;         // [ng][nd]: +1/-1 strictly ascending/descending p grid,
                            // 0 otherwise (search by bisection)
+  int all_mono;            // 1 if mono is 1 for every table with np >= 30
   int shared;              // 1 if every table with np >= 30 has the same p grid,
                            // T-grid sizes and T grids (one cell search per
                            // segment serves all of them)

@@ -287,6 +287,7 @@ static int build_tables(jur_ctx *c) {
       h_mono(ntab + 1, 1), h_pdir(ntab + 1, 0);
   std::vector<double> h_p(ntab * TP + 1, 0.0), h_t(ntab * TP * TT + 1, 0.0);
   std::vector<float4> h_rec(ntab * TP * TT * TU + 1, make_float4(0, 0, 0, 0));
+  c->all_mono = 1;
   for (size_t k = 0; k < ntab; k++) {
     const HostTable &tb = tabs[k];
     h_np[k] = tb.np;
@@ -309,6 +310,7 @@ static int build_tables(jur_ctx *c) {
       }
     }
     h_mono[k] = mono;
+    if (tb.np >= 30 && !mono) c->all_mono = 0;
     int asc = 1, desc = 1;
     for (int ip = 0; ip + 1 < tb.np; ip++) {
       if (!(tb.p[ip + 1] > tb.p[ip])) asc = 0;
@@ -482,6 +484,7 @@ static ModelView model_view(const jur_ctx *c) {
   T.pdir = c->d_pdir.p;
   T.shared = c->shared;
   T.ref_tix = c->ref_tix;
+  T.all_mono = c->all_mono;
   return M;
 }
 

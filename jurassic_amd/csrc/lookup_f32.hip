@@ -91,7 +91,23 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         keep = (ipr == 0 || !(pa <= pq)) && (ipr == npt - 2 || pb <= pq);
     }
     if (!keep) {
+      const int iprev = ipr;
       ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, pq, ipr, pdir) : locate_irr(pg, npt, pq);
+      // start indices for the new corner columns: a column the cell keeps
+      // takes its bracket along, a new one extrapolates from the pair
+      if (ipr == iprev + 1) {
+        const int g0 = gi[0], g1 = gi[1];
+        gi[0] = gi[2];
+        gi[1] = gi[3];
+        gi[2] = 2 * gi[2] - g0;
+        gi[3] = 2 * gi[3] - g1;
+      } else if (ipr == iprev - 1) {
+        const int g2 = gi[2], g3 = gi[3];
+        gi[2] = gi[0];
+        gi[3] = gi[1];
+        gi[0] = 2 * gi[0] - g2;
+        gi[1] = 2 * gi[1] - g3;
+      }
       pa = pg[ipr];
       pb = pg[ipr + 1];
       wp_scale = frcp((float)(pb - pa));
@@ -113,6 +129,15 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
       const size_t c0 = (size_t)tix * T.TP + ipr;
       if (j0 != it0) {
+        if (j0 == it0 + 1) {
+          const int g0 = gi[0];
+          gi[0] = gi[1];
+          gi[1] = 2 * gi[1] - g0;
+        } else if (j0 == it0 - 1) {
+          const int g1 = gi[1];
+          gi[1] = gi[0];
+          gi[0] = 2 * gi[0] - g1;
+        }
         it0 = j0;
         ta0 = (float)tg0[it0];
         dt0 = frcp((float)(tg0[it0 + 1] - tg0[it0]));
@@ -120,6 +145,15 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         n01 = T.nu[c0 * T.TT + it0 + 1];
       }
       if (j1 != it1) {
+        if (j1 == it1 + 1) {
+          const int g2 = gi[2];
+          gi[2] = gi[3];
+          gi[3] = 2 * gi[3] - g2;
+        } else if (j1 == it1 - 1) {
+          const int g3 = gi[3];
+          gi[3] = gi[2];
+          gi[2] = 2 * gi[2] - g3;
+        }
         it1 = j1;
         ta1 = (float)tg0[T.TT + it1];
         dt1 = frcp((float)(tg0[T.TT + it1 + 1] - tg0[T.TT + it1]));
