@@ -265,6 +265,151 @@ def jacobian_bench(args, torch, gpu, abi):
     return out
 
 
+NWIDE_CH = 2048
+WIDE_GASES = ["CO2", "H2O", "O3", "CH4", "N2O", "CO", "HNO3", "F11"]
+
+
+def wide_tables(torch, nu, gases):
+    """Synthetic tables of the wide workload, built on the device in the
+    reference's tbl_t order: per (channel, emitter) 40 p x 20 T x 200 u
+    (table_grids), eps = 1 - exp(-k (u/1e20)^0.8) stored as float,
+    k = s f(nu) (p/1013)^0.7 (T/250)^-1.5 with s = 1e-2 for every emitter and
+    a channel factor f(nu) = 0.2 + 0.8 (0.5 + 0.5 sin(2 pi nu / 97)), so that
+    channels hold different tables."""
+    import numpy as np
+    from jurassic_amd import scenes
+    p, t, u = scenes.table_grids()
+    nd, ng, tp, tt, tu = len(nu), len(gases), len(p), len(t), len(u)
+    dev = torch.device("cuda:0")
+    i32 = torch.int32
+    tabs = dict(np=torch.full((nd, ng), tp, dtype=i32, device=dev),
+                nt=torch.full((nd, ng, tp), tt, dtype=i32, device=dev),
+                nu=torch.full((nd, ng, tp, tt), tu, dtype=i32, device=dev),
+                p=torch.tensor(p, device=dev).expand(nd, ng, tp).contiguous(),
+                t=torch.tensor(t, device=dev).expand(nd, ng, tp, tt).contiguous(),
+                u=torch.tensor(u.astype(np.float32), device=dev).expand(nd, ng, tp, tt, tu).contiguous())
+    eps = torch.empty((nd, ng, tp, tt, tu), dtype=torch.float32, device=dev)
+    pk = (torch.tensor(p, device=dev) / 1013.0) ** 0.7
+    tk = (torch.tensor(t, device=dev) / 250.0) ** -1.5
+    xu = (torch.tensor(u, device=dev) / 1e20) ** 0.8
+    base = 1e-2 * pk[:, None] * tk[None, :]
+    nut = torch.tensor(np.asarray(nu), device=dev)
+    f = 0.2 + 0.8 * (0.5 + 0.5 * torch.sin(2 * math.pi * nut / 97.0))
+    for c0 in range(0, nd, 64):
+        c1 = min(nd, c0 + 64)
+        x = f[c0:c1, None, None, None] * base[None, :, :, None] * xu[None, None, None, :]
+        eps[c0:c1] = (-torch.expm1(-x)).float()[:, None].expand(c1 - c0, ng, tp, tt, tu)
+    tabs["eps"] = eps
+    return tabs
+
+
+def boxcar_filters(nu, width=1.0, step=0.0005):
+    """In-memory copies of the boxcar filter files (write_boxcar_filter)."""
+    import numpy as np
+    half = int(round(width / 2 / step))
+    n = 2 * half + 5
+    i = np.arange(n)
+    w = ((i >= 2) & (i < n - 3)).astype(float)
+    xs = [np.round(v + (i - half - 2) * step, 4) for v in nu]
+    off = np.arange(len(nu) + 1, dtype=np.int32) * n
+    return off, np.concatenate(xs), np.tile(w, len(nu))
+
+
+def wide_bench(args, torch, gpu, abi):
+    """2048 channels x 8 emitters (CO2, H2O, O3, CH4, N2O, CO, HNO3, F11),
+    nadir view angles (i mod 49) deg, continua and refraction on: forward
+    model by EGA and by CGA and adjoint Jacobians for T, H2O and O3
+    (n = 243), FP32 lookups, tables ingested from device memory."""
+    import numpy as np
+    from jurassic_amd import scenes
+    nprof = args.wide_nprof
+    nu = [650.0 + i * 2050.0 / 2047.0 for i in range(NWIDE_CH)]
+    t0 = time.perf_counter()
+    tabs = wide_tables(torch, nu, WIDE_GASES)
+    filt = boxcar_filters(nu)
+    torch.cuda.synchronize()
+    t_tab = time.perf_counter() - t0
+    # rays: profile i looks down at (i mod 49) deg; ray order angle-major
+    ang = np.arange(nprof) % 49
+    order = np.argsort(ang, kind="stable").astype(np.int32)
+    geom = np.zeros((nprof, 7))
+    g49 = scenes.nadir_geometry(705.0, np.arange(49, dtype=float))
+    for c, key in enumerate(("obsz", "obslon", "obslat", "vpz", "vplon", "vplat")):
+        geom[:, 1 + c] = np.asarray(g49[key])[ang[order]]
+    atms = (abi.atm_t * nprof)()
+    for k in range(nprof):
+        atms[k] = scenes.make_profile_atm(WIDE_GASES, iprof=k)
+    dev = torch.device("cuda:0")
+    atm_d = torch.frombuffer(bytearray(bytes(atms)), dtype=torch.uint8).to(dev)
+    prof_d = torch.from_numpy(order).to(dev)
+    geom_d = torch.from_numpy(geom).to(dev)
+    nray, nd = nprof, NWIDE_CH
+    rad_d = torch.empty(nray * nd, dtype=torch.float64, device=dev)
+    tau_d = torch.empty(nray * nd, dtype=torch.float64, device=dev)
+    res = {"workload": "%d atmospheres x 1 nadir view (angle i mod 49 deg) x %d channels x 8 emitters "
+                       "(%s); continua and refraction on; tables 40 p x 20 T x 200 u per (channel, emitter)"
+                       % (nprof, nd, ", ".join(WIDE_GASES)),
+           "table_build_s": t_tab}
+    for formod, name in ((1, "ega"), (0, "cga")):
+        ctl = abi.make_ctl(WIDE_GASES, nu[:1], tblbase="-", tblfmt=2)
+        ctl.formod = formod
+        t0 = time.perf_counter()
+        ctx = gpu.Context.from_device_tables(ctl, nu, tabs, filters=filt)
+        torch.cuda.synchronize()
+        t_ctx = time.perf_counter() - t0
+        log("wide %s context: %.1f s (tables packed in HBM, source function for %d channels)"
+            % (name, t_ctx, nd))
+
+        def fwd():
+            ctx.formod_device(nprof, atm_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
+                              rad_d.data_ptr(), tau_d.data_ptr(), precision=abi.JUR_FP32)
+        fwd()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fwd()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kms = ctx.last_timing()
+        r = dict(radiances_per_s=nray * nd / dt, ms_per_pass=dt * 1e3, context_s=t_ctx,
+                 kernel_ms={"raytrace": kms[0], "lookup": kms[1], "accumulate": kms[2]},
+                 mean_los_points=ctx.last_mean_np())
+        rad = rad_d.view(nray, nd)
+        r["finite"] = bool(torch.isfinite(rad).all().item())
+        log("wide %s forward FP32: %.1f ms per %d radiances (%.3g radiances/s); kernels %s"
+            % (name, dt * 1e3, nray * nd, r["radiances_per_s"], r["kernel_ms"]))
+        if formod == 1:
+            ctl.rett_zmin, ctl.rett_zmax = -100, 100
+            for ig in (1, 2):
+                ctl.retq_zmin[ig], ctl.retq_zmax[ig] = -100, 100
+            xm, n = gpu.state_map(ctl, atms[0])
+            pmap_d = torch.zeros(nprof, dtype=torch.int32, device=dev)
+            k_d = torch.empty(nray * nd * n, dtype=torch.float64, device=dev)
+
+            def adj():
+                ctx.jacobian_device(nprof, atm_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
+                                    rad_d.data_ptr(), tau_d.data_ptr(), xm[None, :], pmap_d.data_ptr(), n,
+                                    k_d.data_ptr(), precision=abi.JUR_FP32)
+            adj()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            adj()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            kms = ctx.last_timing()
+            r["jacobian"] = dict(jacobians_per_s=nprof / dt, ms_per_pass=dt * 1e3, n=n, m=nd,
+                                 kernel_ms={"raytrace": kms[0], "lookup": kms[1],
+                                            "reverse_sweep_and_store": kms[2]},
+                                 finite=bool(torch.isfinite(k_d).all().item()))
+            log("wide adjoint FP32 (T, H2O, O3; n = %d): %.1f ms per %d Jacobians (%.1f Jacobians/s)"
+                % (n, dt * 1e3, nprof, nprof / dt))
+            del k_d
+        ctx.close()
+        res[name] = r
+    del tabs
+    torch.cuda.empty_cache()
+    return res
+
+
 def serving_bench(args, gpu, abi):
     """Poisson stream of mixed requests into the request server."""
     import threading
@@ -433,6 +578,9 @@ def main():
     ap.add_argument("--jac-check", type=int, default=64,
                     help="atmospheres per angle for the FP32-vs-FP64 adjoint check")
     ap.add_argument("--jac-cpu-sample", type=int, default=2)
+    ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--wide-only", action="store_true")
+    ap.add_argument("--wide-nprof", type=int, default=4096)
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -458,6 +606,9 @@ def main():
         return
     if args.jac_only:
         print(json.dumps({"jacobian": jacobian_bench(args, torch, gpu, abi)}))
+        return
+    if args.wide_only:
+        print(json.dumps({"wide": wide_bench(args, torch, gpu, abi)}))
         return
     ctl, atms, prof, geom = build_workload(args.nprof)
     nprof, nray, nd = args.nprof, len(prof), ctl.nd
@@ -558,6 +709,13 @@ def main():
         out["jacobian"] = jacobian_bench(args, torch, gpu, abi)
     if not args.no_serve:
         out["serving"] = serving_bench(args, gpu, abi)
+    if not args.no_wide:
+        try:
+            torch.cuda.empty_cache()
+            out["wide"] = wide_bench(args, torch, gpu, abi)
+        except Exception as e:  # the wide section reports its failure instead of the line's
+            out["wide"] = {"error": "%s: %s" % (type(e).__name__, e)}
+            log("wide section failed: %s" % e)
     print(json.dumps(out))
 
 

@@ -360,6 +360,33 @@ int jur_ctx_create(const ctl_t *ctl, jur_ctx_t **ctx);
 int jur_ctx_create_wide(const ctl_t *ctl, int nd, const double *nu,
                         const int *window, jur_ctx_t **ctx);
 
+/* Emissivity tables already in device memory: the arrays of the
+   reference's tbl_t (src/jurassic.h:778-807), one table per (channel,
+   emitter), channel-major as there, with capacities tp, tt, tu:
+     np[id][ig]; p[id][ig][tp]; nt[id][ig][tp]; t[id][ig][tp][tt];
+     nu[id][ig][tp][tt]; u, eps[id][ig][tp][tt][tu]
+   np < 0 marks a missing table (transparent, as read_tbl's warning,
+   src/jurassic.c:4980-4983). Values are taken as they are, as read_tbl
+   takes a binary table (src/jurassic.c:5049-5083).
+   Filter functions: filt_off == NULL reads <tblbase>_<nu>.filt as
+   read_shape does; otherwise channel id has the filt_off[id+1] -
+   filt_off[id] points filt_nu[filt_off[id] ...], filt_w[...] (host). */
+typedef struct {
+  int tp, tt, tu;
+  const int *np, *nt, *nu;
+  const double *p, *t;
+  const float *u, *eps;
+  const int *filt_off;
+  const double *filt_nu, *filt_w;
+} jur_dtables_t;
+
+/* As jur_ctx_create_wide, with the tables taken from device memory instead
+   of read_tbl's files (src/jurassic.c:4949-5110). The arrays are read during
+   the call only. */
+int jur_ctx_create_device_tables(const ctl_t *ctl, int nd, const double *nu,
+                                 const int *window, const jur_dtables_t *tab,
+                                 jur_ctx_t **ctx);
+
 void jur_ctx_destroy(jur_ctx_t *ctx);
 
 /* Number of channels of the context. */

@@ -14,6 +14,7 @@ import numpy as np
 
 NCL, ND, NG, NP, NR, NSF, NW, LEN = 8, 128, 8, 256, 256, 8, 4, 10000
 NLOS, TBLNS = 4096, 1200
+TBLNP, TBLNT, TBLNU = 41, 30, 320
 
 JUR_OK = 0
 JUR_FP64 = 0

@@ -92,6 +92,52 @@ __global__ void k_zero(unsigned long long *p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
 }
 
+// Bracket records of tables given in the reference's tbl_t order (one
+// table per [channel][emitter], capacities tp, tt, tu) in the library's order
+// (table k = ig * nd + id, capacities TP, TT, TU, counts np/nt/nu already in
+// that order): rec[i] = (u_i, eps_i, u_{i+1}, eps_{i+1}), the last record of a
+// column repeating its knot, zeros past the column.  A table with a column
+// that decreases in u or eps gets mono[k] = 0 (every writer stores 0).
+__global__ void k_pack_tables(int nd, int ng, int TP, int TT, int TU, const int *np, const int *nt,
+                              const int *nu, int tp, int tt, int tu, const float *u_in, const float *eps_in,
+                              float4 *rec, int *mono) {
+  const size_t total = (size_t)nd * ng * TP * TT * TU;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += stride) {
+    size_t q = r;
+    const int i = (int)(q % TU);
+    q /= TU;
+    const int it = (int)(q % TT);
+    q /= TT;
+    const int ip = (int)(q % TP);
+    const int k = (int)(q / TP);
+    const int ig = k / nd, id = k - ig * nd;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ip < np[k] && it < nt[(size_t)k * TP + ip]) {
+      const int n = nu[((size_t)k * TP + ip) * TT + it];
+      if (i < n) {
+        const size_t src = ((((size_t)id * ng + ig) * tp + ip) * tt + it) * tu;
+        const int j = i + 1 < n ? i + 1 : i;
+        const float u0 = u_in[src + i], e0 = eps_in[src + i], u1 = u_in[src + j], e1 = eps_in[src + j];
+        v = make_float4(u0, e0, u1, e1);
+        if (i + 1 < n && !(u1 >= u0 && e1 >= e0)) mono[k] = 0;
+      }
+    }
+    rec[r] = v;
+  }
+}
+
+void launch_pack_tables(int nd, int ng, int TP, int TT, int TU, const int *np, const int *nt, const int *nu,
+                        int tp, int tt, int tu, const float *u_in, const float *eps_in, float4 *rec, int *mono,
+                        hipStream_t s) {
+  const size_t total = (size_t)nd * ng * TP * TT * TU;
+  if (total == 0) return;
+  const size_t want = (total + 255) / 256;
+  const unsigned nblk = (unsigned)(want < 65536 ? want : 65536);
+  hipLaunchKernelGGL(k_pack_tables, dim3(nblk), dim3(256), 0, s, nd, ng, TP, TT, TU, np, nt, nu, tp, tt, tu,
+                     u_in, eps_in, rec, mono);
+}
+
 void launch_zero(void *p, size_t n, hipStream_t s) {
   if (n == 0) return;
   const size_t nb = (n + 255) / 256;

@@ -55,6 +55,8 @@ void launch_finalize(double *, int, int, const ChanInfo *, int, const unsigned c
                      hipStream_t);
 void launch_block_np(const RayInfo *, int, int, int, int *, hipStream_t);
 void launch_zero(void *, size_t, hipStream_t);
+void launch_pack_tables(int, int, int, int, int, const int *, const int *, const int *, int, int, int,
+                        const float *, const float *, float4 *, int *, hipStream_t);
 
 // ------------------------------------------------------------------
 // Errors.
@@ -263,6 +265,44 @@ static int locate_irr_h(const double *xx, int n, double x) {
 
 namespace jur {
 
+// Shared grids: every table with np >= 30 has the same p grid, T-row sizes
+// and T grids as the first such table (one cell search per segment then
+// serves all of them).
+static void detect_shared(jur_ctx *c, const std::vector<int> &h_np, const std::vector<int> &h_nt,
+                          const std::vector<double> &h_p, const std::vector<double> &h_t) {
+  const size_t ntab = (size_t)c->s.nd * c->s.ng;
+  const int TP = c->TP, TT = c->TT;
+  c->shared = 0;
+  c->ref_tix = 0;
+  int ref = -1;
+  for (size_t k = 0; k < ntab; k++)
+    if (h_np[k] >= 30) { ref = (int)k; break; }
+  if (ref < 0) return;
+  bool same = true;
+  for (size_t k = 0; k < ntab && same; k++) {
+    if (h_np[k] < 30) continue;
+    if (h_np[k] != h_np[ref]) { same = false; break; }
+    for (int ip = 0; ip < TP && same; ip++) {
+      if (h_p[k * TP + ip] != h_p[ref * TP + ip] || h_nt[k * TP + ip] != h_nt[ref * TP + ip]) same = false;
+      for (int it = 0; it < TT && same; it++)
+        if (h_t[(k * TP + ip) * TT + it] != h_t[(ref * TP + ip) * TT + it]) same = false;
+    }
+  }
+  c->shared = same ? 1 : 0;
+  c->ref_tix = ref;
+}
+
+// Direction of a table's pressure grid: +1 strictly ascending, -1 strictly
+// descending, 0 otherwise.
+static int p_direction(const double *p, int np) {
+  int asc = 1, desc = 1;
+  for (int ip = 0; ip + 1 < np; ip++) {
+    if (!(p[ip + 1] > p[ip])) asc = 0;
+    if (!(p[ip + 1] < p[ip])) desc = 0;
+  }
+  return (np >= 2 && asc) ? 1 : ((np >= 2 && desc) ? -1 : 0);
+}
+
 static int build_tables(jur_ctx *c) {
   const Setup &s = c->s;
   const int nd = s.nd, ng = s.ng;
@@ -311,12 +351,7 @@ static int build_tables(jur_ctx *c) {
     }
     h_mono[k] = mono;
     if (tb.np >= 30 && !mono) c->all_mono = 0;
-    int asc = 1, desc = 1;
-    for (int ip = 0; ip + 1 < tb.np; ip++) {
-      if (!(tb.p[ip + 1] > tb.p[ip])) asc = 0;
-      if (!(tb.p[ip + 1] < tb.p[ip])) desc = 0;
-    }
-    h_pdir[k] = (tb.np >= 2 && asc) ? 1 : ((tb.np >= 2 && desc) ? -1 : 0);
+    h_pdir[k] = p_direction(tb.p.data(), tb.np);
   }
   int rc;
   if ((rc = c->d_np.upload(h_np.data(), h_np.size()))) return rc;
@@ -328,33 +363,99 @@ static int build_tables(jur_ctx *c) {
   if ((rc = c->d_t.upload(h_t.data(), h_t.size()))) return rc;
   if ((rc = c->d_rec.upload(h_rec.data(), h_rec.size()))) return rc;
   c->table_bytes = h_rec.size() * sizeof(float4);
-  // shared grids: every table with np >= 30 has the same p grid, T-row sizes
-  // and T grids as the first such table
-  c->shared = 0;
-  c->ref_tix = 0;
-  int ref = -1;
-  for (size_t k = 0; k < ntab; k++)
-    if (h_np[k] >= 30) { ref = (int)k; break; }
-  if (ref >= 0) {
-    bool same = true;
-    for (size_t k = 0; k < ntab && same; k++) {
-      if (h_np[k] < 30) continue;
-      if (h_np[k] != h_np[ref]) { same = false; break; }
-      for (int ip = 0; ip < TP && same; ip++) {
-        if (h_p[k * TP + ip] != h_p[ref * TP + ip] || h_nt[k * TP + ip] != h_nt[ref * TP + ip]) same = false;
-        for (int it = 0; it < TT && same; it++)
-          if (h_t[(k * TP + ip) * TT + it] != h_t[(ref * TP + ip) * TT + it]) same = false;
-      }
-    }
-    c->shared = same ? 1 : 0;
-    c->ref_tix = ref;
-  }
+  detect_shared(c, h_np, h_nt, h_p, h_t);
   return JUR_OK;
 }
 
-static int build_srcfunc(jur_ctx *c) {
+// Tables from device memory in the reference's tbl_t order ([channel]
+// [emitter]...): the counts and grids come to the host (the checks above
+// need them), the bracket records are packed on the device.
+static int build_tables_device(jur_ctx *c, const jur_dtables_t *d) {
+  const Setup &s = c->s;
+  const int nd = s.nd, ng = s.ng;
+  const size_t ntab = (size_t)nd * ng;
+  const int tp = d->tp, tt = d->tt, tu = d->tu;
+  if (ntab > 0 && (tp < 1 || tt < 1 || tu < 1 || !d->np || !d->nt || !d->nu || !d->p || !d->t || !d->u ||
+                   !d->eps))
+    return set_err(JUR_ERR_ARG, "invalid device tables");
+  std::vector<int> i_np(ntab + 1), i_nt(ntab * tp + 1), i_nu(ntab * tp * tt + 1);
+  std::vector<double> i_p(ntab * tp + 1), i_t(ntab * tp * tt + 1);
+  if (ntab > 0) {
+    HIPCHK(hipMemcpy(i_np.data(), d->np, ntab * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(i_nt.data(), d->nt, ntab * tp * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(i_nu.data(), d->nu, ntab * tp * tt * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(i_p.data(), d->p, ntab * tp * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(i_t.data(), d->t, ntab * tp * tt * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  // sizes, checked against the capacities as read_tbl checks them
+  int TP = 2, TT = 2, TU = 2;
+  for (size_t q = 0; q < ntab; q++) {
+    const int np = i_np[q];
+    if (np > tp) return set_err(JUR_ERR_ARG, "Too many pressure levels!");
+    for (int ip = 0; ip < np; ip++) {
+      const int nt = i_nt[q * tp + ip];
+      if (nt > tt || nt < 0) return set_err(JUR_ERR_ARG, "Too many temperatures!");
+      for (int it = 0; it < nt; it++) {
+        const int n = i_nu[(q * tp + ip) * tt + it];
+        if (n > tu || n < 0) return set_err(JUR_ERR_ARG, "Too many column densities!");
+        TU = std::max(TU, n);
+      }
+      TT = std::max(TT, nt);
+    }
+    TP = std::max(TP, np);
+  }
+  c->TP = TP;
+  c->TT = TT;
+  c->TU = TU;
+  // the library's order: table k = ig * nd + id
+  std::vector<int> h_np(ntab + 1, -1), h_nt(ntab * TP + 1, 0), h_nu(ntab * TP * TT + 1, 0),
+      h_mono(ntab + 1, 1), h_pdir(ntab + 1, 0);
+  std::vector<double> h_p(ntab * TP + 1, 0.0), h_t(ntab * TP * TT + 1, 0.0);
+  for (int ig = 0; ig < ng; ig++)
+    for (int id = 0; id < nd; id++) {
+      const size_t k = (size_t)ig * nd + id, q = (size_t)id * ng + ig;
+      const int np = i_np[q];
+      h_np[k] = np;
+      for (int ip = 0; ip < np; ip++) {
+        h_p[k * TP + ip] = i_p[q * tp + ip];
+        const int nt = i_nt[q * tp + ip];
+        h_nt[k * TP + ip] = nt;
+        for (int it = 0; it < nt; it++) {
+          h_t[(k * TP + ip) * TT + it] = i_t[(q * tp + ip) * tt + it];
+          h_nu[(k * TP + ip) * TT + it] = i_nu[(q * tp + ip) * tt + it];
+        }
+      }
+      h_pdir[k] = p_direction(&h_p[k * TP], std::max(np, 0));
+    }
+  int rc;
+  if ((rc = c->d_np.upload(h_np.data(), h_np.size()))) return rc;
+  if ((rc = c->d_nt.upload(h_nt.data(), h_nt.size()))) return rc;
+  if ((rc = c->d_nu.upload(h_nu.data(), h_nu.size()))) return rc;
+  if ((rc = c->d_mono.upload(h_mono.data(), h_mono.size()))) return rc;
+  if ((rc = c->d_pdir.upload(h_pdir.data(), h_pdir.size()))) return rc;
+  if ((rc = c->d_p.upload(h_p.data(), h_p.size()))) return rc;
+  if ((rc = c->d_t.upload(h_t.data(), h_t.size()))) return rc;
+  const size_t nrec = ntab * TP * TT * TU + 1;
+  if ((rc = c->d_rec.reserve(nrec))) return rc;
+  c->table_bytes = nrec * sizeof(float4);
+  launch_pack_tables(nd, ng, TP, TT, TU, c->d_np.p, c->d_nt.p, c->d_nu.p, tp, tt, tu, d->u, d->eps,
+                     c->d_rec.p, c->d_mono.p, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(h_mono.data(), c->d_mono.p, ntab * sizeof(int), hipMemcpyDeviceToHost));
+  c->all_mono = 1;
+  for (size_t k = 0; k < ntab; k++)
+    if (h_np[k] >= 30 && !h_mono[k]) c->all_mono = 0;
+  detect_shared(c, h_np, h_nt, h_p, h_t);
+  return JUR_OK;
+}
+
+static int build_srcfunc(jur_ctx *c, const jur_dtables_t *mem = nullptr) {
   const Setup &s = c->s;
   const int nd = s.nd;
+  const bool in_mem = mem && mem->filt_off;
+  if (in_mem && nd > 0 && (!mem->filt_nu || !mem->filt_w))
+    return set_err(JUR_ERR_ARG, "invalid filter functions");
   std::vector<double> st(TBLNS);
   for (int it = 0; it < TBLNS; it++) {
     // LIN(0.0, TMIN, TBLNS - 1.0, TMAX, (double) it)  (src/jurassic.c:3662)
@@ -363,11 +464,18 @@ static int build_srcfunc(jur_ctx *c) {
   std::vector<double> fnu, f, dnu(nd + 1);
   std::vector<int> off(nd + 1), nf(nd + 1);
   for (int id = 0; id < nd; id++) {
-    char fn[2 * LEN + 32];
-    snprintf(fn, sizeof fn, "%s_%.4f.filt", s.tblbase.c_str(), s.nu[id]);
     std::vector<double> x, y;
-    int rc = read_shape(fn, x, y);
-    if (rc) return rc;
+    if (in_mem) {
+      const int a = mem->filt_off[id], b = mem->filt_off[id + 1];
+      if (b - a < 2 || b - a > NSHAPE) return set_err(JUR_ERR_ARG, "invalid filter function of channel %d", id);
+      x.assign(mem->filt_nu + a, mem->filt_nu + b);
+      y.assign(mem->filt_w + a, mem->filt_w + b);
+    } else {
+      char fn[2 * LEN + 32];
+      snprintf(fn, sizeof fn, "%s_%.4f.filt", s.tblbase.c_str(), s.nu[id]);
+      int rc = read_shape(fn, x, y);
+      if (rc) return rc;
+    }
     double d = 1.0;
     for (size_t i = 1; i < x.size(); i++) d = std::min(d, x[i] - x[i - 1]);
     off[id] = (int)fnu.size();
@@ -1231,6 +1339,20 @@ int jur_ctx_create_wide(const ctl_t *ctl, int nd, const double *nu, const int *w
   int rc = setup_from_ctl(ctl, nd, nu, window, c->s);
   if (rc) return rc;
   if ((rc = ctx_build(c.get()))) return rc;
+  *out = c.release();
+  return JUR_OK;
+}
+
+int jur_ctx_create_device_tables(const ctl_t *ctl, int nd, const double *nu, const int *window,
+                                 const jur_dtables_t *tab, jur_ctx_t **out) {
+  if (!ctl || !out || !tab || (nd > 0 && !nu)) return set_err(JUR_ERR_ARG, "NULL argument");
+  std::unique_ptr<jur_ctx> c(new jur_ctx());
+  int rc = setup_from_ctl(ctl, nd, nu, window, c->s);
+  if (rc) return rc;
+  HIPCHK(hipGetDevice(&c->device));
+  if ((rc = build_tables_device(c.get(), tab))) return rc;
+  if ((rc = build_model(c.get()))) return rc;
+  if ((rc = build_srcfunc(c.get(), tab))) return rc;
   *out = c.release();
   return JUR_OK;
 }

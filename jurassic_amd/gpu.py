@@ -44,7 +44,18 @@ EXPORTS = [
     "jur_ctx_last_timing", "jur_ctx_last_mean_np", "jur_state_map", "jur_ctx_jacobian_device",
     "jur_gpu_optimal_estimation", "jur_server_create", "jur_server_destroy", "jur_server_submit",
     "jur_server_wait", "jur_server_fetch", "jur_server_stats", "jur_server_profile",
+    "jur_ctx_create_device_tables",
 ]
+
+
+class jur_dtables_t(C.Structure):
+    """Emissivity tables in device memory, tbl_t order (jurassic_gpu.h)."""
+    _fields_ = [
+        ("tp", C.c_int), ("tt", C.c_int), ("tu", C.c_int),
+        ("np", C.c_void_p), ("nt", C.c_void_p), ("nu", C.c_void_p),
+        ("p", C.c_void_p), ("t", C.c_void_p), ("u", C.c_void_p), ("eps", C.c_void_p),
+        ("filt_off", C.c_void_p), ("filt_nu", C.c_void_p), ("filt_w", C.c_void_p),
+    ]
 
 
 class jur_jac_t(C.Structure):
@@ -74,6 +85,8 @@ def lib():
         L.jur_ctx_create.restype = I
         L.jur_ctx_create_wide.argtypes = [P, I, C.POINTER(D), C.POINTER(I), C.POINTER(P)]
         L.jur_ctx_create_wide.restype = I
+        L.jur_ctx_create_device_tables.argtypes = [P, I, C.POINTER(D), C.POINTER(I), P, C.POINTER(P)]
+        L.jur_ctx_create_device_tables.restype = I
         L.jur_ctx_destroy.argtypes = [P]
         L.jur_ctx_nd.argtypes = [P]
         L.jur_ctx_nd.restype = I
@@ -186,6 +199,38 @@ class Context:
                 C.byref(ctl), len(nu), nu.ctypes.data_as(C.POINTER(C.c_double)),
                 None if w is None else w.ctypes.data_as(C.POINTER(C.c_int)), C.byref(self.ptr)))
         self.nd = lib().jur_ctx_nd(self.ptr)
+
+    @classmethod
+    def from_device_tables(cls, ctl, nu, tables, window=None, filters=None):
+        """Context whose tables come from device memory.  tables: dict of
+        device tensors (torch) np [nd][ng] int32, nt [nd][ng][tp] int32,
+        nu [nd][ng][tp][tt] int32, p [nd][ng][tp] float64, t [nd][ng][tp][tt]
+        float64, u and eps [nd][ng][tp][tt][tu] float32.  filters: None
+        (files) or (off [nd+1] int32, nu, w) host arrays."""
+        self = cls.__new__(cls)
+        self.ptr = C.c_void_p()
+        nu = np.ascontiguousarray(nu, dtype=np.float64)
+        w = None if window is None else np.ascontiguousarray(window, dtype=np.int32)
+        u = tables["u"]
+        tp, tt, tu = u.shape[2], u.shape[3], u.shape[4]
+        for k in ("np", "nt", "nu", "p", "t", "u", "eps"):
+            assert tables[k].is_contiguous(), k
+        d = jur_dtables_t(tp, tt, tu, tables["np"].data_ptr(), tables["nt"].data_ptr(),
+                          tables["nu"].data_ptr(), tables["p"].data_ptr(), tables["t"].data_ptr(),
+                          u.data_ptr(), tables["eps"].data_ptr(), None, None, None)
+        keep = None
+        if filters is not None:
+            off = np.ascontiguousarray(filters[0], dtype=np.int32)
+            fx = np.ascontiguousarray(filters[1], dtype=np.float64)
+            fy = np.ascontiguousarray(filters[2], dtype=np.float64)
+            keep = (off, fx, fy)
+            d.filt_off, d.filt_nu, d.filt_w = off.ctypes.data, fx.ctypes.data, fy.ctypes.data
+        _check(lib().jur_ctx_create_device_tables(
+            C.byref(ctl), len(nu), nu.ctypes.data_as(C.POINTER(C.c_double)),
+            None if w is None else w.ctypes.data_as(C.POINTER(C.c_int)), C.byref(d), C.byref(self.ptr)))
+        del keep
+        self.nd = lib().jur_ctx_nd(self.ptr)
+        return self
 
     def formod_device(self, nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, rad_ptr, tau_ptr,
                       tp_ptr=None, mask_ptr=None, precision=abi.JUR_FP64, stream=None):

@@ -183,6 +183,46 @@ def write_table_bin(path, p, t, u, eps):
                 f.write(np.asarray(eps[ip, it], dtype="<f4").tobytes())
 
 
+def read_table_bin(path):
+    """Read a binary table (layout of write_table_bin): returns (p[np],
+    [t[nt] per p], [[u[nu], eps[nu]] per (p, T)])."""
+    with open(path, "rb") as f:
+        data = f.read()
+    off = 0
+
+    def take(fmt, n):
+        nonlocal off
+        a = np.frombuffer(data, dtype=fmt, count=n, offset=off)
+        off += a.nbytes
+        return a
+    np_ = int(take("<i4", 1)[0])
+    p = take("<f8", np_).copy()
+    ts, cols = [], []
+    for _ in range(np_):
+        nt = int(take("<i4", 1)[0])
+        ts.append(take("<f8", nt).copy())
+        row = []
+        for _ in range(nt):
+            nu = int(take("<i4", 1)[0])
+            row.append((take("<f4", nu).copy(), take("<f4", nu).copy()))
+        cols.append(row)
+    return p, ts, cols
+
+
+def read_filter(path):
+    """Filter function file (two columns, '#' comments) as read_shape reads it."""
+    x, y = [], []
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and not line.lstrip().startswith("#"):
+            try:
+                x.append(float(parts[0]))
+                y.append(float(parts[1]))
+            except ValueError:
+                pass
+    return np.array(x), np.array(y)
+
+
 def write_table_ascii(path, p, t, u, eps):
     """ASCII table rows "p T u eps" as write_tbl (src/jurassic.c:5836-5853)."""
     with open(path, "w") as f:
