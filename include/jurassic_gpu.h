@@ -274,6 +274,40 @@ int jur_gpu_optimal_estimation(const jur_ret_t *ret, const ctl_t *ctl, int n,
                                atm_t *atm_i, double *chisq_hist, int *niter,
                                int method, int precision);
 
+/* Error analysis of a retrieval (optimal_estimation, src/retrieval.c:
+   668-726, with ERR_ANA = 1), at the last Jacobian of the iteration; n is
+   the state size (atm2x) and m the number of finite measurements (obs2y).
+   Every pointer may be NULL (that output is not produced):
+     cov   n x n retrieval covariance (K^T S_eps^-1 K + S_a^-1)^-1
+           (matrix_cov_ret.tab)
+     corr  n x n correlation matrix (matrix_corr.tab)
+     gain  n x m gain matrix cov K^T S_eps^-1 (matrix_gain.tab)
+     avk   n x n averaging kernels gain K (matrix_avk.tab)
+     err_total, err_noise, err_formod  n standard deviations: sqrt of the
+           diagonals of cov, G S_noise G^T, G S_formod G^T (write_stddev)
+     cont, res  n: per state element the sum of its averaging-kernel row
+           over its own quantity and 1 / its diagonal (analyze_avk)
+     atm_*  the same as atmospheres, as the reference writes them to
+           atm_err_{total,noise,formod}.tab, atm_cont.tab and atm_res.tab
+           (copy_atm with init = 1 of the final state, then the values at
+           the state elements). */
+typedef struct {
+  double *cov, *corr, *gain, *avk;
+  double *err_total, *err_noise, *err_formod;
+  double *cont, *res;
+  atm_t *atm_err_total, *atm_err_noise, *atm_err_formod;
+  atm_t *atm_cont, *atm_res;
+} jur_ana_t;
+
+/* jur_gpu_optimal_estimation with the error analysis of retrieval i into
+   ana[i] (ana may be NULL: no analysis). */
+int jur_gpu_optimal_estimation_ana(const jur_ret_t *ret, const ctl_t *ctl,
+                                   int n, obs_t *obs_meas, obs_t *obs_i,
+                                   atm_t *atm_apr, atm_t *atm_i,
+                                   double *chisq_hist, int *niter,
+                                   int method, int precision,
+                                   const jur_ana_t *ana);
+
 /* ------------------------------------------------------------------
    Request server.  Forward models, Jacobians and retrievals are submitted
    at any time from any thread; worker lanes (threads with their own HIP

@@ -166,4 +166,16 @@ struct BatchView {
   const double *ray_geom;   // [nray][7]
 };
 
+// One retrieval of the batched Levenberg-Marquardt engine (lm_kernels.hip,
+// retrieval.cpp): sizes and device buffers.
+struct LmDesc {
+  int n, m;                  // state size, measurement rows (rays x channels)
+  int ray0, nray;            // rays of the retrieval in the current pass
+  double *sainv, *cmat, *a;  // n x n: S_a^-1, K^T S_eps^-1 K, work
+  double *k;                 // m x n: K scaled by 1/sigma_eps
+  double *sig, *dy;          // m: 1/sigma_eps (0: masked), y_meas - y
+  double *dx, *b, *step;     // n
+  double *g;                 // n x m gain matrix (error analysis only)
+};
+
 }  // namespace jur

@@ -16,21 +16,16 @@
 //  k_lm_solve    A = S_a^-1 (1 + lambda) + C, Cholesky A = L L^T, L L^T s = b
 //  k_lm_quad     dx^T S_a^-1 dx (the a priori part of cost_function,
 //                src/retrieval.c:340-369)
+//  k_ana_cov     error analysis (src/retrieval.c:668-726): retrieval
+//                covariance (C + S_a^-1)^-1 into the S_a^-1 buffer
+//  k_ana_gain    gain matrix G = cov K^T S_eps^-1 = cov Kc^T diag(1/sigma)
+//  k_ana_avk     averaging kernels A = G K = cov C into the work buffer
 //
 // Cholesky is the right-looking column algorithm in global memory (the
 // matrices stay in L2: n <= 324 is 0.8 MB); every step is one barrier.
 #include "jur_device.hpp"
 
 namespace jur {
-
-struct LmDesc {
-  int n, m;           // state size, measurement rows (rays x channels)
-  int ray0, nray;     // rays of the retrieval in the current pass
-  double *sainv, *cmat, *a;  // n x n: S_a^-1, K^T S_eps^-1 K, work
-  double *k;          // m x n: K scaled by 1/sigma_eps
-  double *sig, *dy;   // m: 1/sigma_eps (0: masked), y_meas - y
-  double *dx, *b, *step;     // n
-};
 
 constexpr int LM_T = 256;
 
@@ -153,6 +148,8 @@ __device__ void chol_solve_wg(const double *l, int n, double *x) {
   }
 }
 
+__device__ void chol_inverse_wg(const double *a, int n, double *inv);
+
 // S_a^-1: a holds S_a on entry and is overwritten by its Cholesky factor;
 // inv receives the inverse.  A diagonal S_a is inverted elementwise, as the
 // reference does.  status[r] = 1 if S_a is not positive definite.
@@ -177,7 +174,13 @@ __global__ __launch_bounds__(LM_T) void k_lm_cholinv(const LmDesc *D, const int 
     if (threadIdx.x == 0) status[r] = 1;
     return;
   }
-  // column c of the inverse: L y = e_c, L^T z = y, one thread per column
+  chol_inverse_wg(a, n, inv);
+  if (threadIdx.x == 0) status[r] = 0;
+}
+
+// (L L^T)^-1 from the Cholesky factor in the lower triangle of a: column c
+// of the inverse by L y = e_c, L^T z = y, one thread per column.
+__device__ void chol_inverse_wg(const double *a, int n, double *inv) {
   for (int c = threadIdx.x; c < n; c += LM_T) {
     // y (forward), stored in column c of inv, rows >= c
     for (int i = 0; i < n; i++) {
@@ -194,7 +197,6 @@ __global__ __launch_bounds__(LM_T) void k_lm_cholinv(const LmDesc *D, const int 
       inv[(size_t)i * n + c] = v / a[(size_t)i * n + i];
     }
   }
-  if (threadIdx.x == 0) status[r] = 0;
 }
 
 // b = Kc^T (sig_inv * dy) - S_a^-1 dx  (Kc is already scaled by sig_inv).
@@ -296,6 +298,52 @@ __global__ __launch_bounds__(LM_T) void k_lm_cov_apr(const LmDesc *D, const int 
   }
 }
 
+// Retrieval covariance of the error analysis: a = C + S_a^-1 (C =
+// Kc^T Kc of the last Jacobian), a = L L^T, inverse into the S_a^-1 buffer
+// (matrix_invert, src/retrieval.c:373-399).  status[r] = 1 if a is not
+// positive definite.
+__global__ __launch_bounds__(LM_T) void k_ana_cov(const LmDesc *D, const int *list, int *status) {
+  __shared__ int bad;
+  const int r = list[blockIdx.x];
+  const LmDesc d = D[r];
+  const int n = d.n;
+  if (threadIdx.x == 0) bad = 0;
+  for (long e = threadIdx.x; e < (long)n * n; e += LM_T) d.a[e] = d.cmat[e] + d.sainv[e];
+  __syncthreads();
+  if (!chol_wg(d.a, n, &bad)) {
+    if (threadIdx.x == 0) status[r] = 1;
+    return;
+  }
+  chol_inverse_wg(d.a, n, d.sainv);
+  if (threadIdx.x == 0) status[r] = 0;
+}
+
+// G[i][j] = sum_k cov[i][k] Kc[j][k] sig[j] (cov in the S_a^-1 buffer).
+__global__ __launch_bounds__(LM_T) void k_ana_gain(const LmDesc *D, const int *list) {
+  const LmDesc d = D[list[blockIdx.y]];
+  const long tot = (long)d.n * d.m;
+  for (long e = (long)blockIdx.x * LM_T + threadIdx.x; e < tot; e += (long)gridDim.x * LM_T) {
+    const int i = (int)(e / d.m), j = (int)(e - (long)i * d.m);
+    const double *ci = d.sainv + (size_t)i * d.n, *kj = d.k + (size_t)j * d.n;
+    double v = 0;
+    for (int k = 0; k < d.n; k++) v += ci[k] * kj[k];
+    d.g[e] = v * d.sig[j];
+  }
+}
+
+// A[i][j] = sum_k cov[i][k] C[k][j] into the work buffer.
+__global__ __launch_bounds__(LM_T) void k_ana_avk(const LmDesc *D, const int *list) {
+  const LmDesc d = D[list[blockIdx.y]];
+  const long tot = (long)d.n * d.n;
+  for (long e = (long)blockIdx.x * LM_T + threadIdx.x; e < tot; e += (long)gridDim.x * LM_T) {
+    const int i = (int)(e / d.n), j = (int)(e - (long)i * d.n);
+    const double *ci = d.sainv + (size_t)i * d.n;
+    double v = 0;
+    for (int k = 0; k < d.n; k++) v += ci[k] * d.cmat[(size_t)k * d.n + j];
+    d.a[e] = v;
+  }
+}
+
 // ---------------- launchers ----------------
 void launch_lm_cov_apr(const LmDesc *D, const int *list, int cnt, const size_t *off, const double *stage,
                        hipStream_t s) {
@@ -332,6 +380,13 @@ void launch_lm_solve(const LmDesc *D, const int *list, int cnt, const double *lm
 void launch_lm_quad(const LmDesc *D, const int *list, int cnt, double *q, hipStream_t s) {
   if (cnt == 0) return;
   hipLaunchKernelGGL(k_lm_quad, dim3(cnt), dim3(LM_T), 0, s, D, list, q);
+}
+
+void launch_ana(const LmDesc *D, const int *list, int cnt, int *status, hipStream_t s, int phase) {
+  if (cnt == 0) return;
+  if (phase == 0) hipLaunchKernelGGL(k_ana_cov, dim3(cnt), dim3(LM_T), 0, s, D, list, status);
+  if (phase == 1) hipLaunchKernelGGL(k_ana_gain, dim3(64, cnt), dim3(LM_T), 0, s, D, list);
+  if (phase == 2) hipLaunchKernelGGL(k_ana_avk, dim3(64, cnt), dim3(LM_T), 0, s, D, list);
 }
 
 }  // namespace jur

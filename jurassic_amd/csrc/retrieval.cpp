@@ -32,14 +32,6 @@
 
 namespace jur {
 
-struct LmDesc {
-  int n, m;
-  int ray0, nray;
-  double *sainv, *cmat, *a;
-  double *k;
-  double *sig, *dy;
-  double *dx, *b, *step;
-};
 
 void launch_lm_gather(const LmDesc *, const int *, int, int, const double *, int, int, hipStream_t);
 void launch_lm_normal(const LmDesc *, const int *, int, int, hipStream_t);
@@ -49,6 +41,7 @@ void launch_lm_solve(const LmDesc *, const int *, int, const double *, int *, hi
 void launch_lm_quad(const LmDesc *, const int *, int, double *, hipStream_t);
 void launch_lm_stage(const LmDesc *, const int *, int, const size_t *, double *, int, hipStream_t);
 void launch_lm_cov_apr(const LmDesc *, const int *, int, const size_t *, const double *, hipStream_t);
+void launch_ana(const LmDesc *, const int *, int, int *, hipStream_t, int);
 
 // Device memory for the jobs' matrices and vectors: blocks of 2^k doubles
 // kept on free lists and reused, so that admitting a request allocates
@@ -128,7 +121,12 @@ struct Job {
   std::vector<double> own_hist, own_k;
   int own_niter = 0;
 
-  LmDesc desc() { return LmDesc{n, m, ray0, nr, dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], dp[8]}; }
+  const jur_ana_t *ana = nullptr;  // error analysis outputs (retrievals)
+  double *g = nullptr;             // gain matrix during the analysis
+  int gc = 0;
+  LmDesc desc() {
+    return LmDesc{n, m, ray0, nr, dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], dp[8], g};
+  }
   bool failed() const { return rc != JUR_OK; }
 };
 
@@ -282,6 +280,7 @@ struct Engine {
   int eval(const std::vector<int> &ids, const std::vector<char> &wantk);
   int chisq(const std::vector<int> &ids);
   int stage_get(const std::vector<int> &ids, int which);
+  int analyze(const std::vector<int> &ids);
 };
 
 int Engine::add(Job *J) {
@@ -841,6 +840,12 @@ int Engine::round(std::vector<Job *> &done) {
       J.st = ST_EVAL_F;
     }
   }
+  // ---- error analysis of finished retrievals ----
+  std::vector<int> ana;
+  for (size_t r = 0; r < nj; r++)
+    if (jobs[r]->st == ST_DONE && jobs[r]->ana && !jobs[r]->failed() && jobs[r]->kind == JUR_REQ_RETRIEVAL)
+      ana.push_back((int)r);
+  if (!ana.empty() && (rc = analyze(ana))) return rc;
   prof[3] += tnow() - t2;
   // ---- finished jobs leave ----
   std::vector<Job *> keep;
@@ -857,9 +862,153 @@ int Engine::round(std::vector<Job *> &done) {
   return JUR_OK;
 }
 
+// copy_atm(ctl, dst, src, 1) (src/jurassic.c:2921-2972): the geometry of
+// src with the retrievable quantities zeroed (surface emissivities 1).
+static void copy_atm_init(const ctl_t *ctl, atm_t *dst, const atm_t *src) {
+  memcpy(dst, src, sizeof(atm_t));
+  for (int ip = 0; ip < dst->np; ip++) {
+    dst->p[ip] = 0;
+    dst->t[ip] = 0;
+    for (int ig = 0; ig < ctl->ng; ig++) dst->q[ig][ip] = 0;
+    for (int iw = 0; iw < ctl->nw; iw++) dst->k[iw][ip] = 0;
+    dst->clz = 0;
+    dst->cldz = 0;
+    for (int icl = 0; icl < ctl->ncl; icl++) dst->clk[icl] = 0;
+    dst->sfz = 0;
+    dst->sfp = 0;
+    dst->sft = 0;
+    for (int isf = 0; isf < ctl->nsf; isf++) dst->sfeps[isf] = 1;
+  }
+}
+
+// Pointer to the atm_t value of state element i (quantity iqa, level ipa),
+// as analyze_avk addresses it (src/retrieval.c:257-312).
+static double *atm_elem(const ctl_t *ctl, atm_t *a, int iq, int ip) {
+  const int ng = ctl->ng, nw = ctl->nw, ncl = ctl->ncl;
+  if (iq == IDXP) return &a->p[ip];
+  if (iq == IDXT) return &a->t[ip];
+  if (iq >= 2 && iq < 2 + ng) return &a->q[iq - 2][ip];
+  if (iq >= 2 + ng && iq < 2 + ng + nw) return &a->k[iq - 2 - ng][ip];
+  int q = iq - 2 - ng - nw;
+  if (q == 0) return &a->clz;
+  if (q == 1) return &a->cldz;
+  if (q >= 2 && q < 2 + ncl) return &a->clk[q - 2];
+  q -= 2 + ncl;
+  if (q == 0) return &a->sfz;
+  if (q == 1) return &a->sfp;
+  if (q == 2) return &a->sft;
+  return &a->sfeps[q - 3];
+}
+
+// Error analysis (src/retrieval.c:668-726) of finished retrievals at their
+// last Jacobian: covariance, gain and averaging kernels on the device, the
+// diagonals and the atmosphere mappings on the host.
+int Engine::analyze(const std::vector<int> &ids) {
+  int rc;
+  int maxn = 1;
+  for (int r : ids) {
+    Job &J = *jobs[r];
+    J.g = pool.get((size_t)J.n * J.m, J.gc);
+    if (!J.g) return set_err(JUR_ERR_MEM, "device allocation failed");
+    maxn = std::max(maxn, J.n);
+  }
+  if ((rc = upload_desc())) return rc;
+  if ((rc = list(ids))) return rc;
+  launch_lm_normal(d_desc.p, d_list.p, (int)ids.size(), maxn, st);
+  HIPCHK(hipGetLastError());
+  if ((rc = d_status.reserve(jobs.size()))) return rc;
+  launch_ana(d_desc.p, d_list.p, (int)ids.size(), d_status.p, st, 0);
+  HIPCHK(hipGetLastError());
+  std::vector<int> status(jobs.size(), 0);
+  HIPCHK(hipMemcpyAsync(status.data(), d_status.p, jobs.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<int> ok;
+  for (int r : ids) {
+    if (status[r]) fail(jobs[r], set_err(JUR_ERR_ARG, "retrieval covariance is not positive definite"));
+    else ok.push_back(r);
+  }
+  if (!ok.empty()) {
+    if ((rc = list(ok))) return rc;
+    launch_ana(d_desc.p, d_list.p, (int)ok.size(), d_status.p, st, 1);
+    launch_ana(d_desc.p, d_list.p, (int)ok.size(), d_status.p, st, 2);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<std::vector<double>> cov(ok.size()), gain(ok.size()), avk(ok.size());
+  for (size_t q = 0; q < ok.size(); q++) {
+    Job &J = *jobs[ok[q]];
+    const size_t nn = (size_t)J.n * J.n;
+    cov[q].resize(nn);
+    gain[q].resize((size_t)J.n * J.m);
+    avk[q].resize(nn);
+    HIPCHK(hipMemcpyAsync(cov[q].data(), J.dp[0], nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(gain[q].data(), J.g, gain[q].size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(avk[q].data(), J.dp[2], nn * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  for (int r : ids) {
+    pool.put(jobs[r]->g, jobs[r]->gc);
+    jobs[r]->g = nullptr;
+  }
+  const int nd = ctl->nd;
+  for (size_t q = 0; q < ok.size(); q++) {
+    Job &J = *jobs[ok[q]];
+    const jur_ana_t &A = *J.ana;
+    const int n = J.n;
+    const double *C = cov[q].data(), *G = gain[q].data(), *V = avk[q].data();
+    // measurement rows of obs2y: the finite rows of the (ray, channel) grid
+    std::vector<int> rows;
+    for (int i = 0; i < J.m; i++)
+      if (J.sig[i] != 0) rows.push_back(i);
+    const int mv = (int)rows.size();
+    if (A.cov) memcpy(A.cov, C, (size_t)n * n * sizeof(double));
+    if (A.avk) memcpy(A.avk, V, (size_t)n * n * sizeof(double));
+    if (A.corr)
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+          A.corr[(size_t)i * n + j] = C[(size_t)i * n + j] / sqrt(C[(size_t)i * n + i]) / sqrt(C[(size_t)j * n + j]);
+    if (A.gain)
+      for (int i = 0; i < n; i++)
+        for (int jj = 0; jj < mv; jj++) A.gain[(size_t)i * mv + jj] = G[(size_t)i * J.m + rows[jj]];
+    // standard deviations: sqrt(diag(cov)), sqrt(diag(G S G^T)) for the
+    // noise and forward-model errors of set_cov_meas (src/retrieval.c:924-964)
+    std::vector<double> et(n), en(n), ef(n), cont(n, 0.0), res(n);
+    for (int i = 0; i < n; i++) {
+      et[i] = sqrt(C[(size_t)i * n + i]);
+      double sn = 0, sf = 0;
+      for (int jj = 0; jj < mv; jj++) {
+        const int row = rows[jj], id = row % nd;
+        const double g = G[(size_t)i * J.m + row];
+        sn += POW2(g * ret->err_noise[id]);
+        sf += POW2(g * fabs(ret->err_formod[id] / 100 * J.y_m[row]));
+      }
+      en[i] = sqrt(sn);
+      ef[i] = sqrt(sf);
+    }
+    // analyze_avk_quantity: row sums within the quantity, 1 / diagonal
+    for (int i = 0; i < n; i++) {
+      for (int j = 0; j < n; j++)
+        if (J.iqa[j] == J.iqa[i]) cont[i] += V[(size_t)i * n + j];
+      res[i] = 1 / V[(size_t)i * n + i];
+    }
+    if (A.err_total) memcpy(A.err_total, et.data(), n * sizeof(double));
+    if (A.err_noise) memcpy(A.err_noise, en.data(), n * sizeof(double));
+    if (A.err_formod) memcpy(A.err_formod, ef.data(), n * sizeof(double));
+    if (A.cont) memcpy(A.cont, cont.data(), n * sizeof(double));
+    if (A.res) memcpy(A.res, res.data(), n * sizeof(double));
+    const std::vector<double> *vx[5] = {&et, &en, &ef, &cont, &res};
+    atm_t *va[5] = {A.atm_err_total, A.atm_err_noise, A.atm_err_formod, A.atm_cont, A.atm_res};
+    for (int k = 0; k < 5; k++) {
+      if (!va[k]) continue;
+      copy_atm_init(J.sctl, va[k], J.atm_i);
+      for (int i = 0; i < n; i++) *atm_elem(J.sctl, va[k], J.iqa[i], J.ipa[i]) = (*vx[k])[i];
+    }
+  }
+  return JUR_OK;
+}
+
 int optimal_estimation_batch(const jur_ret_t *ret, const ctl_t *ctl, int n, obs_t *obs_meas, obs_t *obs_i,
                              atm_t *atm_apr, atm_t *atm_i, double *chisq_hist, int *niter, int method,
-                             int precision) {
+                             int precision, const jur_ana_t *ana) {
   if (!ret || !ctl || n < 0 || (n > 0 && (!obs_meas || !obs_i || !atm_apr || !atm_i)))
     return set_err(JUR_ERR_ARG, "invalid argument");
   if (method != JUR_JAC_FD && method != JUR_JAC_ADJOINT) return set_err(JUR_ERR_ARG, "invalid Jacobian method");
@@ -886,6 +1035,7 @@ int optimal_estimation_batch(const jur_ret_t *ret, const ctl_t *ctl, int n, obs_
     J.obs_i = &obs_i[r];
     J.hist = chisq_hist ? chisq_hist + (size_t)r * (ret->conv_itmax + 1) : nullptr;
     J.niter = niter ? niter + r : nullptr;
+    J.ana = ana ? ana + r : nullptr;
     if ((rc = E.add(&J))) return set_err(rc, "retrieval %d: %s", r, J.err);
   }
   std::vector<Job *> done;
@@ -1015,7 +1165,14 @@ int jur_gpu_optimal_estimation(const jur_ret_t *ret, const ctl_t *ctl, int n, ob
                                atm_t *atm_apr, atm_t *atm_i, double *chisq_hist, int *niter, int method,
                                int precision) {
   return jur::optimal_estimation_batch(ret, ctl, n, obs_meas, obs_i, atm_apr, atm_i, chisq_hist, niter, method,
-                                       precision);
+                                       precision, nullptr);
+}
+
+int jur_gpu_optimal_estimation_ana(const jur_ret_t *ret, const ctl_t *ctl, int n, obs_t *obs_meas, obs_t *obs_i,
+                                   atm_t *atm_apr, atm_t *atm_i, double *chisq_hist, int *niter, int method,
+                                   int precision, const jur_ana_t *ana) {
+  return jur::optimal_estimation_batch(ret, ctl, n, obs_meas, obs_i, atm_apr, atm_i, chisq_hist, niter, method,
+                                       precision, ana);
 }
 
 int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret, const jur_server_opts_t *opt, jur_server_t **out) {

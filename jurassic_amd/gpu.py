@@ -44,8 +44,17 @@ EXPORTS = [
     "jur_ctx_last_timing", "jur_ctx_last_mean_np", "jur_state_map", "jur_ctx_jacobian_device",
     "jur_gpu_optimal_estimation", "jur_server_create", "jur_server_destroy", "jur_server_submit",
     "jur_server_wait", "jur_server_fetch", "jur_server_stats", "jur_server_profile",
-    "jur_ctx_create_device_tables",
+    "jur_ctx_create_device_tables", "jur_gpu_optimal_estimation_ana",
 ]
+
+ANA_MATS = ("cov", "corr", "gain", "avk")
+ANA_VECS = ("err_total", "err_noise", "err_formod", "cont", "res")
+ANA_ATMS = ("atm_err_total", "atm_err_noise", "atm_err_formod", "atm_cont", "atm_res")
+
+
+class jur_ana_t(C.Structure):
+    """Error-analysis outputs of one retrieval (jurassic_gpu.h)."""
+    _fields_ = [(k, C.c_void_p) for k in ANA_MATS + ANA_VECS + ANA_ATMS]
 
 
 class jur_dtables_t(C.Structure):
@@ -102,6 +111,8 @@ def lib():
         L.jur_ctx_jacobian_device.restype = I
         L.jur_gpu_optimal_estimation.argtypes = [P, P, I, P, P, P, P, C.POINTER(D), C.POINTER(I), I, I]
         L.jur_gpu_optimal_estimation.restype = I
+        L.jur_gpu_optimal_estimation_ana.argtypes = [P, P, I, P, P, P, P, C.POINTER(D), C.POINTER(I), I, I, P]
+        L.jur_gpu_optimal_estimation_ana.restype = I
         L.jur_server_create.argtypes = [P, P, C.POINTER(abi.server_opts_t), C.POINTER(P)]
         L.jur_server_create.restype = I
         L.jur_server_destroy.argtypes = [P]
@@ -149,10 +160,13 @@ def kernel(ctl, atm, obs, method=abi.JUR_JAC_FD, precision=abi.JUR_FP64, m=None,
 
 
 def optimal_estimation(ret, ctl, obs_meas, atm_apr, method=abi.JUR_JAC_ADJOINT,
-                       precision=abi.JUR_FP64):
+                       precision=abi.JUR_FP64, analysis=False):
     """optimal_estimation on the GPU for one retrieval (obs_t, atm_t) or a
     list of them.  Returns (atm_i, obs_i, chisq_hist, niter), each a list
-    when lists were given."""
+    when lists were given; with analysis=True a fifth item, the error
+    analysis (dict of cov, corr, gain, avk, err_total, err_noise,
+    err_formod, cont, res as numpy arrays and atm_err_total, atm_err_noise,
+    atm_err_formod, atm_cont, atm_res as atm_t)."""
     single = not isinstance(obs_meas, (list, tuple))
     obss = [obs_meas] if single else list(obs_meas)
     atms = [atm_apr] if single else list(atm_apr)
@@ -163,16 +177,41 @@ def optimal_estimation(ret, ctl, obs_meas, atm_apr, method=abi.JUR_JAC_ADJOINT,
     ai = (abi.atm_t * n)()
     hist = np.zeros(n * (ret.conv_itmax + 1))
     nit = (C.c_int * n)()
-    _check(lib().jur_gpu_optimal_estimation(
+    ana = None
+    res = []
+    if analysis:
+        ana = (jur_ana_t * n)()
+        for i in range(n):
+            _, nx = state_map(ctl, atms[i])
+            r, _ = abi.obs_arrays(obss[i], ctl)
+            my = int(np.isfinite(r).sum())
+            d = {}
+            for k in ANA_MATS:
+                d[k] = np.zeros((nx, my) if k == "gain" else (nx, nx))
+            for k in ANA_VECS:
+                d[k] = np.zeros(nx)
+            for k in ANA_ATMS:
+                d[k] = abi.atm_t()
+            for k in ANA_MATS + ANA_VECS:
+                setattr(ana[i], k, d[k].ctypes.data)
+            for k in ANA_ATMS:
+                setattr(ana[i], k, C.addressof(d[k]))
+            res.append(d)
+    _check(lib().jur_gpu_optimal_estimation_ana(
         C.byref(ret), C.byref(ctl), n, C.cast(om, C.c_void_p), C.cast(oi, C.c_void_p), C.cast(aa, C.c_void_p),
-        C.cast(ai, C.c_void_p), hist.ctypes.data_as(C.POINTER(C.c_double)), nit, method, precision))
+        C.cast(ai, C.c_void_p), hist.ctypes.data_as(C.POINTER(C.c_double)), nit, method, precision,
+        C.cast(ana, C.c_void_p) if ana is not None else None))
     hist = hist.reshape(n, ret.conv_itmax + 1)
     out_a = [abi.copy_struct(ai[i]) for i in range(n)]
     out_o = [abi.copy_struct(oi[i]) for i in range(n)]
     out_h = [hist[i, :nit[i] + 1].copy() for i in range(n)]
     out_n = [nit[i] for i in range(n)]
     if single:
+        if analysis:
+            return out_a[0], out_o[0], out_h[0], out_n[0], res[0]
         return out_a[0], out_o[0], out_h[0], out_n[0]
+    if analysis:
+        return out_a, out_o, out_h, out_n, res
     return out_a, out_o, out_h, out_n
 
 

@@ -1533,10 +1533,96 @@ static int oe_eval(const ctl_t *ctl, orc_fwd_fn fwd, void *user, atm_t *atm_i,
   return rc;
 }
 
+/* Error analysis (src/retrieval.c:668-726) from the last Jacobian k_i. */
+static int oe_analysis(const orc_ret_t *ret, const ctl_t *ctl, const obs_t *obs_meas,
+                       const double *k_i, const double *s_a_inv, const double *sig_eps_inv,
+                       const int *iqa, size_t m, size_t n, const orc_ana_t *ana) {
+  double *cov = malloc(n * n * sizeof(double)), *gain = malloc(n * m * sizeof(double));
+  double *auxnm = malloc(n * m * sizeof(double)), *a = malloc(n * n * sizeof(double));
+  double *sig_noise = malloc(m * sizeof(double)), *sig_formod = malloc(m * sizeof(double));
+  int rc;
+  /* cov^-1 = K^T S_eps^-1 K + S_a^-1 (matrix_product, :403-444) */
+  for (size_t i = 0; i < n; i++)
+    for (size_t j = 0; j < n; j++) {
+      double v = 0;
+      for (size_t r = 0; r < m; r++) v += (sig_eps_inv[r] * k_i[r * n + i]) * (sig_eps_inv[r] * k_i[r * n + j]);
+      cov[i * n + j] = v + s_a_inv[i * n + j];
+    }
+  rc = matrix_invert(cov, n);
+  if (rc) goto out;
+  if (ana->cov) memcpy(ana->cov, cov, n * n * sizeof(double));
+  if (ana->err_total)
+    for (size_t i = 0; i < n; i++) ana->err_total[i] = sqrt(cov[i * n + i]);
+  if (ana->corr)
+    for (size_t i = 0; i < n; i++)
+      for (size_t j = 0; j < n; j++)
+        ana->corr[i * n + j] = cov[i * n + j] / sqrt(cov[i * n + i]) / sqrt(cov[j * n + j]);
+  /* G = cov K^T S_eps^-1 */
+  for (size_t i = 0; i < n; i++)
+    for (size_t j = 0; j < m; j++) auxnm[i * m + j] = k_i[j * n + i] * POW2(sig_eps_inv[j]);
+  for (size_t i = 0; i < n; i++)
+    for (size_t j = 0; j < m; j++) {
+      double v = 0;
+      for (size_t k = 0; k < n; k++) v += cov[i * n + k] * auxnm[k * m + j];
+      gain[i * m + j] = v;
+    }
+  if (ana->gain) memcpy(ana->gain, gain, n * m * sizeof(double));
+  /* noise and forward-model errors of set_cov_meas (:924-964), then
+     G S G^T = (G S^1/2)(G S^1/2)^T (matrix_product with transpose = 2) */
+  {
+    size_t r = 0;
+    for (int ir = 0; ir < obs_meas->nr; ir++)
+      for (int id = 0; id < ctl->nd; id++)
+        if (isfinite(obs_meas->rad[id][ir])) {
+          sig_noise[r] = ret->err_noise[id];
+          sig_formod[r] = fabs(ret->err_formod[id] / 100 * obs_meas->rad[id][ir]);
+          r++;
+        }
+  }
+  for (int which = 0; which < 2; which++) {
+    const double *sg = which == 0 ? sig_noise : sig_formod;
+    double *out = which == 0 ? ana->err_noise : ana->err_formod;
+    if (!out) continue;
+    for (size_t i = 0; i < n; i++) {
+      double v = 0;
+      for (size_t j = 0; j < m; j++) v += (gain[i * m + j] * sg[j]) * (gain[i * m + j] * sg[j]);
+      out[i] = sqrt(v);
+    }
+  }
+  /* A = G K (:712-716) and analyze_avk (:257-338) */
+  for (size_t i = 0; i < n; i++)
+    for (size_t j = 0; j < n; j++) {
+      double v = 0;
+      for (size_t k = 0; k < m; k++) v += gain[i * m + k] * k_i[k * n + j];
+      a[i * n + j] = v;
+    }
+  if (ana->avk) memcpy(ana->avk, a, n * n * sizeof(double));
+  for (size_t i = 0; i < n; i++) {
+    if (ana->cont) {
+      double v = 0;
+      for (size_t j = 0; j < n; j++)
+        if (iqa[j] == iqa[i]) v += a[i * n + j];
+      ana->cont[i] = v;
+    }
+    if (ana->res) ana->res[i] = 1 / a[i * n + i];
+  }
+out:
+  free(cov); free(gain); free(auxnm); free(a); free(sig_noise); free(sig_formod);
+  return rc;
+}
+
 int orc_optimal_estimation(const orc_ret_t *ret, const ctl_t *ctl,
                            obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
                            atm_t *atm_i, double *chisq_hist, int *niter,
                            orc_fwd_fn fwd, void *user) {
+  return orc_optimal_estimation_ana(ret, ctl, obs_meas, obs_i, atm_apr, atm_i, chisq_hist, niter, fwd, user,
+                                    NULL);
+}
+
+int orc_optimal_estimation_ana(const orc_ret_t *ret, const ctl_t *ctl,
+                               obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
+                               atm_t *atm_i, double *chisq_hist, int *niter,
+                               orc_fwd_fn fwd, void *user, const orc_ana_t *ana) {
   const size_t m = orc_obs2y(ctl, obs_meas, NULL, NULL, NULL);
   int *iqa = malloc(sizeof(int) * (size_t)(NP * (NG + NW + 2) + 64));
   int *ipa = malloc(sizeof(int) * (size_t)(NP * (NG + NW + 2) + 64));
@@ -1654,6 +1740,7 @@ int orc_optimal_estimation(const orc_ret_t *ret, const ctl_t *ctl,
     if ((it == 1 || it % ret->kernel_recomp == 0) && disq < ret->conv_dmin) break;
   }
   if (it > ret->conv_itmax) it = ret->conv_itmax;
+  if (ana) rc = oe_analysis(ret, ctl, obs_meas, k_i, s_a_inv, sig_eps_inv, iqa, m, n, ana);
 done:
   if (niter) *niter = it;
   free(a); free(cov); free(k_i); free(s_a_inv); free(b); free(dx); free(dy);

@@ -144,6 +144,24 @@ int orc_optimal_estimation(const orc_ret_t *ret, const ctl_t *ctl,
                            atm_t *atm_i, double *chisq_hist, int *niter,
                            orc_fwd_fn fwd, void *user);
 
+/* Error analysis of optimal_estimation (src/retrieval.c:668-726) at the
+   last Jacobian: cov, corr, avk n x n; gain n x m; err_total, err_noise,
+   err_formod, cont, res n (standard deviations of write_stddev, the
+   analyze_avk area and 1 / diagonal per state element).  NULL pointers are
+   skipped. */
+typedef struct {
+  double *cov, *corr, *gain, *avk;
+  double *err_total, *err_noise, *err_formod;
+  double *cont, *res;
+} orc_ana_t;
+
+/* orc_optimal_estimation followed by the error analysis into *ana. */
+int orc_optimal_estimation_ana(const orc_ret_t *ret, const ctl_t *ctl,
+                               obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
+                               atm_t *atm_i, double *chisq_hist, int *niter,
+                               orc_fwd_fn fwd, void *user,
+                               const orc_ana_t *ana);
+
 /* A priori covariance (src/retrieval.c:808-920), n x n row-major. */
 int orc_set_cov_apr(const orc_ret_t *ret, const ctl_t *ctl, const atm_t *atm,
                     const int *iqa, const int *ipa, size_t n, double *s_a);

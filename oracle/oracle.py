@@ -82,6 +82,8 @@ def lib():
         L.orc_last_error.restype = C.c_char_p
         L.orc_optimal_estimation.argtypes = [P, P, P, P, P, P, dp, ip, P, P]
         L.orc_optimal_estimation.restype = I
+        L.orc_optimal_estimation_ana.argtypes = [P, P, P, P, P, P, dp, ip, P, P, P]
+        L.orc_optimal_estimation_ana.restype = I
         L.orc_set_cov_apr.argtypes = [P, P, P, ip, ip, S, dp]
         L.orc_set_cov_apr.restype = I
         _LIB = L
@@ -166,18 +168,35 @@ FWD_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_doub
                      C.POINTER(C.c_double))
 
 
-def optimal_estimation(ret, ctl, obs_meas, atm_apr, fwd=None):
-    """Run the oracle LM retrieval; returns (atm_i, obs_i, chisq_hist, niter)."""
+class orc_ana_t(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("cov", "corr", "gain", "avk", "err_total", "err_noise",
+                                          "err_formod", "cont", "res")]
+
+
+def optimal_estimation(ret, ctl, obs_meas, atm_apr, fwd=None, analysis=False):
+    """Run the oracle LM retrieval; returns (atm_i, obs_i, chisq_hist, niter)
+    and, with analysis=True, the error analysis (src/retrieval.c:668-726) as
+    a dict of numpy arrays (cov, corr, gain, avk, err_total, err_noise,
+    err_formod, cont, res)."""
     from jurassic_amd import abi
     obs_i = abi.obs_t()
     atm_i = abi.atm_t()
     hist = np.zeros(ret.conv_itmax + 2)
     nit = C.c_int()
     cb = FWD_FN(fwd) if fwd is not None else None
-    rc = lib().orc_optimal_estimation(C.byref(ret), C.byref(ctl), C.byref(obs_meas),
-                                      C.byref(obs_i), C.byref(atm_apr), C.byref(atm_i),
-                                      _ptr(hist), C.byref(nit),
-                                      C.cast(cb, C.c_void_p) if cb else None, None)
+    ana, res = None, None
+    if analysis:
+        m, n = sizes(ctl, atm_apr, obs_meas)
+        res = {k: np.zeros((n, m) if k == "gain" else (n, n)) for k in ("cov", "corr", "gain", "avk")}
+        res.update({k: np.zeros(n) for k in ("err_total", "err_noise", "err_formod", "cont", "res")})
+        ana = orc_ana_t(*[res[f].ctypes.data for f, _ in orc_ana_t._fields_])
+    rc = lib().orc_optimal_estimation_ana(C.byref(ret), C.byref(ctl), C.byref(obs_meas),
+                                          C.byref(obs_i), C.byref(atm_apr), C.byref(atm_i),
+                                          _ptr(hist), C.byref(nit),
+                                          C.cast(cb, C.c_void_p) if cb else None, None,
+                                          C.byref(ana) if ana is not None else None)
     if rc:
         raise RuntimeError("orc_optimal_estimation: %s" % lib().orc_last_error().decode())
+    if analysis:
+        return atm_i, obs_i, hist[:nit.value + 1], nit.value, res
     return atm_i, obs_i, hist[:nit.value + 1], nit.value

@@ -117,3 +117,50 @@ def test_oe_adjoint_hydrostatic_unsupported(gpu, oracle):
     with pytest.raises(gpu.JurassicError) as e:
         gpu.optimal_estimation(ret, ctl, meas, atm_apr, method=abi.JUR_JAC_ADJOINT)
     assert e.value.code == 6
+
+
+def test_oe_error_analysis_matches_oracle(gpu, oracle):
+    """Error analysis (src/retrieval.c:668-726) of the GPU retrieval against
+    the oracle's on the same problem (finite-difference Jacobians, FP64: the
+    same iteration, so the same final Jacobian to the retrieval's own
+    agreement), and the atmosphere outputs against the state vectors."""
+    ctl, atm_true, atm_apr, obs, ret = _problem()
+    for i in range(ctl.nd):
+        ret.err_formod[i] = 1.0 if i % 2 else 0.0
+    meas = _measure(oracle, ctl, atm_true, obs)
+    *_, a0 = oracle.optimal_estimation(ret, ctl, abi.copy_struct(meas), abi.copy_struct(atm_apr), analysis=True)
+    atm_i, _, _, _, a1 = gpu.optimal_estimation(ret, ctl, abi.copy_struct(meas), abi.copy_struct(atm_apr),
+                                                method=abi.JUR_JAC_FD, precision=abi.JUR_FP64, analysis=True)
+    for k in ("cov", "corr", "gain", "avk", "err_total", "err_noise", "err_formod", "cont", "res"):
+        scale = np.abs(a0[k]).max()
+        np.testing.assert_allclose(a1[k], a0[k], rtol=1e-5, atol=1e-7 * scale, err_msg=k)
+    # atm_err_*, atm_cont, atm_res: the state vectors at the state elements
+    # (T and H2O here), zero elsewhere (copy_atm with init = 1)
+    xm, n = gpu.state_map(ctl, atm_apr)
+    for name, vec in (("atm_err_total", "err_total"), ("atm_err_noise", "err_noise"),
+                      ("atm_err_formod", "err_formod"), ("atm_cont", "cont"), ("atm_res", "res")):
+        a = a1[name]
+        assert a.np == atm_i.np
+        for ip in range(a.np):
+            assert a.t[ip] == a1[vec][xm[abi.xmap_lev(1, ip)]]
+            assert a.q[1][ip] == a1[vec][xm[abi.xmap_lev(3, ip)]]
+            assert a.p[ip] == 0 and a.q[0][ip] == 0 and a.q[2][ip] == 0
+            assert a.z[ip] == atm_i.z[ip]
+
+
+@pytest.mark.parametrize("precision", [abi.JUR_FP64, abi.JUR_FP32])
+def test_oe_error_analysis_adjoint(gpu, oracle, precision):
+    """With adjoint Jacobians: cov is symmetric, the total error squared is
+    its diagonal, the noise error is below the total error (G S_noise G^T <=
+    G S_eps G^T = cov - cov Sa^-1 cov), and A = I - cov Sa^-1 has its
+    eigenvalues in [0, 1), so 0 < trace(A) < n."""
+    ctl, atm_true, atm_apr, obs, ret = _problem()
+    meas = _measure(oracle, ctl, atm_true, obs)
+    *_, a1 = gpu.optimal_estimation(ret, ctl, abi.copy_struct(meas), abi.copy_struct(atm_apr),
+                                    method=abi.JUR_JAC_ADJOINT, precision=precision, analysis=True)
+    cov, avk = a1["cov"], a1["avk"]
+    assert np.allclose(cov, cov.T, rtol=1e-10, atol=1e-14 * np.abs(cov).max())
+    np.testing.assert_allclose(a1["err_total"] ** 2, np.diag(cov), rtol=1e-12)
+    assert np.all(a1["err_total"] > 0)
+    assert np.all(a1["err_noise"] <= a1["err_total"] * (1 + 1e-9))
+    assert 0 < np.trace(avk) < avk.shape[0]

@@ -381,3 +381,60 @@ def test_optimal_estimation_linear_model(oracle):
     xi = np.ctypeslib.as_array(atm_i.t)[:n]
     np.testing.assert_allclose(xi, xhat, rtol=0, atol=1e-6)
     assert np.all(np.diff(hist) <= 1e-12)
+
+
+def test_optimal_estimation_error_analysis_linear_model(oracle):
+    """Error analysis (src/retrieval.c:668-726) on a linear forward model
+    against its closed forms: cov = (K^T Se^-1 K + Sa^-1)^-1 with Se = noise^2
+    + (formod % y)^2, G = cov K^T Se^-1, noise and forward-model errors
+    sqrt(diag(G S G^T)), A = G K, and the identity A = I - cov Sa^-1."""
+    nd = 6
+    ctl = abi.make_ctl(["CO2"], [700.0 + i for i in range(nd)], rett_zmin=0, rett_zmax=100)
+    z = np.arange(0, 20.0, 1.0)
+    atm = abi.make_atm(z, 1013.25 * np.exp(-z / 7), 250 + np.zeros(20), [np.full(20, 4e-4)])
+    obs = scenes.make_obs(scenes.nadir_geometry(705.0, [0.0, 10.0, 20.0]))
+    m, n = 18, 20
+    rng = np.random.default_rng(7)
+    K = rng.normal(size=(m, n)) * 0.05
+    c = 50 + rng.normal(size=m)
+    y = K @ (250 + rng.normal(size=n) * 2) + c
+    for ir in range(3):
+        for id_ in range(nd):
+            obs.rad[id_][ir] = y[ir * nd + id_]
+
+    def fwd(user, xp, yp, kp):
+        x = np.ctypeslib.as_array(xp, shape=(n,))
+        np.ctypeslib.as_array(yp, shape=(m,))[:] = K @ x + c
+        if kp:
+            np.ctypeslib.as_array(kp, shape=(m * n,))[:] = K.ravel()
+
+    ret = oracle.orc_ret_t()
+    ret.kernel_recomp = 3
+    ret.conv_itmax = 30
+    ret.conv_dmin = 1e-8
+    noise = 0.1 + 0.05 * np.arange(nd)
+    formod = np.array([0.0, 1.0, 0.0, 2.0, 0.5, 0.0])  # percent
+    for i in range(nd):
+        ret.err_noise[i] = noise[i]
+        ret.err_formod[i] = formod[i]
+    ret.err_temp, ret.err_temp_cz, ret.err_temp_ch = 2.0, 5.0, 500.0
+    *_, ana = oracle.optimal_estimation(ret, ctl, obs, atm, fwd, analysis=True)
+    sa = 4.0 * np.exp(-np.abs(z[:, None] - z[None, :]) / 5.0)
+    sn = np.tile(noise, 3)
+    sf = np.abs(np.tile(formod, 3) / 100 * y)
+    se_inv = np.diag(1 / (sn ** 2 + sf ** 2))
+    cov = np.linalg.inv(K.T @ se_inv @ K + np.linalg.inv(sa))
+    gain = cov @ K.T @ se_inv
+    avk = gain @ K
+    tol = dict(rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(ana["cov"], cov, **tol)
+    d = np.sqrt(np.diag(cov))
+    np.testing.assert_allclose(ana["corr"], cov / d[:, None] / d[None, :], **tol)
+    np.testing.assert_allclose(ana["gain"], gain, **tol)
+    np.testing.assert_allclose(ana["avk"], avk, **tol)
+    np.testing.assert_allclose(ana["err_total"], d, **tol)
+    np.testing.assert_allclose(ana["err_noise"], np.sqrt(np.diag(gain @ np.diag(sn ** 2) @ gain.T)), **tol)
+    np.testing.assert_allclose(ana["err_formod"], np.sqrt(np.diag(gain @ np.diag(sf ** 2) @ gain.T)), **tol)
+    np.testing.assert_allclose(ana["cont"], avk.sum(axis=1), **tol)
+    np.testing.assert_allclose(ana["res"], 1 / np.diag(avk), **tol)
+    np.testing.assert_allclose(ana["avk"], np.eye(n) - cov @ np.linalg.inv(sa), rtol=0, atol=1e-9)
