@@ -58,14 +58,14 @@ struct TabView {
 // so that the 64 lanes of a wavefront read one field of one point with a
 // single coalesced 512-byte access, and all fields of a point are adjacent.
 // ------------------------------------------------------------------
+// Longitudes and latitudes of the points are used only by the ray tracer
+// itself (tangent point, solar geometry), which computes them where needed.
 enum {
   F_Z = 0,
-  F_LON = 1,
-  F_LAT = 2,
-  F_P = 3,
-  F_T = 4,
-  F_DS = 5,   // step length: raw while tracing, trapezoid rule afterwards
-  F_Q0 = 6    // then q[ng], k[nw], cloud factor (if clouds), u[ng]
+  F_P = 1,
+  F_T = 2,
+  F_DS = 3,   // step length: raw while tracing, trapezoid rule afterwards
+  F_Q0 = 4    // then q[ng], k[nw], cloud factor (if clouds), u[ng]
 };
 
 struct LosView {

@@ -26,19 +26,26 @@ __device__ __forceinline__ int atm_level(const atm_t *atm, int zdir, double z, i
 // the ray stays between the same levels gives bit-identical values.
 struct LevelCache {
   int ip = -1;
-  double z0, p0, pq, t0, tq;
+  double z0, z1, p0, pq, t0, tq;
   bool plog;
 };
 
 __device__ __forceinline__ void intpol_pt(const atm_t *atm, int zdir, int &g, LevelCache &lc,
                                           double z, double *p, double *t) {
   JUR_FP_EXACT
-  const int ip = atm_level(atm, zdir, z, g);
+  // the walk from g returns g when z is still between the cached levels
+  bool stay = false;
+  if (lc.ip >= 0 && g == lc.ip) {
+    if (zdir > 0) stay = (g == 0 || !(lc.z0 > z)) && (g == atm->np - 2 || lc.z1 > z);
+    else if (zdir < 0) stay = (g == 0 || !(lc.z0 <= z)) && (g == atm->np - 2 || lc.z1 <= z);
+  }
+  const int ip = stay ? g : atm_level(atm, zdir, z, g);
   if (ip != lc.ip) {
     lc.ip = ip;
     const double x0 = atm->z[ip], x1 = atm->z[ip + 1];
     const double p0 = atm->p[ip], p1 = atm->p[ip + 1];
     lc.z0 = x0;
+    lc.z1 = x1;
     lc.p0 = p0;
     lc.plog = (p1 / p0 > 0);
     lc.pq = lc.plog ? log(p1 / p0) / (x1 - x0) : (p1 - p0) / (x1 - x0);
@@ -128,6 +135,13 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   const int zdir = zasc ? 1 : (zdesc ? -1 : 0);
   int zg = -1;  // level of the previous LOS point
   LevelCache lc;
+  // level pair of intpol_atm: [0] z0 [1] z1 [2] p0 [3] p quotient [4] t0
+  // [5] t slope [6..] q0[ng], q slopes[ng], k0[nw], k slopes[nw]; field-major,
+  // lane fastest
+  extern __shared__ double lvl[];
+#define LV(f) lvl[(f) * WAVE + lane]
+  int lip = -1;
+  bool lplog = false;
   if (M.nsf > 0) {
     zmin = (atm->sfz > zmin) ? atm->sfz : zmin;
     if (atm->sfp > 0) {
@@ -161,13 +175,20 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     while (fabs(dmin - dmax) > 0.001) {
       const double d = (dmax + dmin) / 2;
       for (int i = 0; i < 3; i++) x[i] = xobs[i] + d * ex0[i];
-      cart2geo(x, &z, &lon, &lat);
+      z = cart2z(x);
       if (z <= zmax && z > zmax - 0.001) break;
       if (z < zmax - 0.0005) dmax = d; else dmin = d;
     }
   }
 
-  double zprev = 0, lonprev = 0, latprev = 0;
+  // Cartesian positions of the points whose longitude and latitude are
+  // needed after the march: the first and the last point, and the two
+  // neighbours of the first minimum of z (tangent_point); cart2geo of the
+  // same x gives the values the reference stores per point.
+  double zprev = 0, xprev[3] = {0, 0, 0}, xfirst[3] = {0, 0, 0}, xlast[3] = {0, 0, 0};
+  double xbefore[3] = {0, 0, 0}, xafter[3] = {0, 0, 0}, zlow = 0;
+  int imin = 0;
+  bool after_pending = false, nan_seen = false;
   for (;;) {
     double ds = M.rayds;
     if (M.raydz > 0) {
@@ -177,7 +198,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       if (cosa != 0) ds = (M.rayds < M.raydz / cosa) ? M.rayds : M.raydz / cosa;
     }
 
-    cart2geo(x, &z, &lon, &lat);
+    z = cart2z(x);
 
     if (z < zmin || z > zmax) {
       if (np == 0) {
@@ -188,19 +209,54 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       }
       stop = (z < zmin ? 2 : 1);
       const double frac = ((z < zmin ? zmin : zmax) - zprev) / (z - zprev);
-      geo2cart(zprev, lonprev, latprev, xh);
+      double zd;
+      cart2geo(xprev, &zd, &lon, &lat);
+      geo2cart(zprev, lon, lat, xh);
       for (int i = 0; i < 3; i++) x[i] = xh[i] + frac * (x[i] - xh[i]);
-      cart2geo(x, &z, &lon, &lat);
+      z = cart2z(x);
       // los->ds[np-1] = ds * frac (the previous point was already written)
       LOSW(F_DS, np - 1) = ds * frac;
       ds = 0;
     }
 
-    // intpol_atm at the LOS point
-    const int ipa = atm_level(atm, zdir, z, zg);
-    const double z0 = atm->z[ipa], z1 = atm->z[ipa + 1];
-    p = logy(z0, atm->p[ipa], z1, atm->p[ipa + 1], z);
-    t = lin(z0, atm->t[ipa], z1, atm->t[ipa + 1], z);
+    // intpol_atm at the LOS point.  The level pair and the quotients of
+    // LOGY (p) and LIN (t, q, k) are kept in LDS while the point stays
+    // between the same two levels (the level walk would return the same
+    // index), so that a step loads nothing from the atmosphere.
+    {
+      bool stay = false;
+      if (lip >= 0) {
+        const double za = LV(0), zb = LV(1);
+        if (zdir > 0) stay = (lip == 0 || !(za > z)) && (lip == atm->np - 2 || zb > z);
+        else if (zdir < 0) stay = (lip == 0 || !(za <= z)) && (lip == atm->np - 2 || zb <= z);
+      }
+      if (!stay) {
+        const int ipa = atm_level(atm, zdir, z, zg);
+        if (ipa != lip) {
+          lip = ipa;
+          const double x0 = atm->z[ipa], x1 = atm->z[ipa + 1];
+          const double y0 = atm->p[ipa], y1 = atm->p[ipa + 1];
+          LV(0) = x0;
+          LV(1) = x1;
+          lplog = (y1 / y0 > 0);
+          LV(2) = y0;
+          LV(3) = lplog ? log(y1 / y0) / (x1 - x0) : (y1 - y0) / (x1 - x0);
+          LV(4) = atm->t[ipa];
+          LV(5) = (atm->t[ipa + 1] - atm->t[ipa]) / (x1 - x0);
+          for (int ig = 0; ig < M.ng; ig++) {
+            LV(6 + ig) = atm->q[ig][ipa];
+            LV(6 + M.ng + ig) = (atm->q[ig][ipa + 1] - atm->q[ig][ipa]) / (x1 - x0);
+          }
+          for (int iw = 0; iw < M.nw; iw++) {
+            LV(6 + 2 * M.ng + iw) = atm->k[iw][ipa];
+            LV(6 + 2 * M.ng + M.nw + iw) = (atm->k[iw][ipa + 1] - atm->k[iw][ipa]) / (x1 - x0);
+          }
+        }
+      }
+    }
+    const double dzl = z - LV(0);
+    p = lplog ? LV(2) * exp(LV(3) * dzl) : LV(2) + LV(3) * dzl;
+    t = LV(4) + LV(5) * dzl;
 
     if (np >= L.cap) {
       // capacity exceeded: the reference limit NLOS -> error, a smaller
@@ -210,14 +266,11 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       return;
     }
     LOSW(F_Z, np) = z;
-    LOSW(F_LON, np) = lon;
-    LOSW(F_LAT, np) = lat;
     LOSW(F_P, np) = p;
     LOSW(F_T, np) = t;
-    for (int ig = 0; ig < M.ng; ig++)
-      LOSW(F_Q0 + ig, np) = lin(z0, atm->q[ig][ipa], z1, atm->q[ig][ipa + 1], z);
+    for (int ig = 0; ig < M.ng; ig++) LOSW(F_Q0 + ig, np) = LV(6 + ig) + LV(6 + M.ng + ig) * dzl;
     for (int iw = 0; iw < M.nw; iw++)
-      LOSW(L.f_k + iw, np) = lin(z0, atm->k[iw][ipa], z1, atm->k[iw][ipa + 1], z);
+      LOSW(L.f_k + iw, np) = LV(6 + 2 * M.ng + iw) + LV(6 + 2 * M.ng + M.nw + iw) * dzl;
     LOSW(F_DS, np) = ds;
     if (L.f_cl >= 0) {
       const double aux = (M.ncl > 0 && atm->cldz > 0)
@@ -225,9 +278,29 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
                              : 0.0;
       LOSW(L.f_cl, np) = aux;
     }
+    // first minimum of z (gsl_stats_min_index semantics, a NaN ends the
+    // search), with the positions of its neighbours
+    if (!nan_seen) {
+      const bool lower = (np == 0) || z < zlow;
+      if (lower) zlow = z;
+      if (lower || isnan(z)) {
+        imin = np;
+        if (np > 0)
+          for (int i = 0; i < 3; i++) xbefore[i] = xprev[i];
+        after_pending = true;
+        nan_seen = isnan(z);
+      } else if (after_pending) {
+        for (int i = 0; i < 3; i++) xafter[i] = x[i];
+        after_pending = false;
+      }
+    } else if (after_pending) {
+      for (int i = 0; i < 3; i++) xafter[i] = x[i];
+      after_pending = false;
+    }
+    if (np == 0)
+      for (int i = 0; i < 3; i++) xfirst[i] = x[i];
+    for (int i = 0; i < 3; i++) xlast[i] = xprev[i] = x[i];
     zprev = z;
-    lonprev = lon;
-    latprev = lat;
     np++;
 
     if (stop) {
@@ -268,20 +341,13 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     return;
   }
 
-  // tangent_point: first minimum altitude (gsl_stats_min_index semantics)
-  int imin = 0;
-  {
-    double m = LOSW(F_Z, 0);
-    for (int i = 0; i < np; i++) {
-      const double v = LOSW(F_Z, i);
-      if (v < m) { m = v; imin = i; }
-      if (isnan(v)) { imin = i; break; }
-    }
-  }
+  // tangent_point at the first minimum imin found during the march
+  double zl, lonl, latl;
+  cart2geo(xlast, &zl, &lonl, &latl);
   if (imin <= 0 || imin >= np - 1) {
     R.tpz = LOSW(F_Z, np - 1);
-    R.tplon = LOSW(F_LON, np - 1);
-    R.tplat = LOSW(F_LAT, np - 1);
+    R.tplon = lonl;
+    R.tplat = latl;
   } else {
     const double yy0 = LOSW(F_Z, imin - 1);
     const double yy1 = LOSW(F_Z, imin);
@@ -295,20 +361,24 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     const double xt = -b / (2 * a);
     double v[3], v0[3], v2[3], dummy;
     R.tpz = a * xt * xt + b * xt + c;
-    geo2cart(yy0, LOSW(F_LON, imin - 1), LOSW(F_LAT, imin - 1), v0);
-    geo2cart(yy2, LOSW(F_LON, imin + 1), LOSW(F_LAT, imin + 1), v2);
+    double lon0, lat0, lon2, lat2;
+    cart2geo(xbefore, &dummy, &lon0, &lat0);
+    cart2geo(xafter, &dummy, &lon2, &lat2);
+    geo2cart(yy0, lon0, lat0, v0);
+    geo2cart(yy2, lon2, lat2, v2);
     for (int i = 0; i < 3; i++) v[i] = lin(0.0, v0[i], x2, v2[i], xt);
     cart2geo(v, &dummy, &R.tplon, &R.tplat);
   }
 
   // solar incidence (formod_pencil, src/jurassic.c:3297-3326)
   if (M.sftype >= 3 && R.sft > 0) {
-    const double lonl = LOSW(F_LON, np - 1), latl = LOSW(F_LAT, np - 1);
     const double sza2 = (M.sfsza < 0) ? sza(otime, lonl, latl) : M.sfsza;
     if (sza2 < 89.999) {
       double x0[3], x1[3];
       geo2cart(LOSW(F_Z, np - 1), lonl, latl, x0);
-      geo2cart(LOSW(F_Z, 0), LOSW(F_LON, 0), LOSW(F_LAT, 0), x1);
+      double lonf, latf, zf;
+      cart2geo(xfirst, &zf, &lonf, &latf);
+      geo2cart(LOSW(F_Z, 0), lonf, latf, x1);
       for (int i = 0; i < 3; i++) x1[i] -= x0[i];
       const double cosa = (x0[0] * x1[0] + x0[1] * x1[1] + x0[2] * x1[2]) /
                           sqrt(x0[0] * x0[0] + x0[1] * x0[1] + x0[2] * x0[2]) /
@@ -368,7 +438,8 @@ __global__ __launch_bounds__(WAVE) void k_hydrostatic(atm_t *atm, int nprof, dou
 
 void launch_raytrace(const ModelView &M, const BatchView &B, const LosView &L, RayInfo *info,
                      int blk0, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_raytrace, dim3(nblk), dim3(WAVE), 0, s, M, B, L, info, blk0);
+  const size_t lds = (size_t)(6 + 2 * M.ng + 2 * M.nw) * WAVE * sizeof(double);
+  hipLaunchKernelGGL(k_raytrace, dim3(nblk), dim3(WAVE), lds, s, M, B, L, info, blk0);
 }
 
 void launch_hydrostatic(atm_t *atm, int nprof, double hydz, int ig_h2o, hipStream_t s) {
