@@ -113,17 +113,12 @@ __device__ bool chol_wg(double *a, int n, int *bad) {
     const double l = a[(size_t)j * n + j];
     for (int i = j + 1 + threadIdx.x; i < n; i += LM_T) a[(size_t)i * n + j] /= l;
     __syncthreads();
-    // trailing update of the lower triangle, rows i > j, columns j < k <= i
-    const int w = n - j - 1;
-    const long tot = (long)w * (w + 1) / 2;
-    for (long e = threadIdx.x; e < tot; e += LM_T) {
-      // e -> (ii, kk) with 0 <= kk <= ii < w
-      int ii = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
-      while ((long)ii * (ii + 1) / 2 > e) ii--;
-      while ((long)(ii + 1) * (ii + 2) / 2 <= e) ii++;
-      const int kk = (int)(e - (long)ii * (ii + 1) / 2);
-      const int i = j + 1 + ii, k = j + 1 + kk;
-      a[(size_t)i * n + k] -= a[(size_t)i * n + j] * a[(size_t)k * n + j];
+    // trailing update of the lower triangle, rows i > j, columns j < k <= i,
+    // on a 16 x 16 thread tiling (each element once per step)
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    for (int i = j + 1 + ty; i < n; i += 16) {
+      const double lij = a[(size_t)i * n + j];
+      for (int k = j + 1 + tx; k <= i; k += 16) a[(size_t)i * n + k] -= lij * a[(size_t)k * n + j];
     }
     __syncthreads();
   }
