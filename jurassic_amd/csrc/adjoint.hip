@@ -448,6 +448,83 @@ __device__ __forceinline__ double beta_grad(const ModelView &M, const CtmChan &K
   return beta;
 }
 
+// Per-lane constants of beta_grad_f.
+struct CtmChanF {
+  float h2o_l2r;    // log2(cw260 / cw296) / 36: S = slf 2^(l2r (296 - T))
+  double h2o_dlr;   // d ln S / dT = -ln(cw260 / cw296) / 36
+  double bq;        // 0.7193876 nu
+};
+
+__device__ __forceinline__ CtmChanF ctm_chan_f(const CtmChan &K, double nu) {
+  CtmChanF f;
+  f.h2o_l2r = K.h2o_ratio > 0 ? (float)(log2(K.h2o_ratio) / 36.0) : 0.f;
+  f.h2o_dlr = K.h2o_ratio > 0 ? log(K.h2o_ratio) * (-1.0 / 36) : 0.0;
+  f.bq = .7193876 * nu;
+  return f;
+}
+
+// beta_grad for the Jacobian of the FP32 forward model, whose continua are
+// evaluated in float (radiance_f32.hip): the transcendentals (the H2O
+// self-continuum power, tanh, the N2/O2 exponentials) in float, and the
+// divisions by T, p and ds as products with the reciprocals rt, rp, rds;
+// the products and sums stay FP64 (the column densities and the continuum
+// coefficients span ranges beyond float).
+__device__ __forceinline__ double beta_grad_f(const ModelView &M, const CtmChan &K, const CtmChanF &KF,
+                                              double nu, double p, double t, double rt, double rp, double rds,
+                                              double q_h2o, double u_co2, double u_h2o, double kwin, double clx,
+                                              BetaGrad &g) {
+  g.dp = g.dt = g.du_co2 = g.du_h2o = g.dq_h2o = 0;
+  double beta = kwin + clx;
+  if (M.ig_co2 >= 0 && K.co2) {
+    const double dt230 = t - 230, dt260 = t - 260, dt296 = t - 296;
+    const double ctw = dt260 * 5.050505e-4 * dt296 * K.co2_230 - dt230 * 9.259259e-4 * dt296 * K.co2_260 +
+                       dt230 * 4.208754e-4 * dt260 * K.co2_296;
+    const double dctw = 5.050505e-4 * K.co2_230 * (dt260 + dt296) -
+                        9.259259e-4 * K.co2_260 * (dt230 + dt296) +
+                        4.208754e-4 * K.co2_296 * (dt230 + dt260);
+    const double f = (1 / JUR_NA / 1000 / JUR_P0) * rds;
+    beta += u_co2 * p * ctw * f;
+    g.du_co2 = p * ctw * f;
+    g.dp += u_co2 * ctw * f;
+    g.dt += u_co2 * p * dctw * f;
+  }
+  if (M.ig_h2o >= 0 && K.h2o) {
+    const double S = K.h2o_slf * (double)exp2f(KF.h2o_l2r * (float)(296 - t));
+    const double dS = S * KF.h2o_dlr;
+    const double bq = KF.bq;
+    const double th = (double)tanhf((float)(bq * rt));
+    const double a1u = nu * th;
+    const double da1u = nu * (1 - th * th) * (-bq * rt * rt);
+    const double a2 = 296 * rt, da2 = -296 * rt * rt;
+    const double mix = q_h2o * S + (1 - q_h2o) * K.h2o_frn;
+    const double a3 = p / JUR_P0 * mix * 1e-20;
+    const double da3 = p / JUR_P0 * q_h2o * dS * 1e-20;
+    beta += a1u * u_h2o * a2 * a3 * rds;
+    g.du_h2o = a1u * a2 * a3 * rds;
+    g.dq_h2o = a1u * u_h2o * a2 * (p / JUR_P0 * (S - K.h2o_frn) * 1e-20) * rds;
+    g.dp += a1u * u_h2o * a2 * (mix / JUR_P0 * 1e-20) * rds;
+    g.dt += u_h2o * (da1u * a2 * a3 + a1u * da2 * a3 + a1u * a2 * da3) * rds;
+  }
+  if (M.ctm_n2 && K.n2) {
+    const double qn = 0.79, r = p / JUR_P0 * 273.0 * rt;
+    const double ex = (double)__expf((float)(K.n2_beta * (1 / 296.0 - rt)));
+    const double F = qn + (1 - qn) * (1.294 - 0.4545 * t / 296.0);
+    const double base = 0.1 * r * r * ex * qn * K.n2_b;
+    const double c = base * F;
+    beta += c;
+    g.dp += 2 * c * rp;
+    g.dt += c * (-2 * rt + K.n2_beta * rt * rt) + base * (1 - qn) * (-0.4545 / 296.0);
+  }
+  if (M.ctm_o2 && K.o2) {
+    const double r = p / JUR_P0 * 273.0 * rt;
+    const double c = 0.1 * r * r * (double)__expf((float)(K.o2_beta * (1 / 296.0 - rt))) * 0.21 * K.o2_b;
+    beta += c;
+    g.dp += 2 * c * rp;
+    g.dt += c * (-2 * rt + K.o2_beta * rt * rt);
+  }
+  return beta;
+}
+
 // formod_srcfunc value and its slope in T.
 __device__ __forceinline__ double srcfunc_d(const double *st, const double *sr_col, double t, double &d) {
   const int it = locate_reg(st, TBLNS, t);
@@ -539,8 +616,9 @@ struct LevelWindow {
 // rad, tau, H_i (scratch), the number of points before the first opaque
 // segment, and the K entries of continua, source function, window
 // extinction, clouds and surface.
+// The FP32 variant is held to 3 waves per SIMD (166 VGPRs).
 template <typename SF>
-__global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(sizeof(SF) == sizeof(float) ? 3 : 1, 8))) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
                                                   const RayInfo *info, int blk0, int nblk, JacView J,
                                                   double *rad_out, double *tau_out) {
   const int nd = M.nd, ng = M.ng;
@@ -592,6 +670,31 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
   }
   const bool has_k = C.window < M.nw;
   const CtmChan K = ctm_chan(M.ctm, nu);
+  // The Jacobian of the FP32 forward model (float continua and source
+  // function interpolation) takes the continua's transcendentals and the
+  // segment transmittance's exponential in float; the sweeps stay FP64.
+  constexpr bool F32 = sizeof(SF) == sizeof(float);
+  const CtmChanF KF = ctm_chan_f(K, nu);
+  // Segment terms of point (p, t, ds): beta and its gradient, the source
+  // function and its slope, and s = tg exp(-beta ds); rt = 1/t, rp = 1/p.
+  auto segment = [&](double p, double t, double ds, double q_h2o, double u_co2, double u_h2o, double kwin,
+                     double clx, double tg, BetaGrad &bg, double &src, double &dsrc, double &rt,
+                     double &rp) {
+    double beta;
+    if constexpr (F32) {
+      rt = 1 / t;
+      rp = 1 / p;
+      beta = beta_grad_f(M, K, KF, nu, p, t, rt, rp, 1 / ds, q_h2o, u_co2, u_h2o, kwin, clx, bg);
+      src = srcfunc_d(M.st, sr_col, t, dsrc);
+      return tg * (double)__expf((float)(-beta * ds));
+    } else {
+      rt = 1 / t;
+      rp = 1 / p;
+      beta = beta_grad(M, K, nu, p, t, ds, q_h2o, u_co2, u_h2o, kwin, clx, bg);
+      src = srcfunc_d(M.st, sr_col, t, dsrc);
+      return tg * exp(-beta * ds);
+    }
+  };
   const int fq_h2o = M.ig_h2o >= 0 ? F_Q0 + M.ig_h2o : -1;
   const int fu_h2o = M.ig_h2o >= 0 ? L.f_u + M.ig_h2o : -1;
   const int fu_co2 = M.ig_co2 >= 0 ? L.f_u + M.ig_co2 : -1;
@@ -611,14 +714,10 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       qp = q;
       const double p = LOSR(F_P, ip), t = LOSR(F_T, ip), ds = LOSR(F_DS, ip);
       BetaGrad bg;
-      const double beta = beta_grad(M, K, nu, p, t, ds, fq_h2o >= 0 ? LOSR(fq_h2o, ip) : 0.0,
-                                    fu_co2 >= 0 ? LOSR(fu_co2, ip) : 0.0,
-                                    fu_h2o >= 0 ? LOSR(fu_h2o, ip) : 0.0,
-                                    has_k ? LOSR(L.f_k + C.window, ip) : 0.0,
-                                    cloud ? LOSR(L.f_cl, ip) * kcl : 0.0, bg);
-      double dsrc;
-      const double src = srcfunc_d(M.st, sr_col, t, dsrc);
-      const double s = tg * exp(-beta * ds);
+      double src, dsrc, rt, rp;
+      const double s = segment(p, t, ds, fq_h2o >= 0 ? LOSR(fq_h2o, ip) : 0.0, fu_co2 >= 0 ? LOSR(fu_co2, ip) : 0.0,
+                               fu_h2o >= 0 ? LOSR(fu_h2o, ip) : 0.0, has_k ? LOSR(L.f_k + C.window, ip) : 0.0,
+                               cloud ? LOSR(L.f_cl, ip) * kcl : 0.0, tg, bg, src, dsrc, rt, rp);
       SCR(0, ip) = tau;
       if (J.nscr > 1) SCR(1, ip) = dr;
       rad += src * (1 - s) * tau;
@@ -675,11 +774,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       const double u_h2o = fu_h2o >= 0 ? LOSR(fu_h2o, ip) : 0.0;
       const double aux = cloud ? LOSR(L.f_cl, ip) : 0.0;
       BetaGrad bg;
-      const double beta = beta_grad(M, K, nu, p, t, ds, q_h2o, u_co2, u_h2o,
-                                    has_k ? LOSR(L.f_k + C.window, ip) : 0.0, aux * kcl, bg);
-      double dsrc;
-      const double src = srcfunc_d(M.st, sr_col, t, dsrc);
-      const double s = tg * exp(-beta * ds);
+      double src, dsrc, rt, rp;
+      const double s = segment(p, t, ds, q_h2o, u_co2, u_h2o, has_k ? LOSR(L.f_k + C.window, ip) : 0.0,
+                               aux * kcl, tg, bg, src, dsrc, rt, rp);
       const double tau_i = SCR(0, ip);
       const double dr_i = J.nscr > 1 ? SCR(1, ip) : 0.0;
       const double G = tau_i * (tb - src) + db * (dr_i - src);
@@ -691,11 +788,12 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       const double beta_bar = -H * ds;
 
       // u = 10 q p / (kB T) ds
-      const double ucoef = 10 * p / (JUR_KB * t) * ds;
+      const double ucoef = F32 ? 10 * p * rt * ds / JUR_KB : 10 * p / (JUR_KB * t) * ds;
       const double uc_bar = beta_bar * bg.du_co2, uh_bar = beta_bar * bg.du_h2o;
+      const double uu = uc_bar * u_co2 + uh_bar * u_h2o;
       double v[5];
-      v[0] = beta_bar * bg.dt + src_bar * dsrc - (uc_bar * u_co2 + uh_bar * u_h2o) / t;  // T
-      v[1] = beta_bar * bg.dp + (uc_bar * u_co2 + uh_bar * u_h2o) / p;                  // p
+      v[0] = beta_bar * bg.dt + src_bar * dsrc - (F32 ? uu * rt : uu / t);  // T
+      v[1] = beta_bar * bg.dp + (F32 ? uu * rp : uu / p);                   // p
       v[2] = beta_bar;                                                                  // k
       v[3] = uc_bar * ucoef;                                                            // q_co2
       v[4] = uh_bar * ucoef + beta_bar * bg.dq_h2o;                                     // q_h2o
@@ -807,17 +905,29 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
       const TabView &T = M.tab;
       const double *pg = T.p + (size_t)tix * T.TP;
       const int *ntp = T.nt + (size_t)tix * T.TP;
+      // wave-uniform bases with 32-bit per-lane offsets (as in k_lookup_f32)
+      const size_t tcols = (size_t)T.TP * T.TT;
+      const double *tgb = T.t + (size_t)tix * tcols;
+      const int *nub = T.nu + (size_t)tix * tcols;
+      const float4 *recb = T.rec + (size_t)tix * tcols * T.TU;
+      const double *losb = L.los + (size_t)bl * L.cap * L.nfield * WAVE;
+      const unsigned nfield = L.nfield;
+      const float *epsg = (const float *)E.eps + E.off[bl] + ((size_t)id * ng + g) * npb * WAVE;
+      const double *scb = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE;
+      const unsigned nscr = J.nscr;
+#define LOSQ(f, ip) ld32(losb, ((unsigned)(ip) * nfield + (unsigned)(f)) * WAVE + lane)
+#define EPSQ(ip) ld32(epsg, (unsigned)(ip) * WAVE + lane)
       int it0 = -1, it1 = -1, nt0 = 0, nt1 = 0, n00 = 0, n01 = 0, n10 = 0, n11 = 0;
       double pa = 0, pb = 0;
       float wp_scale = 0, g00 = 0, gi0 = 0, g10 = 0, gi1 = 0, ta0 = 0, dt0 = 0, ta1 = 0, dt1 = 0;
-      const double *tg0 = nullptr;
+      unsigned tq0 = 0;  // first T-grid entry of p row ipr
       double p = 0, t = 0, useg = 0;
       float epf = 0.f;
       if (nef > 0) {
-        p = LOSR(F_P, nef - 1);
-        t = LOSR(F_T, nef - 1);
-        useg = LOSR(fu, nef - 1);
-        epf = nef > 1 ? (float)EPS_AT(g, nef - 2) : 0.f;
+        p = LOSQ(F_P, nef - 1);
+        t = LOSQ(F_T, nef - 1);
+        useg = LOSQ(fu, nef - 1);
+        epf = nef > 1 ? EPSQ(nef - 2) : 0.f;
       }
       for (int ip = nef - 1; ip >= 0; ip--) {
         const int ipn = ip > 0 ? ip - 1 : 0;
@@ -850,11 +960,13 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
           wp_scale = frcp((float)(pb - pa));
           nt0 = ntp[ipr];
           nt1 = ntp[ipr + 1];
-          tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
-          g00 = (float)tg0[0];
-          gi0 = frcp((float)(tg0[1] - tg0[0]));
-          g10 = (float)tg0[T.TT];
-          gi1 = frcp((float)(tg0[T.TT + 1] - tg0[T.TT]));
+          tq0 = (unsigned)ipr * T.TT;
+          const double t00 = ld32(tgb, tq0), t01 = ld32(tgb, tq0 + 1);
+          const double t10 = ld32(tgb, tq0 + T.TT), t11 = ld32(tgb, tq0 + T.TT + 1);
+          g00 = (float)t00;
+          gi0 = frcp((float)(t01 - t00));
+          g10 = (float)t10;
+          gi1 = frcp((float)(t11 - t10));
           it0 = it1 = -1;
         }
         LkPart d{1, 0, 0, 0};
@@ -866,7 +978,6 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
           j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
           int j1 = (int)((tq - g10) * gi1);
           j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
-          const size_t c0 = (size_t)tix * T.TP + ipr;
           if (j0 != it0) {
             if (j0 == it0 + 1) {
               const int a0 = gi[0];
@@ -878,10 +989,11 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
               gi[0] = 2 * gi[0] - a1;
             }
             it0 = j0;
-            ta0 = (float)tg0[it0];
-            dt0 = frcp((float)(tg0[it0 + 1] - tg0[it0]));
-            n00 = T.nu[c0 * T.TT + it0];
-            n01 = T.nu[c0 * T.TT + it0 + 1];
+            const double a = ld32(tgb, tq0 + it0), b = ld32(tgb, tq0 + it0 + 1);
+            ta0 = (float)a;
+            dt0 = frcp((float)(b - a));
+            n00 = ld32(nub, tq0 + it0);
+            n01 = ld32(nub, tq0 + it0 + 1);
           }
           if (j1 != it1) {
             if (j1 == it1 + 1) {
@@ -894,17 +1006,19 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
               gi[2] = 2 * gi[2] - a3;
             }
             it1 = j1;
-            ta1 = (float)tg0[T.TT + it1];
-            dt1 = frcp((float)(tg0[T.TT + it1 + 1] - tg0[T.TT + it1]));
-            n10 = T.nu[(c0 + 1) * T.TT + it1];
-            n11 = T.nu[(c0 + 1) * T.TT + it1 + 1];
+            const double a = ld32(tgb, tq0 + T.TT + it1), b = ld32(tgb, tq0 + T.TT + it1 + 1);
+            ta1 = (float)a;
+            dt1 = frcp((float)(b - a));
+            n10 = ld32(nub, tq0 + T.TT + it1);
+            n11 = ld32(nub, tq0 + T.TT + it1 + 1);
           }
           if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
-            const float4 *kc[4];
-            kc[0] = T.rec + (c0 * T.TT + it0) * T.TU;
-            kc[1] = kc[0] + T.TU;
-            kc[2] = T.rec + ((c0 + 1) * T.TT + it1) * T.TU;
-            kc[3] = kc[2] + T.TU;
+            unsigned ko[4];
+            ko[0] = (tq0 + it0) * T.TU;
+            ko[1] = ko[0] + T.TU;
+            ko[2] = (tq0 + T.TT + it1) * T.TU;
+            ko[3] = ko[2] + T.TU;
+            const float4 *kc[4] = {recb + ko[0], recb + ko[1], recb + ko[2], recb + ko[3]};
             const int nc[4] = {n00, n01, n10, n11};
             float4 r[4], rn[4];
             int gs[4];
@@ -913,14 +1027,14 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
               int g0 = gi[c] - 1;
               g0 = g0 < 0 ? 0 : (g0 > nc[c] - 2 ? nc[c] - 2 : g0);
               gs[c] = g0;
-              r[c] = kc[c][g0];
-              rn[c] = kc[c][g0 + 1 < nc[c] - 2 ? g0 + 1 : nc[c] - 2];
+              r[c] = ld32(recb, ko[c] + g0);
+              rn[c] = ld32(recb, ko[c] + (g0 + 1 < nc[c] - 2 ? g0 + 1 : nc[c] - 2));
             }
             __builtin_amdgcn_sched_barrier(0);
-            pn = LOSR(F_P, ipn);
-            tn = LOSR(F_T, ipn);
-            un = LOSR(fu, ipn);
-            en_f = ipn > 0 ? (float)EPS_AT(g, ipn - 1) : 0.f;
+            pn = LOSQ(F_P, ipn);
+            tn = LOSQ(F_T, ipn);
+            un = LOSQ(fu, ipn);
+            en_f = ipn > 0 ? EPSQ(ipn - 1) : 0.f;
             loaded = true;
             __builtin_amdgcn_sched_barrier(0);
             const float us = (float)useg;
@@ -962,21 +1076,24 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
           }
         }
         if (!loaded) {
-          pn = LOSR(F_P, ipn);
-          tn = LOSR(F_T, ipn);
-          un = LOSR(fu, ipn);
-          en_f = ipn > 0 ? (float)EPS_AT(g, ipn - 1) : 0.f;
+          pn = LOSQ(F_P, ipn);
+          tn = LOSQ(F_T, ipn);
+          un = LOSQ(fu, ipn);
+          en_f = ipn > 0 ? EPSQ(ipn - 1) : 0.f;
         }
-        const double H = SCR(0, ip);
-        const double z = LOSR(F_Z, ip), ds = LOSR(F_DS, ip);
+        const double H = ld32(scb, (unsigned)ip * nscr * WAVE + lane);
+        const double z = LOSQ(F_Z, ip), ds = LOSQ(F_DS, ip);
         const double trp = 1.0 - (double)epf;
-        const double lam = (Hn - H) / trc + carry;
+        // the chain rule's divisions by the transmittance, T and p as float
+        // reciprocals (relative error 1e-7, far below the FP32 lookups')
+        const double rt = (double)frcp((float)t), rp = (double)frcp((float)p);
+        const double lam = (Hn - H) * (double)frcp((float)trc) + carry;
         const double u_bar = lam * d.du, p_bar = lam * d.dp, t_bar = lam * d.dt;
         carry = lam * d.dep;
-        const double ucoef = 10 * p / (JUR_KB * t) * ds;
+        const double ucoef = 10 * p * rt * ds * (1 / JUR_KB);
         double v[3];
-        v[0] = t_bar - u_bar * useg / t;
-        v[1] = p_bar + u_bar * useg / p;
+        v[0] = t_bar - u_bar * useg * rt;
+        v[1] = p_bar + u_bar * useg * rp;
         v[2] = u_bar * ucoef;
         W.add(atm, z, lev, v, 1, p, xm, qi, kt);
         Hn = H;
@@ -987,6 +1104,8 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
         epf = en_f;
       }
       W.flush(xm, qi, kt);
+#undef LOSQ
+#undef EPSQ
       return;
     }
   } else {

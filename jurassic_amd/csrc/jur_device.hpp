@@ -250,6 +250,19 @@ __device__ __forceinline__ int col_walk_f(const float4 *col, int n, float x, int
 // Fast float reciprocal and division for the FP32 path (v_rcp_f32, 1 ulp):
 // the default HIP float division here is IEEE-correct, which the FP64
 // path's float slopes need, but the FP32 path does not.
+// base[i] with a 32-bit element index: a wave-uniform base plus a per-lane
+// 32-bit byte offset, which the compiler addresses as SGPR base + VGPR offset
+// without 64-bit arithmetic per lane.  For arrays far below 4 GB (one
+// table, one ray block's LOS or emissivities).
+template <typename V>
+__device__ __forceinline__ V ld32(const V *base, unsigned i) {
+  return *(const V *)((const char *)base + (size_t)(i * (unsigned)sizeof(V)));
+}
+template <typename V>
+__device__ __forceinline__ void st32(V *base, unsigned i, V v) {
+  *(V *)((char *)base + (size_t)(i * (unsigned)sizeof(V))) = v;
+}
+
 __device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 

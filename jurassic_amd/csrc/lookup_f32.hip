@@ -21,7 +21,8 @@
 
 namespace jur {
 
-#define LOSR(f, ip) LOS_AT(los, L, f, ip)
+// LOS field f of point ip: wave-uniform block base, 32-bit lane offset
+#define LOSR(f, ip) ld32(losb, ((unsigned)(ip) * nfield + (unsigned)(f)) * WAVE + lane)
 
 template <bool CGA>
 __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, LosView L,
@@ -42,13 +43,20 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
   const RayInfo R = info[ir];
   if (R.status != JUR_OK) return;
 
-  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  // wave-uniform bases (the ray block's LOS and emissivity chain, the
+  // table's arrays) with 32-bit per-lane offsets
+  const double *losb = L.los + (size_t)bl * L.cap * L.nfield * WAVE;
+  const unsigned nfield = L.nfield;
   const int npb = E.npb[bl];
-  float *out = (float *)E.eps + E.off[bl] + ((size_t)id * ng + ig) * npb * WAVE + lane;
+  float *outb = (float *)E.eps + E.off[bl] + ((size_t)id * ng + ig) * npb * WAVE;
   const int mono = T.mono[tix];
   const int pdir = T.pdir[tix];
   const double *pg = T.p + (size_t)tix * T.TP;
   const int *ntp = T.nt + (size_t)tix * T.TP;
+  const size_t tcols = (size_t)T.TP * T.TT;  // (p, T) columns per table
+  const double *tgb = T.t + (size_t)tix * tcols;
+  const int *nub = T.nu + (size_t)tix * tcols;
+  const float4 *recb = T.rec + (size_t)tix * tcols * T.TU;
 
   float ep = 0.f;
   double cu = 0, cp = 0, ct = 0;
@@ -62,7 +70,7 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
   float wp_scale = 0, g00 = 0, gi0 = 0, g10 = 0, gi1 = 0;
   float ta0 = 0, dt0 = 0, ta1 = 0, dt1 = 0, lpr = 0;
   int nt0 = 0, nt1 = 0, n00 = 0, n01 = 0, n10 = 0, n11 = 0;
-  const double *tg0 = nullptr;
+  unsigned tq0 = 0;  // first T-grid entry of p row ipr
 
   // LOS values of the current point; the next point's are loaded after the
   // corner records of the current one, so that waiting for the records does
@@ -114,11 +122,13 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       if (CGA) lpr = (float)(1.0 / log(pb / pa));
       nt0 = ntp[ipr];
       nt1 = ntp[ipr + 1];
-      tg0 = T.t + ((size_t)tix * T.TP + ipr) * T.TT;
-      g00 = (float)tg0[0];
-      gi0 = frcp((float)(tg0[1] - tg0[0]));
-      g10 = (float)tg0[T.TT];
-      gi1 = frcp((float)(tg0[T.TT + 1] - tg0[T.TT]));
+      tq0 = (unsigned)ipr * T.TT;
+      const double t00 = ld32(tgb, tq0), t01 = ld32(tgb, tq0 + 1);
+      const double t10 = ld32(tgb, tq0 + T.TT), t11 = ld32(tgb, tq0 + T.TT + 1);
+      g00 = (float)t00;
+      gi0 = frcp((float)(t01 - t00));
+      g10 = (float)t10;
+      gi1 = frcp((float)(t11 - t10));
       it0 = it1 = -1;
     }
     double pn = p, tn = t, un = useg;
@@ -127,7 +137,6 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
       int j1 = (int)((tq - g10) * gi1);
       j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
-      const size_t c0 = (size_t)tix * T.TP + ipr;
       if (j0 != it0) {
         if (j0 == it0 + 1) {
           const int g0 = gi[0];
@@ -139,10 +148,11 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
           gi[0] = 2 * gi[0] - g1;
         }
         it0 = j0;
-        ta0 = (float)tg0[it0];
-        dt0 = frcp((float)(tg0[it0 + 1] - tg0[it0]));
-        n00 = T.nu[c0 * T.TT + it0];
-        n01 = T.nu[c0 * T.TT + it0 + 1];
+        const double a = ld32(tgb, tq0 + it0), b = ld32(tgb, tq0 + it0 + 1);
+        ta0 = (float)a;
+        dt0 = frcp((float)(b - a));
+        n00 = ld32(nub, tq0 + it0);
+        n01 = ld32(nub, tq0 + it0 + 1);
       }
       if (j1 != it1) {
         if (j1 == it1 + 1) {
@@ -155,17 +165,20 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
           gi[2] = 2 * gi[2] - g3;
         }
         it1 = j1;
-        ta1 = (float)tg0[T.TT + it1];
-        dt1 = frcp((float)(tg0[T.TT + it1 + 1] - tg0[T.TT + it1]));
-        n10 = T.nu[(c0 + 1) * T.TT + it1];
-        n11 = T.nu[(c0 + 1) * T.TT + it1 + 1];
+        const double a = ld32(tgb, tq0 + T.TT + it1), b = ld32(tgb, tq0 + T.TT + it1 + 1);
+        ta1 = (float)a;
+        dt1 = frcp((float)(b - a));
+        n10 = ld32(nub, tq0 + T.TT + it1);
+        n11 = ld32(nub, tq0 + T.TT + it1 + 1);
       }
       if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
-        const float4 *kc[4];
-        kc[0] = T.rec + (c0 * T.TT + it0) * T.TU;
-        kc[1] = kc[0] + T.TU;
-        kc[2] = T.rec + ((c0 + 1) * T.TT + it1) * T.TU;
-        kc[3] = kc[2] + T.TU;
+        // first record of each corner column in the table
+        unsigned ko[4];
+        ko[0] = (tq0 + it0) * T.TU;
+        ko[1] = ko[0] + T.TU;
+        ko[2] = (tq0 + T.TT + it1) * T.TU;
+        ko[3] = ko[2] + T.TU;
+        const float4 *kc[4] = {recb + ko[0], recb + ko[1], recb + ko[2], recb + ko[3]};
         const int nc[4] = {n00, n01, n10, n11};
         const float us = (float)useg;
         float ec[4];
@@ -194,8 +207,8 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
             int g = gi[c];
             g = g < 0 ? 0 : (g > nc[c] - 2 ? nc[c] - 2 : g);
             gi[c] = g;
-            r[c] = kc[c][g];
-            rn[c] = kc[c][g + 1 < nc[c] - 2 ? g + 1 : nc[c] - 2];
+            r[c] = ld32(recb, ko[c] + g);
+            rn[c] = ld32(recb, ko[c] + (g + 1 < nc[c] - 2 ? g + 1 : nc[c] - 2));
           }
           __builtin_amdgcn_sched_barrier(0);
           pn = LOSR(F_P, ipn);
@@ -234,10 +247,10 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       tn = LOSR(F_T, ipn);
       un = LOSR(L.f_u + ig, ipn);
     }
-    out[(size_t)ip * WAVE] = ep;
+    st32(outb, (unsigned)ip * WAVE + lane, ep);
     if (ep >= 1.f) {
       // tau_path below float resolution: every later segment is opaque too
-      for (ip++; ip < R.np; ip++) out[(size_t)ip * WAVE] = 1.f;
+      for (ip++; ip < R.np; ip++) st32(outb, (unsigned)ip * WAVE + lane, 1.f);
       break;
     }
     p = pn;
