@@ -465,7 +465,8 @@ def serving_bench(args, gpu, abi):
         else:
             reqs.append(gpu.Server.request(int(kinds[i]), atms[which[i]], geo[key], rett=(-100.0, 100.0),
                                            retq=retq))
-    srv = gpu.Server(ctl, ret, precision=abi.JUR_FP32, lanes=args.serve_lanes)
+    srv = gpu.Server(ctl, ret, precision=abi.JUR_FP32, lanes=args.serve_lanes,
+                     forward_lane=1 if args.serve_forward_lane else 0)
     # warm: tables, kernels, allocations
     for kind in (abi.JUR_REQ_FORWARD, abi.JUR_REQ_JACOBIAN, abi.JUR_REQ_RETRIEVAL):
         i = srv.submit(gpu.Server.request(kind, apr if kind == abi.JUR_REQ_RETRIEVAL else atms[0],
@@ -530,7 +531,7 @@ def serving_bench(args, gpu, abi):
         "throughput_req_per_s": nreq / (t_done.max() - t_sub.min()),
         "offered_req_per_s": nreq / t_arr[-1],
         "submit_lag_max_s": lag[0],
-        "device_rounds": rounds - r0, "max_in_flight": max_active, "lanes": args.serve_lanes,
+        "device_rounds": rounds - r0, "max_in_flight": max_active, "lanes": args.serve_lanes, "forward_lane": bool(args.serve_forward_lane),
         "latency_ms": {"p50": float(np.percentile(lat[ok], 50) * 1e3), "p99": float(np.percentile(lat[ok], 99) * 1e3),
                        "max": float(lat[ok].max() * 1e3)},
         "wall_s": wall,
@@ -572,6 +573,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-lanes", type=int, default=1)
+    ap.add_argument("--serve-forward-lane", action="store_true",
+                    help="one more server lane for forward requests")
     ap.add_argument("--jac-only", action="store_true", help="skip the forward-model section")
     ap.add_argument("--jac-nprof", type=int, default=NJAC_PROF)
     ap.add_argument("--jac-steps", type=int, default=2)

@@ -330,6 +330,10 @@ typedef struct {
   size_t max_bytes;       /* device memory for requests in flight (0: 16 GiB) */
   int max_batch;          /* most requests in flight (0: no limit) */
   int lanes;              /* worker lanes, each with its own stream (0: 1) */
+  int forward_lane;       /* 1: one more lane that takes the forward
+                             requests (forward pass, no Jacobian-mode
+                             sweep); the other lanes then take Jacobians
+                             and retrievals */
 } jur_server_opts_t;
 
 typedef struct {

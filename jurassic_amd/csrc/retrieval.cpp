@@ -223,6 +223,10 @@ struct Engine {
   const ctl_t *ctl = nullptr;  // forward model
   jur_ctx *c = nullptr;
   int method = JUR_JAC_ADJOINT, precision = JUR_FP64;
+  // forward-only engine (a server lane for forward requests): radiances from
+  // the forward pass, not from the adjoint pass that keeps retrieval costs
+  // consistent with their Jacobians
+  bool fwd_pass = false;
   std::vector<Job *> jobs;  // active
   DevBuf<LmDesc> d_desc;
   DevBuf<int> d_list, d_status;
@@ -508,7 +512,7 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
   std::vector<atm_t> atms(np);
   std::vector<int> prof(nray), xm, pmap(np);
   int nmap = 0;
-  const bool jac_pass = with_k || method == JUR_JAC_ADJOINT;
+  const bool jac_pass = with_k || (method == JUR_JAC_ADJOINT && !fwd_pass);
   std::vector<double> geom((size_t)7 * nray);
   std::vector<unsigned char> mask((size_t)nray * nd);
   for (int k = 0; k < np; k++) {
@@ -717,7 +721,7 @@ int Engine::round(std::vector<Job *> &done) {
     t1 = tnow();
   }
   const double t2 = tnow();
-  prof[1] += t1 - t0;
+  prof[fwd_pass ? 2 : 1] += t1 - t0;
   prof[2] += t2 - t1;
   for (int r : need_f)
     if (jobs[r]->st == ST_EVAL_F_OUT) jobs[r]->st = ST_DONE;
@@ -1059,6 +1063,8 @@ struct jur_lane {
   std::unique_ptr<jur_ctx> ctx;
   hipStream_t st = nullptr;
   std::thread th;
+  unsigned kinds = 7u;  // request kinds the lane takes (bit JUR_REQ_*)
+  bool takes(const jur::Job *J) const { return kinds >> J->kind & 1u; }
 };
 
 struct jur_server {
@@ -1099,17 +1105,26 @@ void jur_server::loop(jur_lane *L) {
     std::vector<Job *> admit;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_in.wait(lk, [&] { return stop || fatal || !queue.empty() || !E.jobs.empty(); });
+      auto queued = [&] {
+        for (const Job *J : queue)
+          if (L->takes(J)) return true;
+        return false;
+      };
+      cv_in.wait(lk, [&] { return stop || fatal || !E.jobs.empty() || queued(); });
       if (stop || fatal) return;
-      // admission: in arrival order while the device footprint fits (an
-      // idle lane always takes one request)
-      while (!queue.empty()) {
-        Job *J = queue.front();
+      // admission: the lane's kinds in arrival order while the device
+      // footprint fits (an idle lane always takes one request)
+      for (auto it = queue.begin(); it != queue.end();) {
+        Job *J = *it;
+        if (!L->takes(J)) {
+          ++it;
+          continue;
+        }
         const size_t need = (size_t)J->obs_meas->nr * ctl.nd * 8 * 4 +
                             (J->kind == JUR_REQ_FORWARD ? 0 : (size_t)J->obs_meas->nr * ctl.nd * NP * 8 * 2);
         const bool idle = E.jobs.empty() && admit.empty();
         if (!idle && (in_use + need > budget || (max_batch > 0 && in_flight >= max_batch))) break;
-        queue.pop_front();
+        it = queue.erase(it);
         in_use += need;
         in_flight++;
         J->admit_bytes = need;
@@ -1180,10 +1195,14 @@ int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret, const jur_server_o
   if (!ctl || !ret || !out) return set_err(JUR_ERR_ARG, "NULL argument");
   const int method = opt ? opt->method : JUR_JAC_ADJOINT;
   const int precision = opt ? opt->precision : JUR_FP32;
-  const int nlanes = (opt && opt->lanes > 0) ? opt->lanes : 1;
+  // forward_lane: lane 0 takes the forward requests, so that forward
+  // models do not wait for Jacobian rounds; the others take the rest
+  const bool split = opt && opt->forward_lane;
+  const int nlanes = ((opt && opt->lanes > 0) ? opt->lanes : 1) + (split ? 1 : 0);
   if (method != JUR_JAC_FD && method != JUR_JAC_ADJOINT) return set_err(JUR_ERR_ARG, "invalid method");
   if (precision != JUR_FP64 && precision != JUR_FP32) return set_err(JUR_ERR_ARG, "invalid precision");
   if (nlanes > 16) return set_err(JUR_ERR_ARG, "at most 16 lanes");
+  if (opt && opt->lanes < 0) return set_err(JUR_ERR_ARG, "negative lane count");
   std::unique_ptr<jur_server> s(new jur_server());
   memcpy(&s->ctl, ctl, sizeof(ctl_t));
   memcpy(&s->ret, ret, sizeof(jur_ret_t));
@@ -1206,6 +1225,10 @@ int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret, const jur_server_o
     L->E.method = method;
     L->E.precision = precision;
     L->E.st = L->st;
+    if (split) {
+      L->kinds = l == 0 ? 1u << JUR_REQ_FORWARD : (1u << JUR_REQ_JACOBIAN | 1u << JUR_REQ_RETRIEVAL);
+      L->E.fwd_pass = l == 0;
+    }
     s->lanes.push_back(std::move(L));
   }
   jur_server *raw = s.get();
@@ -1309,7 +1332,8 @@ int jur_server_submit(jur_server_t *s, const jur_request_t *req, uint64_t *id) {
     s->all[J->id] = std::move(J);
     s->queue.push_back(raw);
   }
-  s->cv_in.notify_one();
+  // every lane: the one a single wakeup reaches may not take this kind
+  s->cv_in.notify_all();
   return JUR_OK;
 }
 

@@ -56,8 +56,8 @@ def _mix(oracle):
     return ctl, reqs
 
 
-def _run(gpu, ctl, ret, reqs, precision):
-    srv = gpu.Server(ctl, ret, precision=precision)
+def _run(gpu, ctl, ret, reqs, precision, lanes=0, forward_lane=0):
+    srv = gpu.Server(ctl, ret, precision=precision, lanes=lanes, forward_lane=forward_lane)
     try:
         keep = []
         ids = []
@@ -128,6 +128,32 @@ def test_server_matches_direct_calls(gpu, oracle):
                                                    abi.copy_struct(r["atm"]), precision=prec)
             assert res.niter == n1
             np.testing.assert_allclose(hist, h1, rtol=1e-9)
+
+
+def test_server_forward_lane(gpu, oracle):
+    """With a forward lane, forward requests take the forward pass (the
+    kernels of formod: equal to the last bit), and Jacobians and retrievals
+    are those of the one-lane server; so are they with two general lanes."""
+    ctl, reqs = _mix(oracle)
+    ret = _ret(ctl.nd)
+    prec = abi.JUR_FP64
+    base, _ = _run(gpu, ctl, ret, reqs, prec)
+    for lanes, fl in ((1, 1), (2, 0)):
+        got, (rounds, _) = _run(gpu, ctl, ret, reqs, prec, lanes=lanes, forward_lane=fl)
+        assert rounds >= 1
+        for r, (res, (obs, atm, k, hist)), (res0, (obs0, atm0, k0, hist0)) in zip(reqs, got, base):
+            if r["kind"] == abi.JUR_REQ_FORWARD and fl:
+                o = abi.copy_struct(r["obs"])
+                gpu.formod(ctl, abi.copy_struct(r["atm"]), o, prec)
+                assert np.array_equal(abi.obs_arrays(obs, ctl)[0], abi.obs_arrays(o, ctl)[0])
+            else:
+                np.testing.assert_allclose(abi.obs_arrays(obs, ctl)[0], abi.obs_arrays(obs0, ctl)[0],
+                                           rtol=1e-12)
+            if r["kind"] == abi.JUR_REQ_JACOBIAN:
+                np.testing.assert_allclose(k, k0, rtol=1e-12, atol=1e-12 * np.abs(k0).max())
+            if r["kind"] == abi.JUR_REQ_RETRIEVAL:
+                assert res.niter == res0.niter
+                np.testing.assert_allclose(hist, hist0, rtol=1e-9)
 
 
 def test_server_reports_request_errors(gpu, oracle):
