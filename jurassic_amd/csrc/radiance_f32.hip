@@ -123,13 +123,57 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   float drefl = 0.f, tau_refl = 1.f;
   float epp[NG];  // path emissivity of each emitter before the segment
   for (int g = 0; g < NG; g++) epp[g] = 0.f;
+  // The next point's emissivities and LOS fields are loaded one iteration
+  // ahead, so that a point's arithmetic does not wait for its own loads
+  // (absent fields read F_DS in their place and are not used).
+  const int f_k = has_k ? L.f_k + C.window : F_DS;
+  const int f_cl = L.f_cl >= 0 ? L.f_cl : F_DS;
+  const int f_uc = co2_on ? L.f_u + M.ig_co2 : F_DS;
+  const int f_uh = h2o_on ? L.f_u + M.ig_h2o : F_DS;
+  const int f_qh = h2o_on ? F_Q0 + M.ig_h2o : F_DS;
+  float en[NG];
+  double pn = 0, tn = 0, dsn = 0, kn = 0, cln = 0, ucn = 0, uhn = 0, qhn = 0;
+  auto load = [&](int ip) {
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+      en[g] = (g < ng && (gmask >> g & 1u)) ? eps_g[((size_t)g * npb + ip) * WAVE] : 0.f;
+    pn = LOSR(F_P, ip);
+    tn = LOSR(F_T, ip);
+    dsn = LOSR(F_DS, ip);
+    kn = LOSR(f_k, ip);
+    cln = LOSR(f_cl, ip);
+    ucn = LOSR(f_uc, ip);
+    uhn = LOSR(f_uh, ip);
+    qhn = LOSR(f_qh, ip);
+  };
+  // the source-function bracket of the next point (its T row of sr_col),
+  // loaded at the end of the iteration before
+  double s0n = 0, s1n = 0, stn = 0;
+  auto load_src = [&](double t) {
+    int it = (int)(((float)t - st0) * inv_dst);
+    it = it < 0 ? 0 : (it > TBLNS - 2 ? TBLNS - 2 : it);
+    s0n = sr_col[it];
+    s1n = sr_col[it + 1];
+    stn = M.st[it];
+  };
+  if (R.np > 0) {
+    load(0);
+    load_src(tn);
+  }
   for (int ip = 0; ip < R.np; ip++) {
+    float ec[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) ec[g] = en[g];
+    const double pc = pn, tc = tn, dsc = dsn, kc = kn, clc = cln, ucc = ucn, uhc = uhn, qhc = qhn;
+    const double s0c = s0n, s1c = s1n, stc = stn;
+    load(ip + 1 < R.np ? ip + 1 : ip);
+
     bool opaque = false;
     float etot = 0.f;
 #pragma unroll
     for (int g = 0; g < NG; g++) {
       if (g >= ng || !(gmask >> g & 1u)) continue;
-      const float ep = eps_g[((size_t)g * npb + ip) * WAVE];
+      const float ep = ec[g];
       opaque |= (ep >= 1.f);
       // segment emissivity 1 - (1 - ep)/(1 - epp) = (ep - epp)/(1 - epp)
       const float e = fdiv(ep - epp[g], 1.f - epp[g]);
@@ -138,20 +182,20 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
     }
     if (opaque) break;
 
-    const float pf = (float)LOSR(F_P, ip);
-    const float tf = (float)LOSR(F_T, ip);
-    const float dsf = (float)LOSR(F_DS, ip);
-    float beta = has_k ? (float)LOSR(L.f_k + C.window, ip) : 0.f;
-    if (L.f_cl >= 0) beta += (float)LOSR(L.f_cl, ip) * kcl;
+    const float pf = (float)pc;
+    const float tf = (float)tc;
+    const float dsf = (float)dsc;
+    float beta = has_k ? (float)kc : 0.f;
+    if (L.f_cl >= 0) beta += (float)clc * kcl;
     if (co2_on) {
-      const float uc = (float)LOSR(L.f_u + M.ig_co2, ip);
+      const float uc = (float)ucc;
       const float d230 = tf - 230.f, d260 = tf - 260.f, d296 = tf - 296.f;
       const float ctw = co2_a * d260 * d296 - co2_b * d230 * d296 + co2_c * d230 * d260;
       beta += uc * pf * kctm * ctw / dsf;
     }
     if (h2o_on) {
-      const float uh = (float)LOSR(L.f_u + M.ig_h2o, ip);
-      const float qh = (float)LOSR(F_Q0 + M.ig_h2o, ip);
+      const float uh = (float)uhc;
+      const float qh = (float)qhc;
       const float slf = h2o_slf * exp2f(h2o_l2r * (296.f - tf));
       const float a1 = nuf * uh * tanhf(0.7193876f / tf * nuf);
       const float a3 = pf * (float)(1.0 / JUR_P0) * (qh * slf + (1.f - qh) * h2o_frn) * 1e-20f;
@@ -168,10 +212,8 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
     }
 
     // source function: linear in the uniform 1200-point temperature grid
-    int it = (int)((tf - st0) * inv_dst);
-    it = it < 0 ? 0 : (it > TBLNS - 2 ? TBLNS - 2 : it);
-    const float s0 = (float)sr_col[it], s1 = (float)sr_col[it + 1];
-    const float src = s0 + (s1 - s0) * ((tf - (float)M.st[it]) * inv_dst);
+    const float s0 = (float)s0c, s1 = (float)s1c;
+    const float src = s0 + (s1 - s0) * ((tf - (float)stc) * inv_dst);
 
     const float c = -expm1f(-beta * dsf);
     const float e = etot + c - etot * c;
@@ -185,6 +227,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
     }
     drefl = drefl - drefl * e + src * e;
     tau_refl = tau_refl - tau_refl * e;
+    load_src(tn);
   }
 
   double radd = (double)rad.s - (double)rad.c;
