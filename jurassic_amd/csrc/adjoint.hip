@@ -616,9 +616,8 @@ struct LevelWindow {
 // rad, tau, H_i (scratch), the number of points before the first opaque
 // segment, and the K entries of continua, source function, window
 // extinction, clouds and surface.
-// The FP32 variant is held to 3 waves per SIMD (166 VGPRs).
 template <typename SF>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(sizeof(SF) == sizeof(float) ? 3 : 1, 8))) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
+__global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
                                                   const RayInfo *info, int blk0, int nblk, JacView J,
                                                   double *rad_out, double *tau_out) {
   const int nd = M.nd, ng = M.ng;
@@ -699,30 +698,74 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(sizeof(SF)
   const int fu_h2o = M.ig_h2o >= 0 ? L.f_u + M.ig_h2o : -1;
   const int fu_co2 = M.ig_co2 >= 0 ? L.f_u + M.ig_co2 : -1;
 
+  // Point inputs of the sweeps, loaded one point ahead so that a point's
+  // arithmetic does not wait for its own loads (absent fields read F_DS in
+  // their place and are not used).  The emitters' stored values are loaded
+  // with them and multiplied (in qprod's order) one iteration later.
+  struct PtIn {
+    double z, p, t, ds, qh, uc, uh, kw, cl;
+  };
+  const int f_qh = fq_h2o >= 0 ? fq_h2o : F_DS, f_uc = fu_co2 >= 0 ? fu_co2 : F_DS;
+  const int f_uh = fu_h2o >= 0 ? fu_h2o : F_DS, f_kw = has_k ? L.f_k + C.window : F_DS;
+  const int f_cl = cloud ? L.f_cl : F_DS;
+  auto ld_pt = [&](int ip, PtIn &x, bool withz) {
+    if (withz) x.z = LOSR(F_Z, ip);
+    x.p = LOSR(F_P, ip);
+    x.t = LOSR(F_T, ip);
+    x.ds = LOSR(F_DS, ip);
+    x.qh = LOSR(f_qh, ip);
+    x.uc = LOSR(f_uc, ip);
+    x.uh = LOSR(f_uh, ip);
+    x.kw = LOSR(f_kw, ip);
+    x.cl = LOSR(f_cl, ip);
+  };
+  SF en[NG];
+  auto ld_eps = [&](int ip) {
+#pragma unroll
+    for (int g = 0; g < NG; g++) en[g] = (g < ng && (gm >> g & 1u)) ? EPS_AT(g, ip) : SF(0);
+  };
+  auto prod_en = [&]() {
+    double q = 1;
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+      if (g < ng && (gm >> g & 1u)) q *= trans_of(en[g]);
+    return q;
+  };
+  PtIn nx{};
+
   // ---------------- forward sweep: tau_i (and D_i) per point ----------------
   double rad = 0, tau = 1, dr = 0;
   int nef = R.np;
   {
-    double qp = 1;
+    double qp = 1, qn = 1;
+    if (R.np > 0) {
+      ld_pt(0, nx, false);
+      ld_eps(0);
+      qn = prod_en();
+    }
     for (int ip = 0; ip < R.np; ip++) {
-      const double q = qprod(ip);
+      const PtIn cu = nx;
+      const double q = qn;
+      const int ipn = ip + 1 < R.np ? ip + 1 : ip;
+      ld_pt(ipn, nx, false);
+      ld_eps(ipn);
       if (!(q > 0)) {
         nef = ip;
         break;
       }
       const double tg = q / qp;
       qp = q;
-      const double p = LOSR(F_P, ip), t = LOSR(F_T, ip), ds = LOSR(F_DS, ip);
       BetaGrad bg;
       double src, dsrc, rt, rp;
-      const double s = segment(p, t, ds, fq_h2o >= 0 ? LOSR(fq_h2o, ip) : 0.0, fu_co2 >= 0 ? LOSR(fu_co2, ip) : 0.0,
-                               fu_h2o >= 0 ? LOSR(fu_h2o, ip) : 0.0, has_k ? LOSR(L.f_k + C.window, ip) : 0.0,
-                               cloud ? LOSR(L.f_cl, ip) * kcl : 0.0, tg, bg, src, dsrc, rt, rp);
+      const double s = segment(cu.p, cu.t, cu.ds, fq_h2o >= 0 ? cu.qh : 0.0, fu_co2 >= 0 ? cu.uc : 0.0,
+                               fu_h2o >= 0 ? cu.uh : 0.0, has_k ? cu.kw : 0.0, cloud ? cu.cl * kcl : 0.0, tg,
+                               bg, src, dsrc, rt, rp);
       SCR(0, ip) = tau;
       if (J.nscr > 1) SCR(1, ip) = dr;
       rad += src * (1 - s) * tau;
       dr = dr * s + src * (1 - s);
       tau *= s;
+      qn = prod_en();
     }
   }
 
@@ -764,21 +807,32 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(sizeof(SF)
     LevelWindow<5> W;
     int lev = -1;
     double qi_cur = nef > 0 ? qprod(nef - 1) : 1;
+    // point ip's inputs and scratch, and the emitters' values at ip - 1,
+    // loaded one iteration ahead
+    double sc0n = 0, sc1n = 0;
+    auto ld_rev = [&](int ip) {
+      ld_pt(ip, nx, true);
+      sc0n = SCR(0, ip);
+      sc1n = J.nscr > 1 ? SCR(1, ip) : 0.0;
+      if (ip > 0) ld_eps(ip - 1);
+    };
+    if (nef > 0) ld_rev(nef - 1);
     for (int ip = nef - 1; ip >= 0; ip--) {
-      const double qm = ip > 0 ? qprod(ip - 1) : 1;
+      const PtIn cu = nx;
+      const double tau_i = sc0n, dr_i = sc1n;
+      const double qm = ip > 0 ? prod_en() : 1;
+      if (ip > 0) ld_rev(ip - 1);
       const double tg = qi_cur / qm;
       qi_cur = qm;
-      const double z = LOSR(F_Z, ip), p = LOSR(F_P, ip), t = LOSR(F_T, ip), ds = LOSR(F_DS, ip);
-      const double q_h2o = fq_h2o >= 0 ? LOSR(fq_h2o, ip) : 0.0;
-      const double u_co2 = fu_co2 >= 0 ? LOSR(fu_co2, ip) : 0.0;
-      const double u_h2o = fu_h2o >= 0 ? LOSR(fu_h2o, ip) : 0.0;
-      const double aux = cloud ? LOSR(L.f_cl, ip) : 0.0;
+      const double z = cu.z, p = cu.p, t = cu.t, ds = cu.ds;
+      const double q_h2o = fq_h2o >= 0 ? cu.qh : 0.0;
+      const double u_co2 = fu_co2 >= 0 ? cu.uc : 0.0;
+      const double u_h2o = fu_h2o >= 0 ? cu.uh : 0.0;
+      const double aux = cloud ? cu.cl : 0.0;
       BetaGrad bg;
       double src, dsrc, rt, rp;
-      const double s = segment(p, t, ds, q_h2o, u_co2, u_h2o, has_k ? LOSR(L.f_k + C.window, ip) : 0.0,
-                               aux * kcl, tg, bg, src, dsrc, rt, rp);
-      const double tau_i = SCR(0, ip);
-      const double dr_i = J.nscr > 1 ? SCR(1, ip) : 0.0;
+      const double s = segment(p, t, ds, q_h2o, u_co2, u_h2o, has_k ? cu.kw : 0.0, aux * kcl, tg, bg, src, dsrc,
+                               rt, rp);
       const double G = tau_i * (tb - src) + db * (dr_i - src);
       const double src_bar = (1 - s) * (tau_i + db);
       tb = src * (1 - s) + tb * s;
