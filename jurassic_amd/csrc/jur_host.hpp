@@ -58,6 +58,14 @@ struct DevBuf {
     if (m) HIPCHK(hipMemcpy(p, h, m * sizeof(T), hipMemcpyHostToDevice));
     return JUR_OK;
   }
+  // Ordered on stream s, not waited for: h must stay valid until the
+  // stream is next synchronized.
+  int upload_async(const T *h, size_t m, hipStream_t s) {
+    int rc = reserve(m);
+    if (rc) return rc;
+    if (m) HIPCHK(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
+    return JUR_OK;
+  }
   // Ordered on stream s (complete on return).
   int upload(const T *h, size_t m, hipStream_t s) {
     int rc = reserve(m);

@@ -534,18 +534,20 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
     }
     if (wantk[k]) xm.insert(xm.end(), J.xmap.begin(), J.xmap.end());
   }
-  if ((rc = p_atm.upload(atms.data(), np, st))) return rc;
-  if ((rc = p_prof.upload(prof.data(), nray, st))) return rc;
-  if ((rc = p_geom.upload(geom.data(), geom.size(), st))) return rc;
-  if ((rc = p_mask.upload(mask.data(), mask.size(), st))) return rc;
+  // stream-ordered uploads from buffers that live to the end of this call;
+  // the pass synchronizes the stream before it returns
+  if ((rc = p_atm.upload_async(atms.data(), np, st))) return rc;
+  if ((rc = p_prof.upload_async(prof.data(), nray, st))) return rc;
+  if ((rc = p_geom.upload_async(geom.data(), geom.size(), st))) return rc;
+  if ((rc = p_mask.upload_async(mask.data(), mask.size(), st))) return rc;
   if ((rc = p_rad.reserve((size_t)nray * nd))) return rc;
   if ((rc = p_tau.reserve((size_t)nray * nd))) return rc;
   BatchView B{np, p_atm.p, nray, p_prof.p, p_geom.p};
   if (!with_k) maxn = 1;
   if (jac_pass) {
-    if ((rc = p_pmap.upload(pmap.data(), np, st))) return rc;
+    if ((rc = p_pmap.upload_async(pmap.data(), np, st))) return rc;
     if (xm.empty()) xm.assign(JUR_XMAP_LEN, -1);  // no job wants K: one unused map
-    if ((rc = p_xmap.upload(xm.data(), xm.size(), st))) return rc;
+    if ((rc = p_xmap.upload_async(xm.data(), xm.size(), st))) return rc;
     if ((rc = p_k.reserve((size_t)nray * nd * maxn))) return rc;
     JacArgs ja;
     ja.xmap = p_xmap.p;
@@ -558,6 +560,8 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
     rc = forward_pass(c, B, precision, p_mask.p, p_rad.p, p_tau.p, nullptr, nullptr, st);
   }
   if (rc) {
+    // the uploads above read this call's buffers
+    (void)hipStreamSynchronize(st);
     // A ray that fails (observer below the surface, too many LOS points)
     // fails the pass: evaluate the jobs one by one so that only its own job
     // fails.
@@ -569,11 +573,29 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
       if ((rc = eval(std::vector<int>{ids[k]}, std::vector<char>{wantk[k]}))) return rc;
     return JUR_OK;
   }
-  h_buf.resize((size_t)nray * nd * 2);
+  // radiances, transmittances and the K rows of the Jacobian requests, with
+  // one wait
+  std::vector<size_t> koff(ids.size(), 0);
+  size_t ktot = 0;
+  if (with_k)
+    for (size_t q = 0; q < ids.size(); q++) {
+      const Job &J = *jobs[ids[q]];
+      if (J.kind != JUR_REQ_JACOBIAN || !wantk[q]) continue;
+      koff[q] = ktot;
+      ktot += (size_t)J.m * maxn;
+    }
+  h_buf.resize((size_t)nray * nd * 2 + ktot);
   HIPCHK(hipMemcpyAsync(h_buf.data(), p_rad.p, (size_t)nray * nd * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipMemcpyAsync(h_buf.data() + (size_t)nray * nd, p_tau.p, (size_t)nray * nd * sizeof(double),
                         hipMemcpyDeviceToHost, st));
+  double *krows = h_buf.data() + (size_t)nray * nd * 2;
+  if (with_k)
+    for (size_t q = 0; q < ids.size(); q++) {
+      const Job &J = *jobs[ids[q]];
+      if (J.kind != JUR_REQ_JACOBIAN || !wantk[q]) continue;
+      HIPCHK(hipMemcpyAsync(krows + koff[q], p_k.p + (size_t)J.ray0 * nd * maxn,
+                            (size_t)J.m * maxn * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
   HIPCHK(hipStreamSynchronize(st));
   for (int r : ids) {
     Job &J = *jobs[r];
@@ -586,15 +608,11 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
       }
   }
   if (!with_k) return JUR_OK;
-  // Jacobian requests: copy their finite rows out
-  std::vector<double> rows;
+  // Jacobian requests: their finite rows
   for (size_t q = 0; q < ids.size(); q++) {
     Job &J = *jobs[ids[q]];
     if (J.kind != JUR_REQ_JACOBIAN || !wantk[q]) continue;
-    rows.resize((size_t)J.m * maxn);
-    HIPCHK(hipMemcpyAsync(rows.data(), p_k.p + (size_t)J.ray0 * nd * maxn, rows.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    const double *rows = krows + koff[q];
     size_t v = 0;
     for (int ir = 0; ir < J.nr; ir++)
       for (int id = 0; id < nd; id++)
