@@ -975,13 +975,18 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
       double pa = 0, pb = 0;
       float wp_scale = 0, g00 = 0, gi0 = 0, g10 = 0, gi1 = 0, ta0 = 0, dt0 = 0, ta1 = 0, dt1 = 0;
       unsigned tq0 = 0;  // first T-grid entry of p row ipr
-      double p = 0, t = 0, useg = 0;
+      // the point's inputs (loaded during the previous point: LOS, the
+      // emissivity before it, and its H and geometry for the chain rule)
+      double p = 0, t = 0, useg = 0, Hc = 0, zc = 0, dsc = 0;
       float epf = 0.f;
       if (nef > 0) {
         p = LOSQ(F_P, nef - 1);
         t = LOSQ(F_T, nef - 1);
         useg = LOSQ(fu, nef - 1);
         epf = nef > 1 ? EPSQ(nef - 2) : 0.f;
+        Hc = ld32(scb, (unsigned)(nef - 1) * nscr * WAVE + lane);
+        zc = LOSQ(F_Z, nef - 1);
+        dsc = LOSQ(F_DS, nef - 1);
       }
       for (int ip = nef - 1; ip >= 0; ip--) {
         const int ipn = ip > 0 ? ip - 1 : 0;
@@ -1024,7 +1029,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
           it0 = it1 = -1;
         }
         LkPart d{1, 0, 0, 0};
-        double pn = p, tn = t, un = useg;
+        double pn = p, tn = t, un = useg, Hx = Hc, zx = zc, dsx = dsc;
         float en_f = epf;
         bool loaded = false;
         if (nt0 >= 2 && nt1 >= 2) {
@@ -1089,6 +1094,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
             tn = LOSQ(F_T, ipn);
             un = LOSQ(fu, ipn);
             en_f = ipn > 0 ? EPSQ(ipn - 1) : 0.f;
+            Hx = ld32(scb, (unsigned)ipn * nscr * WAVE + lane);
+            zx = LOSQ(F_Z, ipn);
+            dsx = LOSQ(F_DS, ipn);
             loaded = true;
             __builtin_amdgcn_sched_barrier(0);
             const float us = (float)useg;
@@ -1134,9 +1142,11 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
           tn = LOSQ(F_T, ipn);
           un = LOSQ(fu, ipn);
           en_f = ipn > 0 ? EPSQ(ipn - 1) : 0.f;
+          Hx = ld32(scb, (unsigned)ipn * nscr * WAVE + lane);
+          zx = LOSQ(F_Z, ipn);
+          dsx = LOSQ(F_DS, ipn);
         }
-        const double H = ld32(scb, (unsigned)ip * nscr * WAVE + lane);
-        const double z = LOSQ(F_Z, ip), ds = LOSQ(F_DS, ip);
+        const double H = Hc, z = zc, ds = dsc;
         const double trp = 1.0 - (double)epf;
         // the chain rule's divisions by the transmittance, T and p as float
         // reciprocals (relative error 1e-7, far below the FP32 lookups')
@@ -1156,6 +1166,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
         t = tn;
         useg = un;
         epf = en_f;
+        Hc = Hx;
+        zc = zx;
+        dsc = dsx;
       }
       W.flush(xm, qi, kt);
 #undef LOSQ
