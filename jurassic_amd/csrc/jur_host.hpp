@@ -141,6 +141,7 @@ struct jur_ctx {
   jur::DevBuf<int> d_npb;
   jur::DevBuf<size_t> d_eoff;
   jur::DevBuf<jur::RayInfo> d_info;
+  jur::DevBuf<char> d_tail;  // per-ray state from the march to the ray finish
   jur::DevBuf<double> d_zr;
   jur::DevBuf<unsigned long long> d_red;
   jur::DevBuf<double> d_rad0, d_tau0;

@@ -30,7 +30,8 @@
 namespace jur {
 
 // kernel launchers (raytrace.hip, radiance_f64.hip, radiance_f32.hip, aux_kernels.hip)
-void launch_raytrace(const ModelView &, const BatchView &, const LosView &, RayInfo *, int, int,
+size_t ray_tail_bytes();
+void launch_raytrace(const ModelView &, const BatchView &, const LosView &, RayInfo *, void *, int, int,
                      hipStream_t);
 void launch_hydrostatic(atm_t *, int, double, int, hipStream_t);
 void launch_lookup_f64(const ModelView &, const BatchView &, const LosView &, const EpsView &,
@@ -646,6 +647,7 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
 
   const int nblk = (B.nray + WAVE - 1) / WAVE;
   if ((rc = c->d_info.reserve(B.nray))) return rc;
+  if ((rc = c->d_tail.reserve((size_t)B.nray * ray_tail_bytes()))) return rc;
   if ((rc = c->d_red.reserve(3))) return rc;
 
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -689,7 +691,7 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     for (int b0 = 0; b0 < nblk; b0 += chunk) {
       const int nb = std::min(chunk, nblk - b0);
       HIPCHK(hipEventRecord(c->ev[0], st));
-      launch_raytrace(M, B, L, c->d_info.p, b0, nb, st);
+      launch_raytrace(M, B, L, c->d_info.p, c->d_tail.p, b0, nb, st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->ev[1], st));
       launch_block_np(c->d_info.p, B.nray, b0, nb, c->d_npb.p, st);

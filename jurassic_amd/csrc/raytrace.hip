@@ -82,8 +82,51 @@ __device__ __forceinline__ double sza(double sec, double lon, double lat) {
 
 #define LOSW(f, ip) LOS_AT(base, L, f, ip)
 
+// First minimum of z along the LOS (gsl_stats_min_index semantics: a NaN
+// ends the search) and the positions of its two neighbours, for
+// tangent_point.
+struct MinTrack {
+  double zlow = 0;
+  double xbefore[3] = {0, 0, 0}, xafter[3] = {0, 0, 0};
+  int imin = 0;
+  bool after_pending = false, nan_seen = false;
+  // point np at z, position x; xprev is the position of point np - 1
+  __device__ __forceinline__ void add(int np, double z, const double *x, const double *xprev) {
+    if (!nan_seen) {
+      const bool lower = (np == 0) || z < zlow;
+      if (lower) zlow = z;
+      if (lower || isnan(z)) {
+        imin = np;
+        if (np > 0)
+          for (int i = 0; i < 3; i++) xbefore[i] = xprev[i];
+        after_pending = true;
+        nan_seen = isnan(z);
+      } else if (after_pending) {
+        for (int i = 0; i < 3; i++) xafter[i] = x[i];
+        after_pending = false;
+      }
+    } else if (after_pending) {
+      for (int i = 0; i < 3; i++) xafter[i] = x[i];
+      after_pending = false;
+    }
+  }
+};
+
+// What the march (k_raytrace) hands to k_ray_finish for a ray with LOS
+// points: the step that left the altitude range, which is clipped to it
+// there, and the state for the tangent point.
+struct RayTail {
+  double xprev[3];  // the last point written
+  double xcut[3];   // the step beyond the altitude range
+  double xfirst[3];
+  double zprev, zcut, dscut;  // altitudes of both, step length
+  double zmin, zmax;
+  MinTrack mt;
+  int zdir;
+};
+
 __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, LosView L,
-                                                   RayInfo *info, int blk0) {
+                                                   RayInfo *info, RayTail *tail, int blk0) {
   JUR_FP_EXACT
   const int lane = threadIdx.x;
   const int blk = blk0 + blockIdx.x;
@@ -100,13 +143,13 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     return;
   }
   const atm_t *atm = &B.atm[iprof];
-  const double otime = g[0], obsz = g[1], obslon = g[2], obslat = g[3];
+  const double obsz = g[1], obslon = g[2], obslat = g[3];
   const double vpz = g[4], vplon = g[5], vplat = g[6];
 
   const double h = 0.02, zrefrac = 60;
-  double ex0[3], ex1[3], lat, lon, n, ngr[3], norm, p, t, x[3], xh[3], xobs[3], xvp[3];
+  double ex0[3], ex1[3], n, ngr[3], norm, p, t, x[3], xh[3], xobs[3], xvp[3];
   double z = 1e99, zmax, zmin;
-  int stop = 0, np = 0;
+  int np = 0;
 
   RayInfo R;
   R.np = 0;
@@ -185,10 +228,14 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   // needed after the march: the first and the last point, and the two
   // neighbours of the first minimum of z (tangent_point); cart2geo of the
   // same x gives the values the reference stores per point.
-  double zprev = 0, xprev[3] = {0, 0, 0}, xfirst[3] = {0, 0, 0}, xlast[3] = {0, 0, 0};
-  double xbefore[3] = {0, 0, 0}, xafter[3] = {0, 0, 0}, zlow = 0;
+  // The positions go straight to the ray's tail (they change rarely), the
+  // rest of MinTrack's state stays in registers.
+  RayTail *tl = tail + ir;
+  double zprev = 0, xprev[3] = {0, 0, 0};
+  double zlow = 0;
   int imin = 0;
   bool after_pending = false, nan_seen = false;
+  double dscut = 0;
   for (;;) {
     double ds = M.rayds;
     if (M.raydz > 0) {
@@ -201,22 +248,14 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     z = cart2z(x);
 
     if (z < zmin || z > zmax) {
-      if (np == 0) {
-        // The entry search ended outside the altitude range, so there is no
-        // previous point to clip to (the reference would read los->z[-1]):
-        // the ray has no LOS points.
-        break;
-      }
-      stop = (z < zmin ? 2 : 1);
-      const double frac = ((z < zmin ? zmin : zmax) - zprev) / (z - zprev);
-      double zd;
-      cart2geo(xprev, &zd, &lon, &lat);
-      geo2cart(zprev, lon, lat, xh);
-      for (int i = 0; i < 3; i++) x[i] = xh[i] + frac * (x[i] - xh[i]);
-      z = cart2z(x);
-      // los->ds[np-1] = ds * frac (the previous point was already written)
-      LOSW(F_DS, np - 1) = ds * frac;
-      ds = 0;
+      // The step left the altitude range: k_ray_finish clips it to the
+      // range and writes the last point (its cart2geo/geo2cart would hold
+      // registers for the whole march here).  With no point written yet
+      // the entry search ended outside the range, so there is no previous
+      // point to clip to (the reference would read los->z[-1]): the ray has
+      // no LOS points.
+      dscut = ds;
+      break;
     }
 
     // intpol_atm at the LOS point.  The level pair and the quotients of
@@ -278,36 +317,29 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
                              : 0.0;
       LOSW(L.f_cl, np) = aux;
     }
-    // first minimum of z (gsl_stats_min_index semantics, a NaN ends the
-    // search), with the positions of its neighbours
+    // MinTrack::add
     if (!nan_seen) {
       const bool lower = (np == 0) || z < zlow;
       if (lower) zlow = z;
       if (lower || isnan(z)) {
         imin = np;
         if (np > 0)
-          for (int i = 0; i < 3; i++) xbefore[i] = xprev[i];
+          for (int i = 0; i < 3; i++) tl->mt.xbefore[i] = xprev[i];
         after_pending = true;
         nan_seen = isnan(z);
       } else if (after_pending) {
-        for (int i = 0; i < 3; i++) xafter[i] = x[i];
+        for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
         after_pending = false;
       }
     } else if (after_pending) {
-      for (int i = 0; i < 3; i++) xafter[i] = x[i];
+      for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
       after_pending = false;
     }
     if (np == 0)
-      for (int i = 0; i < 3; i++) xfirst[i] = x[i];
-    for (int i = 0; i < 3; i++) xlast[i] = xprev[i] = x[i];
+      for (int i = 0; i < 3; i++) tl->xfirst[i] = x[i];
+    for (int i = 0; i < 3; i++) xprev[i] = x[i];
     zprev = z;
     np++;
-
-    if (stop) {
-      if (M.nsf > 0 && atm->sft > 0) t = atm->sft;
-      R.sft = (stop == 2 ? t : -999);
-      break;
-    }
 
     if (M.refrac && z <= zrefrac) n = 1 + 7.753e-05 * p / t;
     else n = 1;
@@ -315,17 +347,27 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     for (int i = 0; i < 3; i++) ex1[i] = ex0[i] * n;
 
     if (M.refrac && z <= zrefrac) {
-      double pp, tt;
+      // the refractivity at the midpoint and at the three probes displaced
+      // by h along each axis, in the reference's order (each displacement
+      // is undone by subtracting h before the next axis moves); one probe
+      // per iteration, not unrolled, which keeps the march at 168 VGPRs
+      double zq[4], rq[4];
       for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
-      int zg2 = zg;
-      intpol_pt(atm, zdir, zg2, lc, cart2z(xh), &pp, &tt);
-      n = 7.753e-05 * pp / tt;
+      zq[0] = cart2z(xh);
       for (int i = 0; i < 3; i++) {
         xh[i] += h;
-        intpol_pt(atm, zdir, zg2, lc, cart2z(xh), &pp, &tt);
-        ngr[i] = (7.753e-05 * pp / tt - n) / h;
+        zq[i + 1] = cart2z(xh);
         xh[i] -= h;
       }
+      int zg2 = zg;
+#pragma unroll 1
+      for (int k = 0; k < 4; k++) {
+        double pp, tt;
+        intpol_pt(atm, zdir, zg2, lc, zq[k], &pp, &tt);
+        rq[k] = 7.753e-05 * pp / tt;
+      }
+      n = rq[0];
+      for (int i = 0; i < 3; i++) ngr[i] = (rq[i + 1] - n) / h;
     } else
       for (int i = 0; i < 3; i++) ngr[i] = 0;
 
@@ -336,23 +378,109 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     for (int i = 0; i < 3; i++) ex0[i] = ex1[i];
   }
   R.np = np;
-  if (np == 0) {
+  if (np > 0) {
+    for (int i = 0; i < 3; i++) {
+      tl->xprev[i] = xprev[i];
+      tl->xcut[i] = x[i];
+    }
+    tl->zprev = zprev;
+    tl->zcut = z;
+    tl->dscut = dscut;
+    tl->zmin = zmin;
+    tl->zmax = zmax;
+    tl->mt.zlow = zlow;
+    tl->mt.imin = imin;
+    tl->mt.after_pending = after_pending;
+    tl->mt.nan_seen = nan_seen;
+    tl->zdir = zdir;
+  }
+  info[ir] = R;
+}
+
+// The end of a ray's march (k_raytrace): the step that left the altitude
+// range is clipped to it and becomes the last LOS point, as raytrace does
+// (src/jurassic.c:4290-4306), then the tangent point, the solar-incidence
+// factor and the trapezoid step lengths and column densities.  Same
+// operations as in the march, so the same bits.
+__global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, LosView L, RayInfo *info,
+                                                     const RayTail *tail, int blk0) {
+  JUR_FP_EXACT
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + blockIdx.x) * WAVE + lane;
+  if (ir >= B.nray) return;
+  RayInfo R = info[ir];
+  if (R.status != JUR_OK || R.np == 0) return;
+  double *base = L.los + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane;
+  const atm_t *atm = &B.atm[B.ray_prof[ir]];
+  const RayTail T = tail[ir];
+  MinTrack mt = T.mt;
+  int np = R.np;
+
+  // the last step, clipped to the altitude range
+  const int stop = (T.zcut < T.zmin ? 2 : 1);
+  const double frac = ((T.zcut < T.zmin ? T.zmin : T.zmax) - T.zprev) / (T.zcut - T.zprev);
+  double zd, lon, lat, xh[3], x[3];
+  cart2geo(T.xprev, &zd, &lon, &lat);
+  geo2cart(T.zprev, lon, lat, xh);
+  for (int i = 0; i < 3; i++) x[i] = xh[i] + frac * (T.xcut[i] - xh[i]);
+  const double z = cart2z(x);
+  // los->ds[np-1] = ds * frac (the previous point was already written)
+  LOSW(F_DS, np - 1) = T.dscut * frac;
+
+  // intpol_atm at the clipped point with the march's level pair formulas
+  int g = -1;
+  const int ipa = atm_level(atm, T.zdir, z, g);
+  const double x0 = atm->z[ipa], x1 = atm->z[ipa + 1];
+  const double y0 = atm->p[ipa], y1 = atm->p[ipa + 1];
+  const bool lplog = (y1 / y0 > 0);
+  const double pq = lplog ? log(y1 / y0) / (x1 - x0) : (y1 - y0) / (x1 - x0);
+  const double tq = (atm->t[ipa + 1] - atm->t[ipa]) / (x1 - x0);
+  const double dzl = z - x0;
+  const double p = lplog ? y0 * exp(pq * dzl) : y0 + pq * dzl;
+  double t = atm->t[ipa] + tq * dzl;
+  if (np >= L.cap) {
+    R.status = (L.cap >= NLOS) ? JUR_ERR_LOS : 100;
     info[ir] = R;
     return;
   }
+  LOSW(F_Z, np) = z;
+  LOSW(F_P, np) = p;
+  LOSW(F_T, np) = t;
+  for (int ig = 0; ig < M.ng; ig++) {
+    const double q0 = atm->q[ig][ipa];
+    const double qs = (atm->q[ig][ipa + 1] - atm->q[ig][ipa]) / (x1 - x0);
+    LOSW(F_Q0 + ig, np) = q0 + qs * dzl;
+  }
+  for (int iw = 0; iw < M.nw; iw++) {
+    const double k0 = atm->k[iw][ipa];
+    const double ks = (atm->k[iw][ipa + 1] - atm->k[iw][ipa]) / (x1 - x0);
+    LOSW(L.f_k + iw, np) = k0 + ks * dzl;
+  }
+  LOSW(F_DS, np) = 0.0;
+  if (L.f_cl >= 0) {
+    const double aux = (M.ncl > 0 && atm->cldz > 0)
+                           ? exp(-0.5 * (((z - atm->clz) / atm->cldz) * ((z - atm->clz) / atm->cldz)))
+                           : 0.0;
+    LOSW(L.f_cl, np) = aux;
+  }
+  mt.add(np, z, x, T.xprev);
+  np++;
+  if (M.nsf > 0 && atm->sft > 0) t = atm->sft;
+  R.sft = (stop == 2 ? t : -999);
+  R.np = np;
 
   // tangent_point at the first minimum imin found during the march
   double zl, lonl, latl;
-  cart2geo(xlast, &zl, &lonl, &latl);
-  if (imin <= 0 || imin >= np - 1) {
+  cart2geo(x, &zl, &lonl, &latl);
+  if (mt.imin <= 0 || mt.imin >= np - 1) {
     R.tpz = LOSW(F_Z, np - 1);
     R.tplon = lonl;
     R.tplat = latl;
   } else {
-    const double yy0 = LOSW(F_Z, imin - 1);
-    const double yy1 = LOSW(F_Z, imin);
-    const double yy2 = LOSW(F_Z, imin + 1);
-    const double d1 = LOSW(F_DS, imin), d2 = LOSW(F_DS, imin + 1);
+    const double yy0 = LOSW(F_Z, mt.imin - 1);
+    const double yy1 = LOSW(F_Z, mt.imin);
+    const double yy2 = LOSW(F_Z, mt.imin + 1);
+    const double d1 = LOSW(F_DS, mt.imin), d2 = LOSW(F_DS, mt.imin + 1);
     const double x1 = sqrt(d1 * d1 - (yy1 - yy0) * (yy1 - yy0));
     const double x2 = x1 + sqrt(d2 * d2 - (yy2 - yy1) * (yy2 - yy1));
     const double a = 1 / (x1 - x2) * (-(yy0 - yy1) / x1 + (yy0 - yy2) / x2);
@@ -362,8 +490,8 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     double v[3], v0[3], v2[3], dummy;
     R.tpz = a * xt * xt + b * xt + c;
     double lon0, lat0, lon2, lat2;
-    cart2geo(xbefore, &dummy, &lon0, &lat0);
-    cart2geo(xafter, &dummy, &lon2, &lat2);
+    cart2geo(mt.xbefore, &dummy, &lon0, &lat0);
+    cart2geo(mt.xafter, &dummy, &lon2, &lat2);
     geo2cart(yy0, lon0, lat0, v0);
     geo2cart(yy2, lon2, lat2, v2);
     for (int i = 0; i < 3; i++) v[i] = lin(0.0, v0[i], x2, v2[i], xt);
@@ -372,12 +500,12 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
 
   // solar incidence (formod_pencil, src/jurassic.c:3297-3326)
   if (M.sftype >= 3 && R.sft > 0) {
-    const double sza2 = (M.sfsza < 0) ? sza(otime, lonl, latl) : M.sfsza;
+    const double sza2 = (M.sfsza < 0) ? sza(B.ray_geom[(size_t)7 * ir], lonl, latl) : M.sfsza;
     if (sza2 < 89.999) {
       double x0[3], x1[3];
       geo2cart(LOSW(F_Z, np - 1), lonl, latl, x0);
       double lonf, latf, zf;
-      cart2geo(xfirst, &zf, &lonf, &latf);
+      cart2geo(T.xfirst, &zf, &lonf, &latf);
       geo2cart(LOSW(F_Z, 0), lonf, latf, x1);
       for (int i = 0; i < 3; i++) x1[i] -= x0[i];
       const double cosa = (x0[0] * x1[0] + x0[1] * x1[1] + x0[2] * x1[2]) /
@@ -436,10 +564,14 @@ __global__ __launch_bounds__(WAVE) void k_hydrostatic(atm_t *atm, int nprof, dou
   }
 }
 
-void launch_raytrace(const ModelView &M, const BatchView &B, const LosView &L, RayInfo *info,
+size_t ray_tail_bytes() { return sizeof(RayTail); }
+
+// tail: ray_tail_bytes() per ray of the batch
+void launch_raytrace(const ModelView &M, const BatchView &B, const LosView &L, RayInfo *info, void *tail,
                      int blk0, int nblk, hipStream_t s) {
   const size_t lds = (size_t)(6 + 2 * M.ng + 2 * M.nw) * WAVE * sizeof(double);
-  hipLaunchKernelGGL(k_raytrace, dim3(nblk), dim3(WAVE), lds, s, M, B, L, info, blk0);
+  hipLaunchKernelGGL(k_raytrace, dim3(nblk), dim3(WAVE), lds, s, M, B, L, info, (RayTail *)tail, blk0);
+  hipLaunchKernelGGL(k_ray_finish, dim3(nblk), dim3(WAVE), 0, s, M, B, L, info, (const RayTail *)tail, blk0);
 }
 
 void launch_hydrostatic(atm_t *atm, int nprof, double hydz, int ig_h2o, hipStream_t s) {
