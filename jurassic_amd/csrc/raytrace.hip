@@ -14,8 +14,10 @@ namespace jur {
 // strictly monotone altitude grid, a walk from the previous point's level
 // (the LOS moves by a fraction of a level per step), which returns the same
 // index.
-__device__ __forceinline__ int atm_level(const atm_t *atm, int zdir, double z, int &g) {
-  g = (zdir != 0 && g >= 0) ? irr_walk(atm->z, atm->np, z, g, zdir) : locate_irr(atm->z, atm->np, z);
+// anp is atm->np, read once by the caller: a load of it in the march would
+// wait (vmcnt) for the LOS stores issued before it.
+__device__ __forceinline__ int atm_level(const atm_t *atm, int anp, int zdir, double z, int &g) {
+  g = (zdir != 0 && g >= 0) ? irr_walk(atm->z, anp, z, g, zdir) : locate_irr(atm->z, anp, z);
   return g;
 }
 
@@ -30,16 +32,16 @@ struct LevelCache {
   bool plog;
 };
 
-__device__ __forceinline__ void intpol_pt(const atm_t *atm, int zdir, int &g, LevelCache &lc,
+__device__ __forceinline__ void intpol_pt(const atm_t *atm, int anp, int zdir, int &g, LevelCache &lc,
                                           double z, double *p, double *t) {
   JUR_FP_EXACT
   // the walk from g returns g when z is still between the cached levels
   bool stay = false;
   if (lc.ip >= 0 && g == lc.ip) {
-    if (zdir > 0) stay = (g == 0 || !(lc.z0 > z)) && (g == atm->np - 2 || lc.z1 > z);
-    else if (zdir < 0) stay = (g == 0 || !(lc.z0 <= z)) && (g == atm->np - 2 || lc.z1 <= z);
+    if (zdir > 0) stay = (g == 0 || !(lc.z0 > z)) && (g == anp - 2 || lc.z1 > z);
+    else if (zdir < 0) stay = (g == 0 || !(lc.z0 <= z)) && (g == anp - 2 || lc.z1 <= z);
   }
-  const int ip = stay ? g : atm_level(atm, zdir, z, g);
+  const int ip = stay ? g : atm_level(atm, anp, zdir, z, g);
   if (ip != lc.ip) {
     lc.ip = ip;
     const double x0 = atm->z[ip], x1 = atm->z[ip + 1];
@@ -165,7 +167,8 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   // monotonicity of the altitude grid
   zmin = zmax = atm->z[0];
   int zasc = 1, zdesc = 1;
-  for (int i = 0; i < atm->np; i++) {
+  const int anp = atm->np;
+  for (int i = 0; i < anp; i++) {
     const double v = atm->z[i];
     if (v < zmin) zmin = v;
     if (v > zmax) zmax = v;
@@ -266,11 +269,11 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       bool stay = false;
       if (lip >= 0) {
         const double za = LV(0), zb = LV(1);
-        if (zdir > 0) stay = (lip == 0 || !(za > z)) && (lip == atm->np - 2 || zb > z);
-        else if (zdir < 0) stay = (lip == 0 || !(za <= z)) && (lip == atm->np - 2 || zb <= z);
+        if (zdir > 0) stay = (lip == 0 || !(za > z)) && (lip == anp - 2 || zb > z);
+        else if (zdir < 0) stay = (lip == 0 || !(za <= z)) && (lip == anp - 2 || zb <= z);
       }
       if (!stay) {
-        const int ipa = atm_level(atm, zdir, z, zg);
+        const int ipa = atm_level(atm, anp, zdir, z, zg);
         if (ipa != lip) {
           lip = ipa;
           const double x0 = atm->z[ipa], x1 = atm->z[ipa + 1];
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
 #pragma unroll 1
       for (int k = 0; k < 4; k++) {
         double pp, tt;
-        intpol_pt(atm, zdir, zg2, lc, zq[k], &pp, &tt);
+        intpol_pt(atm, anp, zdir, zg2, lc, zq[k], &pp, &tt);
         rq[k] = 7.753e-05 * pp / tt;
       }
       n = rq[0];
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
 
   // intpol_atm at the clipped point with the march's level pair formulas
   int g = -1;
-  const int ipa = atm_level(atm, T.zdir, z, g);
+  const int ipa = atm_level(atm, atm->np, T.zdir, z, g);
   const double x0 = atm->z[ipa], x1 = atm->z[ipa + 1];
   const double y0 = atm->p[ipa], y1 = atm->p[ipa + 1];
   const bool lplog = (y1 / y0 > 0);
