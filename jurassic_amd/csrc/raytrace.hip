@@ -352,9 +352,8 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     if (M.refrac && z <= zrefrac) {
       // the refractivity at the midpoint and at the three probes displaced
       // by h along each axis, in the reference's order (each displacement
-      // is undone by subtracting h before the next axis moves); one probe
-      // per iteration, not unrolled, which keeps the march at 168 VGPRs
-      double zq[4], rq[4];
+      // is undone by subtracting h before the next axis moves)
+      double zq[4], pq[4], tq[4];
       for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
       zq[0] = cart2z(xh);
       for (int i = 0; i < 3; i++) {
@@ -363,14 +362,25 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
         xh[i] -= h;
       }
       int zg2 = zg;
-#pragma unroll 1
+      // all four between the probes' cached levels (intpol_pt's walk would
+      // return them for each): the cached formula for each, side by side;
+      // otherwise intpol_pt probe by probe (not unrolled)
+      bool all = lc.ip >= 0 && zg2 == lc.ip && zdir != 0;
       for (int k = 0; k < 4; k++) {
-        double pp, tt;
-        intpol_pt(atm, anp, zdir, zg2, lc, zq[k], &pp, &tt);
-        rq[k] = 7.753e-05 * pp / tt;
+        if (zdir > 0) all = all && (zg2 == 0 || !(lc.z0 > zq[k])) && (zg2 == anp - 2 || lc.z1 > zq[k]);
+        else all = all && (zg2 == 0 || !(lc.z0 <= zq[k])) && (zg2 == anp - 2 || lc.z1 <= zq[k]);
       }
-      n = rq[0];
-      for (int i = 0; i < 3; i++) ngr[i] = (rq[i + 1] - n) / h;
+      if (all) {
+        for (int k = 0; k < 4; k++) {
+          pq[k] = lc.plog ? lc.p0 * exp(lc.pq * (zq[k] - lc.z0)) : lc.p0 + lc.pq * (zq[k] - lc.z0);
+          tq[k] = lc.t0 + lc.tq * (zq[k] - lc.z0);
+        }
+      } else {
+#pragma unroll 1
+        for (int k = 0; k < 4; k++) intpol_pt(atm, anp, zdir, zg2, lc, zq[k], &pq[k], &tq[k]);
+      }
+      n = 7.753e-05 * pq[0] / tq[0];
+      for (int i = 0; i < 3; i++) ngr[i] = (7.753e-05 * pq[i + 1] / tq[i + 1] - n) / h;
     } else
       for (int i = 0; i < 3; i++) ngr[i] = 0;
 
