@@ -123,6 +123,8 @@ struct RayTail {
   double xfirst[3];
   double zprev, zcut, dscut;  // altitudes of both, step length
   double zmin, zmax;
+  double ds1, ds2;            // raw step lengths of the last two points written
+  double ds_imin, ds_imin1;   // raw step lengths at the first minimum and after it
   MinTrack mt;
   int zdir;
 };
@@ -239,6 +241,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   int imin = 0;
   bool after_pending = false, nan_seen = false;
   double dscut = 0;
+  double ds1 = 0, ds2 = 0;  // raw step lengths of the last two points written
   for (;;) {
     double ds = M.rayds;
     if (M.raydz > 0) {
@@ -307,37 +310,52 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       info[ir] = R;
       return;
     }
+    // the trapezoid segment length of the point and its column densities
+    // (src/jurassic.c:4410-4419) from its raw step length ds and the
+    // previous point's: 0.5 (ds_prev + ds), the first point 0.5 ds (the
+    // clip of the last step changes the last two points, which k_ray_finish
+    // rewrites)
+    const double dtr = np == 0 ? ds * 0.5 : 0.5 * (ds1 + ds);
     LOSW(F_Z, np) = z;
     LOSW(F_P, np) = p;
     LOSW(F_T, np) = t;
-    for (int ig = 0; ig < M.ng; ig++) LOSW(F_Q0 + ig, np) = LV(6 + ig) + LV(6 + M.ng + ig) * dzl;
+    for (int ig = 0; ig < M.ng; ig++) {
+      const double q = LV(6 + ig) + LV(6 + M.ng + ig) * dzl;
+      LOSW(F_Q0 + ig, np) = q;
+      LOSW(L.f_u + ig, np) = 10 * q * p / (JUR_KB * t) * dtr;
+    }
     for (int iw = 0; iw < M.nw; iw++)
       LOSW(L.f_k + iw, np) = LV(6 + 2 * M.ng + iw) + LV(6 + 2 * M.ng + M.nw + iw) * dzl;
-    LOSW(F_DS, np) = ds;
+    LOSW(F_DS, np) = dtr;
     if (L.f_cl >= 0) {
       const double aux = (M.ncl > 0 && atm->cldz > 0)
                              ? exp(-0.5 * (((z - atm->clz) / atm->cldz) * ((z - atm->clz) / atm->cldz)))
                              : 0.0;
       LOSW(L.f_cl, np) = aux;
     }
-    // MinTrack::add
+    // MinTrack::add, with the raw step lengths at the minimum and after it
     if (!nan_seen) {
       const bool lower = (np == 0) || z < zlow;
       if (lower) zlow = z;
       if (lower || isnan(z)) {
         imin = np;
+        tl->ds_imin = ds;
         if (np > 0)
           for (int i = 0; i < 3; i++) tl->mt.xbefore[i] = xprev[i];
         after_pending = true;
         nan_seen = isnan(z);
       } else if (after_pending) {
         for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
+        tl->ds_imin1 = ds;
         after_pending = false;
       }
     } else if (after_pending) {
       for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
+      tl->ds_imin1 = ds;
       after_pending = false;
     }
+    ds2 = ds1;
+    ds1 = ds;
     if (np == 0)
       for (int i = 0; i < 3; i++) tl->xfirst[i] = x[i];
     for (int i = 0; i < 3; i++) xprev[i] = x[i];
@@ -401,6 +419,8 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     tl->dscut = dscut;
     tl->zmin = zmin;
     tl->zmax = zmax;
+    tl->ds1 = ds1;
+    tl->ds2 = ds2;
     tl->mt.zlow = zlow;
     tl->mt.imin = imin;
     tl->mt.after_pending = after_pending;
@@ -437,8 +457,9 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
   geo2cart(T.zprev, lon, lat, xh);
   for (int i = 0; i < 3; i++) x[i] = xh[i] + frac * (T.xcut[i] - xh[i]);
   const double z = cart2z(x);
-  // los->ds[np-1] = ds * frac (the previous point was already written)
-  LOSW(F_DS, np - 1) = T.dscut * frac;
+  // los->ds[np-1] = ds * frac: the raw step length of the last point written
+  const int npm = np;
+  const double clip = T.dscut * frac;
 
   // intpol_atm at the clipped point with the march's level pair formulas
   int g = -1;
@@ -456,20 +477,33 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
     info[ir] = R;
     return;
   }
+  // trapezoid segment lengths (src/jurassic.c:4410-4411) of the last point
+  // written, from its clipped raw length, and of this one (raw length 0);
+  // the march wrote those of the others
+  const double dtr0 = npm == 1 ? clip * 0.5 : 0.5 * (T.ds2 + clip);
+  const double dtr1 = 0.5 * (clip + 0.0);
+  LOSW(F_DS, npm - 1) = dtr0;
+  {
+    const double p0 = LOSW(F_P, npm - 1), t0 = LOSW(F_T, npm - 1);
+    for (int ig = 0; ig < M.ng; ig++)
+      LOSW(L.f_u + ig, npm - 1) = 10 * LOSW(F_Q0 + ig, npm - 1) * p0 / (JUR_KB * t0) * dtr0;
+  }
   LOSW(F_Z, np) = z;
   LOSW(F_P, np) = p;
   LOSW(F_T, np) = t;
   for (int ig = 0; ig < M.ng; ig++) {
     const double q0 = atm->q[ig][ipa];
     const double qs = (atm->q[ig][ipa + 1] - atm->q[ig][ipa]) / (x1 - x0);
-    LOSW(F_Q0 + ig, np) = q0 + qs * dzl;
+    const double q = q0 + qs * dzl;
+    LOSW(F_Q0 + ig, np) = q;
+    LOSW(L.f_u + ig, np) = 10 * q * p / (JUR_KB * t) * dtr1;
   }
   for (int iw = 0; iw < M.nw; iw++) {
     const double k0 = atm->k[iw][ipa];
     const double ks = (atm->k[iw][ipa + 1] - atm->k[iw][ipa]) / (x1 - x0);
     LOSW(L.f_k + iw, np) = k0 + ks * dzl;
   }
-  LOSW(F_DS, np) = 0.0;
+  LOSW(F_DS, np) = dtr1;
   if (L.f_cl >= 0) {
     const double aux = (M.ncl > 0 && atm->cldz > 0)
                            ? exp(-0.5 * (((z - atm->clz) / atm->cldz) * ((z - atm->clz) / atm->cldz)))
@@ -493,7 +527,12 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
     const double yy0 = LOSW(F_Z, mt.imin - 1);
     const double yy1 = LOSW(F_Z, mt.imin);
     const double yy2 = LOSW(F_Z, mt.imin + 1);
-    const double d1 = LOSW(F_DS, mt.imin), d2 = LOSW(F_DS, mt.imin + 1);
+    // the raw step lengths at imin and imin + 1 (tangent_point runs
+    // before the trapezoid rule in raytrace)
+    auto rawds = [&](int i) {
+      return i == npm ? 0.0 : (i == npm - 1 ? clip : (i == mt.imin ? T.ds_imin : T.ds_imin1));
+    };
+    const double d1 = rawds(mt.imin), d2 = rawds(mt.imin + 1);
     const double x1 = sqrt(d1 * d1 - (yy1 - yy0) * (yy1 - yy0));
     const double x2 = x1 + sqrt(d2 * d2 - (yy2 - yy1) * (yy2 - yy1));
     const double a = 1 / (x1 - x2) * (-(yy0 - yy1) / x1 + (yy0 - yy2) / x2);
@@ -529,15 +568,6 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
     }
   }
 
-  // trapezoid segment lengths and column densities (src/jurassic.c:4410-4419),
-  // after the tangent point, which uses the raw step lengths
-  for (int ip = np - 1; ip >= 1; ip--) LOSW(F_DS, ip) = 0.5 * (LOSW(F_DS, ip - 1) + LOSW(F_DS, ip));
-  LOSW(F_DS, 0) *= 0.5;
-  for (int ip = 0; ip < np; ip++) {
-    const double p = LOSW(F_P, ip), t = LOSW(F_T, ip), ds = LOSW(F_DS, ip);
-    for (int ig = 0; ig < M.ng; ig++)
-      LOSW(L.f_u + ig, ip) = 10 * LOSW(F_Q0 + ig, ip) * p / (JUR_KB * t) * ds;
-  }
   info[ir] = R;
 }
 
