@@ -23,8 +23,9 @@
 //   H_i = G_i s_i,   d rad / d beta_i = -H_i ds_i
 //   d rad / d e_g,i (through s_i and s_i+1) = (H_i+1 - H_i) / P_g,i
 //   L_g,i = that + L_g,i+1 dE_g,i+1 / d e_g,i         (EGA chain)
-// The table partials dE/d(e_prev, p, T, u) are recomputed from the stored
-// path emissivities with the slopes of the brackets that the lookup used
+// The table partials dE/d(e_prev, p, T, u) are those of the brackets that
+// the lookup used: the FP32 EGA lookup over monotone tables stores them per
+// step; otherwise they are recomputed from the stored path emissivities
 // (piecewise-linear in u, linear in T, linear (EGA) or logarithmic (CGA) in
 // p; zero where eps00 is clamped).  Continua and the source function are
 // differentiated analytically.
@@ -37,7 +38,7 @@
 // forward sweep (tau_i, D_i) and the reverse sweep of the accumulation (H_i,
 // kept in place of tau_i).  k_adj_chain, one wavefront per (block, channel,
 // emitter) like the lookup kernels: the emitter's chain in reverse with its
-// table partials recomputed.  A lane sums the contributions of the current
+// table partials (stored by the FP32 EGA lookup, recomputed otherwise).  A lane sums the contributions of the current
 // level pair in registers and adds them to its row of the block tile
 // kt[(block, channel)][column][lane] when the pair changes, so tile accesses
 // of a wavefront are contiguous; T and p columns, which every emitter chain
@@ -96,22 +97,6 @@ __device__ __forceinline__ double u_vs(const float4 &r, double x, double &u) {
   const double s = (double)((r.z - r.x) / (r.w - r.y));
   u = (double)r.x + s * (x - (double)r.y);
   return s;
-}
-
-// FP32 path: eps_from_rec_f with its slope.
-__device__ __forceinline__ float eps_rec_f_vs(const float4 &r, double x, float &s) {
-  if (x < (double)r.x) {
-    s = fdiv(r.y, r.x);
-    return s * (float)x;
-  }
-  if (x > (double)r.z) {
-    const float a = fdiv(__logf(1.f - r.w), r.z);
-    const float ex = __expf(a * (float)x);
-    s = ex > 0.f ? -a * ex : 0.f;
-    return 1.f - ex;
-  }
-  s = fdiv(r.w - r.y, r.z - r.x);
-  return r.y + s * (float)(x - (double)r.x);
 }
 
 // FP32 path: col_eps_bis and col_u_bis with their slopes.
@@ -255,50 +240,10 @@ __device__ __forceinline__ LkPart lookup_partials(const TabView &T, int tix, int
   return d;
 }
 
+
 // FP32 path: partials of one chain step of k_lookup_f32, with its float
 // cell weights, its knot-relative update inside a bracket and its
 // reciprocal-multiply slopes.
-// General FP32 EGA step on one monotone column (ega_step_slow) with its
-// partials de = d eps / d ep and se = d eps / d u_seg; the inverse search
-// starts at g and g returns its bracket.
-__device__ __forceinline__ float ega_corner_f32_d(const float4 *col, int n, float ep, double x, int &g,
-                                                  float &de, float &se) {
-  float4 r;
-  const int i = col_walk_f<1>(col, n, ep, g, r);
-  g = i;
-  float e, su;
-  if (ep < r.y) {
-    su = fdiv(r.x, r.y);
-    const double arg = (double)(su * ep) + x;
-    float4 r2;
-    col_walk_r<0>(col, n, arg, i, r, r2);
-    e = eps_rec_f_vs(r2, arg, se);
-    de = se * su;
-  } else if (ep > r.w) {
-    // e = 1 - (1 - ep) exp(a u_seg), as ega_step_slow evaluates it
-    const float a = fdiv(__logf(1.f - r.w), r.z);
-    const float ex = __expf(a * (float)x);
-    e = 1.f - (1.f - ep) * ex;
-    se = ex > 0.f ? -a * (1.f - ep) * ex : 0.f;
-    de = ex;
-  } else {
-    const float dur = r.z - r.x, der = r.w - r.y;
-    su = fdiv(dur, der);
-    const float off = (ep - r.y) * su + (float)x;
-    if (off < dur) {
-      se = fdiv(der, dur);
-      e = r.y + off * se;
-    } else {
-      const double arg = (double)r.x + (double)off;
-      float4 r2;
-      col_walk_r<0>(col, n, arg, i, r, r2);
-      e = eps_rec_f_vs(r2, arg, se);
-    }
-    de = se * su;
-  }
-  return e;
-}
-
 template <bool CGA>
 __device__ __forceinline__ LkPart lookup_partials_f32(const TabView &T, int tix, int npt, int mono, int pdir,
                                                       float ep, double pq, float tq, double x, int &ipr,
@@ -951,35 +896,23 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
   if constexpr (FAST) {
     static_assert(F32 && !CGA, "the fast chain is the FP32 EGA one");
     {
-      // FP32 EGA on monotone columns: the cell and its T brackets are kept
-      // from point to point as in k_lookup_f32, and each corner starts from
-      // the record pair below the previous point's inverse bracket (walking
-      // backwards the path emissivity falls), which covers the common case
-      // without a search.
-      const TabView &T = M.tab;
-      const double *pg = T.p + (size_t)tix * T.TP;
-      const int *ntp = T.nt + (size_t)tix * T.TP;
-      // wave-uniform bases with 32-bit per-lane offsets (as in k_lookup_f32)
-      const size_t tcols = (size_t)T.TP * T.TT;
-      const double *tgb = T.t + (size_t)tix * tcols;
-      const int *nub = T.nu + (size_t)tix * tcols;
-      const float4 *recb = T.rec + (size_t)tix * tcols * T.TU;
+      // FP32 EGA on monotone tables: k_lookup_f32 stored each step's
+      // partials, so the chain is the recurrence alone.  Each point's
+      // inputs (partials, LOS values, the emissivity before it, H) are
+      // loaded one point ahead.
       const double *losb = L.los + (size_t)bl * L.cap * L.nfield * WAVE;
       const unsigned nfield = L.nfield;
       const float *epsg = (const float *)E.eps + E.off[bl] + ((size_t)id * ng + g) * npb * WAVE;
+      const float4 *pab = E.part + E.off[bl] + ((size_t)id * ng + g) * npb * WAVE;
       const double *scb = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE;
       const unsigned nscr = J.nscr;
 #define LOSQ(f, ip) ld32(losb, ((unsigned)(ip) * nfield + (unsigned)(f)) * WAVE + lane)
 #define EPSQ(ip) ld32(epsg, (unsigned)(ip) * WAVE + lane)
-      int it0 = -1, it1 = -1, nt0 = 0, nt1 = 0, n00 = 0, n01 = 0, n10 = 0, n11 = 0;
-      double pa = 0, pb = 0;
-      float wp_scale = 0, g00 = 0, gi0 = 0, g10 = 0, gi1 = 0, ta0 = 0, dt0 = 0, ta1 = 0, dt1 = 0;
-      unsigned tq0 = 0;  // first T-grid entry of p row ipr
-      // the point's inputs (loaded during the previous point: LOS, the
-      // emissivity before it, and its H and geometry for the chain rule)
       double p = 0, t = 0, useg = 0, Hc = 0, zc = 0, dsc = 0;
       float epf = 0.f;
+      float4 dc = make_float4(1.f, 0.f, 0.f, 0.f);
       if (nef > 0) {
+        dc = ld32(pab, (unsigned)(nef - 1) * WAVE + lane);
         p = LOSQ(F_P, nef - 1);
         t = LOSQ(F_T, nef - 1);
         useg = LOSQ(fu, nef - 1);
@@ -990,162 +923,12 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
       }
       for (int ip = nef - 1; ip >= 0; ip--) {
         const int ipn = ip > 0 ? ip - 1 : 0;
-        const float tq = (float)t;
-        bool keep = false;
-        if (ipr >= 0 && pdir != 0) {
-          if (pdir > 0)
-            keep = (ipr == 0 || !(pa > p)) && (ipr == npt - 2 || pb > p);
-          else
-            keep = (ipr == 0 || !(pa <= p)) && (ipr == npt - 2 || pb <= p);
-        }
-        if (!keep) {
-          const int iprev = ipr;
-          ipr = (pdir != 0 && ipr >= 0) ? irr_walk(pg, npt, p, ipr, pdir) : locate_irr(pg, npt, p);
-          if (ipr == iprev + 1) {
-            const int a0 = gi[0], a1 = gi[1];
-            gi[0] = gi[2];
-            gi[1] = gi[3];
-            gi[2] = 2 * gi[2] - a0;
-            gi[3] = 2 * gi[3] - a1;
-          } else if (ipr == iprev - 1) {
-            const int a2 = gi[2], a3 = gi[3];
-            gi[2] = gi[0];
-            gi[3] = gi[1];
-            gi[0] = 2 * gi[0] - a2;
-            gi[1] = 2 * gi[1] - a3;
-          }
-          pa = pg[ipr];
-          pb = pg[ipr + 1];
-          wp_scale = frcp((float)(pb - pa));
-          nt0 = ntp[ipr];
-          nt1 = ntp[ipr + 1];
-          tq0 = (unsigned)ipr * T.TT;
-          const double t00 = ld32(tgb, tq0), t01 = ld32(tgb, tq0 + 1);
-          const double t10 = ld32(tgb, tq0 + T.TT), t11 = ld32(tgb, tq0 + T.TT + 1);
-          g00 = (float)t00;
-          gi0 = frcp((float)(t01 - t00));
-          g10 = (float)t10;
-          gi1 = frcp((float)(t11 - t10));
-          it0 = it1 = -1;
-        }
-        LkPart d{1, 0, 0, 0};
-        double pn = p, tn = t, un = useg, Hx = Hc, zx = zc, dsx = dsc;
-        float en_f = epf;
-        bool loaded = false;
-        if (nt0 >= 2 && nt1 >= 2) {
-          int j0 = (int)((tq - g00) * gi0);
-          j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
-          int j1 = (int)((tq - g10) * gi1);
-          j1 = j1 < 0 ? 0 : (j1 > nt1 - 2 ? nt1 - 2 : j1);
-          if (j0 != it0) {
-            if (j0 == it0 + 1) {
-              const int a0 = gi[0];
-              gi[0] = gi[1];
-              gi[1] = 2 * gi[1] - a0;
-            } else if (j0 == it0 - 1) {
-              const int a1 = gi[1];
-              gi[1] = gi[0];
-              gi[0] = 2 * gi[0] - a1;
-            }
-            it0 = j0;
-            const double a = ld32(tgb, tq0 + it0), b = ld32(tgb, tq0 + it0 + 1);
-            ta0 = (float)a;
-            dt0 = frcp((float)(b - a));
-            n00 = ld32(nub, tq0 + it0);
-            n01 = ld32(nub, tq0 + it0 + 1);
-          }
-          if (j1 != it1) {
-            if (j1 == it1 + 1) {
-              const int a2 = gi[2];
-              gi[2] = gi[3];
-              gi[3] = 2 * gi[3] - a2;
-            } else if (j1 == it1 - 1) {
-              const int a3 = gi[3];
-              gi[3] = gi[2];
-              gi[2] = 2 * gi[2] - a3;
-            }
-            it1 = j1;
-            const double a = ld32(tgb, tq0 + T.TT + it1), b = ld32(tgb, tq0 + T.TT + it1 + 1);
-            ta1 = (float)a;
-            dt1 = frcp((float)(b - a));
-            n10 = ld32(nub, tq0 + T.TT + it1);
-            n11 = ld32(nub, tq0 + T.TT + it1 + 1);
-          }
-          if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
-            unsigned ko[4];
-            ko[0] = (tq0 + it0) * T.TU;
-            ko[1] = ko[0] + T.TU;
-            ko[2] = (tq0 + T.TT + it1) * T.TU;
-            ko[3] = ko[2] + T.TU;
-            const float4 *kc[4] = {recb + ko[0], recb + ko[1], recb + ko[2], recb + ko[3]};
-            const int nc[4] = {n00, n01, n10, n11};
-            float4 r[4], rn[4];
-            int gs[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-              int g0 = gi[c] - 1;
-              g0 = g0 < 0 ? 0 : (g0 > nc[c] - 2 ? nc[c] - 2 : g0);
-              gs[c] = g0;
-              r[c] = ld32(recb, ko[c] + g0);
-              rn[c] = ld32(recb, ko[c] + (g0 + 1 < nc[c] - 2 ? g0 + 1 : nc[c] - 2));
-            }
-            __builtin_amdgcn_sched_barrier(0);
-            pn = LOSQ(F_P, ipn);
-            tn = LOSQ(F_T, ipn);
-            un = LOSQ(fu, ipn);
-            en_f = ipn > 0 ? EPSQ(ipn - 1) : 0.f;
-            Hx = ld32(scb, (unsigned)ipn * nscr * WAVE + lane);
-            zx = LOSQ(F_Z, ipn);
-            dsx = LOSQ(F_DS, ipn);
-            loaded = true;
-            __builtin_amdgcn_sched_barrier(0);
-            const float us = (float)useg;
-            float ec[4], dep[4], deu[4];
-            bool slow[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-              int gg = gs[c];
-              ega_step_fast2_d(r[c], rn[c], nc[c], epf, us, gg, ec[c], dep[c], deu[c], slow[c]);
-              gi[c] = slow[c] ? gi[c] : gg;
-            }
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-              if (slow[c]) ec[c] = ega_corner_f32_d(kc[c], nc[c], epf, useg, gi[c], dep[c], deu[c]);
-            }
-            const float w0 = (tq - ta0) * dt0, w1 = (tq - ta1) * dt1;
-            const float dwp = wp_scale;
-            const float wp = (float)(p - pa) * dwp;
-            const float q0 = 1.f - wp, q1 = wp;
-            const float cw[4] = {q0 * (1.f - w0), q0 * w0, q1 * (1.f - w1), q1 * w1};
-            const float tw[4] = {-q0 * dt0, q0 * dt0, -q1 * dt1, q1 * dt1};
-            const float pw[4] = {-(1.f - w0) * dwp, -w0 * dwp, (1.f - w1) * dwp, w1 * dwp};
-            float sdep = 0.f, sdu = 0.f, sdt = 0.f, sdp = 0.f;
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-              sdep += cw[c] * dep[c];
-              sdu += cw[c] * deu[c];
-              sdt += tw[c] * ec[c];
-              sdp += pw[c] * ec[c];
-            }
-            const float a = ec[0] + (ec[1] - ec[0]) * w0;
-            const float b = ec[2] + (ec[3] - ec[2]) * w1;
-            const float en = a + (b - a) * wp;
-            if (en < 0.f || en > 1.f) {
-              d = LkPart{0, 0, 0, 0};  // clamped: constant
-            } else {
-              d = LkPart{sdep, sdp, sdt, sdu};
-            }
-          }
-        }
-        if (!loaded) {
-          pn = LOSQ(F_P, ipn);
-          tn = LOSQ(F_T, ipn);
-          un = LOSQ(fu, ipn);
-          en_f = ipn > 0 ? EPSQ(ipn - 1) : 0.f;
-          Hx = ld32(scb, (unsigned)ipn * nscr * WAVE + lane);
-          zx = LOSQ(F_Z, ipn);
-          dsx = LOSQ(F_DS, ipn);
-        }
+        const float4 dx = ld32(pab, (unsigned)ipn * WAVE + lane);
+        const double pn = LOSQ(F_P, ipn), tn = LOSQ(F_T, ipn), un = LOSQ(fu, ipn);
+        const float en_f = ipn > 0 ? EPSQ(ipn - 1) : 0.f;
+        const double Hx = ld32(scb, (unsigned)ipn * nscr * WAVE + lane);
+        const double zx = LOSQ(F_Z, ipn), dsx = LOSQ(F_DS, ipn);
+        const LkPart d{dc.x, dc.y, dc.z, dc.w};
         const double H = Hc, z = zc, ds = dsc;
         const double trp = 1.0 - (double)epf;
         // the chain rule's divisions by the transmittance, T and p as float
@@ -1169,6 +952,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
         Hc = Hx;
         zc = zx;
         dsc = dsx;
+        dc = dx;
       }
       W.flush(xm, qi, kt);
 #undef LOSQ
@@ -1282,7 +1066,7 @@ static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L,
   if (M.ng == 0) return;
   const dim3 grid(nb8 * M.nd * M.ng), blk(WAVE);
   if constexpr (!CGA && sizeof(SF) == sizeof(float)) {
-    if (M.tab.all_mono) {
+    if (E.part) {
       hipLaunchKernelGGL((k_adj_chain<false, float, true>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
       return;
     }

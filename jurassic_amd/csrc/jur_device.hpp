@@ -429,6 +429,63 @@ __device__ __forceinline__ float ega_step_slow(const float4 *col, int n, float e
   return eps_from_rec_f(r2, arg);
 }
 
+// FP32 path: eps_from_rec_f with its slope.
+__device__ __forceinline__ float eps_rec_f_vs(const float4 &r, double x, float &s) {
+  if (x < (double)r.x) {
+    s = fdiv(r.y, r.x);
+    return s * (float)x;
+  }
+  if (x > (double)r.z) {
+    const float a = fdiv(__logf(1.f - r.w), r.z);
+    const float ex = __expf(a * (float)x);
+    s = ex > 0.f ? -a * ex : 0.f;
+    return 1.f - ex;
+  }
+  s = fdiv(r.w - r.y, r.z - r.x);
+  return r.y + s * (float)(x - (double)r.x);
+}
+
+// General FP32 EGA step on one monotone column (ega_step_slow) with its
+// partials de = d eps / d ep and se = d eps / d u_seg; the inverse search
+// starts at g and g returns its bracket.
+__device__ __forceinline__ float ega_corner_f32_d(const float4 *col, int n, float ep, double x, int &g,
+                                                  float &de, float &se) {
+  float4 r;
+  const int i = col_walk_f<1>(col, n, ep, g, r);
+  g = i;
+  float e, su;
+  if (ep < r.y) {
+    su = fdiv(r.x, r.y);
+    const double arg = (double)(su * ep) + x;
+    float4 r2;
+    col_walk_r<0>(col, n, arg, i, r, r2);
+    e = eps_rec_f_vs(r2, arg, se);
+    de = se * su;
+  } else if (ep > r.w) {
+    // e = 1 - (1 - ep) exp(a u_seg), as ega_step_slow evaluates it
+    const float a = fdiv(__logf(1.f - r.w), r.z);
+    const float ex = __expf(a * (float)x);
+    e = 1.f - (1.f - ep) * ex;
+    se = ex > 0.f ? -a * (1.f - ep) * ex : 0.f;
+    de = ex;
+  } else {
+    const float dur = r.z - r.x, der = r.w - r.y;
+    su = fdiv(dur, der);
+    const float off = (ep - r.y) * su + (float)x;
+    if (off < dur) {
+      se = fdiv(der, dur);
+      e = r.y + off * se;
+    } else {
+      const double arg = (double)r.x + (double)off;
+      float4 r2;
+      col_walk_r<0>(col, n, arg, i, r, r2);
+      e = eps_rec_f_vs(r2, arg, se);
+    }
+    de = se * su;
+  }
+  return e;
+}
+
 // Reference-order lookups for columns that are not monotone (bisection).
 __device__ __forceinline__ float col_eps_bis(const float4 *col, int n, double x) {
   const float4 r0 = col[0];

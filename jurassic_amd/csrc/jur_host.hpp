@@ -138,6 +138,7 @@ struct jur_ctx {
   // work buffers
   jur::DevBuf<double> d_los;
   jur::DevBuf<char> d_eps;
+  jur::DevBuf<float4> d_part;  // FP32 EGA Jacobian: per-step lookup partials
   jur::DevBuf<int> d_npb;
   jur::DevBuf<size_t> d_eoff;
   jur::DevBuf<jur::RayInfo> d_info;

@@ -89,6 +89,8 @@ struct EpsView {
   void *eps;
   const size_t *off;   // [nblk] element offset of each block (chunk-relative)
   const int *npb;      // [nblk] points per chain of each block
+  float4 *part;        // FP32 EGA Jacobian: per chain step (d/d e_prev, d/dp, d/dT,
+                       // d/du) at the emissivity's index; null otherwise
 };
 
 // Jacobian pass (adjoint.hip).  Per chunk of ray blocks:

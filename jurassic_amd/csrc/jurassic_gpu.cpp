@@ -660,11 +660,15 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     const size_t esz = (precision == JUR_FP32) ? sizeof(float) : sizeof(double);
     // D_i is needed only where the surface reflects (sfeps < 1 needs NSF > 0)
     const int nscr = (s.sftype >= 2 && s.nsf > 0) ? 2 : 1;
+    // FP32 EGA Jacobian over monotone tables: the lookups store each step's
+    // partials (16 B per point, channel and emitter) for the reverse chains
+    const bool part = jac && precision == JUR_FP32 && M.formod != 0 && M.tab.all_mono;
     // Memory per ray block: the LOS at full capacity, and per LOS point of
     // the block's longest ray the emissivities (and for the Jacobian the
     // forward-sweep scratch), plus the Jacobian's K tile.
     const size_t los_blk = (size_t)L.nfield * L.cap * WAVE * sizeof(double);
-    const size_t pt_blk = (size_t)s.nd * s.ng * WAVE * esz + (jac ? (size_t)s.nd * nscr * WAVE * sizeof(double) : 0);
+    const size_t pt_blk = (size_t)s.nd * s.ng * WAVE * (esz + (part ? sizeof(float4) : 0)) +
+                          (jac ? (size_t)s.nd * nscr * WAVE * sizeof(double) : 0);
     const size_t tile_blk = jac ? (size_t)s.nd * jac->ncol * WAVE * sizeof(double) : 0;
     size_t budget = c->los_budget;
     if (budget == 0) {
@@ -672,7 +676,8 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
       // most 96 GiB
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
-      fr += c->d_los.n * sizeof(double) + c->d_eps.n + (c->d_kt.n + c->d_scr.n) * sizeof(double);
+      fr += c->d_los.n * sizeof(double) + c->d_eps.n + c->d_part.n * sizeof(float4) +
+            (c->d_kt.n + c->d_scr.n) * sizeof(double);
       budget = std::min(fr / 2, (size_t)96 << 30);
     }
     // Rays are traced in chunks whose LOS takes at most half the budget; the
@@ -719,10 +724,11 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
         }
         const int ns = s1 - s0;
         if ((rc = c->d_eps.reserve(tot * esz + 1))) return rc;
+        if (part && (rc = c->d_part.reserve(tot + 1))) return rc;
         HIPCHK(hipMemcpyAsync(c->d_eoff.p, eoff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
         LosView Ls = L;
         Ls.los = L.los + (size_t)s0 * L.cap * L.nfield * WAVE;
-        EpsView E{c->d_eps.p, c->d_eoff.p, c->d_npb.p + s0};
+        EpsView E{c->d_eps.p, c->d_eoff.p, c->d_npb.p + s0, part ? c->d_part.p : nullptr};
         const int bs = b0 + s0;
         HIPCHK(hipEventRecord(c->ev[1], st));
         if (precision == JUR_FP32) launch_lookup_f32(M, B, Ls, E, c->d_info.p, bs, ns, st);

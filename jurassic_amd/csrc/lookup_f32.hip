@@ -17,6 +17,10 @@
 //    kernel forms the segment emissivity (eps_i - eps_{i-1}) / (1 - eps_{i-1});
 //    the adjoint restarts any segment from eps_{i-1}); 1 marks an opaque
 //    path (tau_path below float resolution).
+//  * For the FP32 EGA Jacobian over monotone tables (PART), each step also
+//    stores its partials d eps_path / d (eps_prev, p, T, u_seg) with the
+//    slopes of the brackets it used (see adjoint.hip), so that the reverse
+//    chains need no table access.
 #include "jur_device.hpp"
 
 namespace jur {
@@ -24,7 +28,7 @@ namespace jur {
 // LOS field f of point ip: wave-uniform block base, 32-bit lane offset
 #define LOSR(f, ip) ld32(losb, ((unsigned)(ip) * nfield + (unsigned)(f)) * WAVE + lane)
 
-template <bool CGA>
+template <bool CGA, bool PART>
 __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, LosView L,
                                                      EpsView E, const RayInfo *info, int blk0,
                                                      int nblk) {
@@ -49,6 +53,7 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
   const unsigned nfield = L.nfield;
   const int npb = E.npb[bl];
   float *outb = (float *)E.eps + E.off[bl] + ((size_t)id * ng + ig) * npb * WAVE;
+  float4 *partb = PART ? E.part + E.off[bl] + ((size_t)id * ng + ig) * npb * WAVE : nullptr;
   const int mono = T.mono[tix];
   const int pdir = T.pdir[tix];
   const double *pg = T.p + (size_t)tix * T.TP;
@@ -132,6 +137,7 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       it0 = it1 = -1;
     }
     double pn = p, tn = t, un = useg;
+    float4 dpart = make_float4(1.f, 0.f, 0.f, 0.f);  // a cell without data: e_new = e_prev
     if (nt0 >= 2 && nt1 >= 2) {
       int j0 = (int)((tq - g00) * gi0);
       j0 = j0 < 0 ? 0 : (j0 > nt0 - 2 ? nt0 - 2 : j0);
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         const float4 *kc[4] = {recb + ko[0], recb + ko[1], recb + ko[2], recb + ko[3]};
         const int nc[4] = {n00, n01, n10, n11};
         const float us = (float)useg;
-        float ec[4];
+        float ec[4], dep[4] = {0.f, 0.f, 0.f, 0.f}, deu[4] = {0.f, 0.f, 0.f, 0.f};
         if (CGA || !mono) {
           if (!mono) {
 #pragma unroll 1
@@ -217,11 +223,24 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
           __builtin_amdgcn_sched_barrier(0);
           // common case: the inverse and forward brackets are those records
           bool slow[4];
+          if constexpr (PART) {
 #pragma unroll
-          for (int c = 0; c < 4; c++) ega_step_fast2(r[c], rn[c], nc[c], ep, us, useg, gi[c], ec[c], slow[c]);
+            for (int c = 0; c < 4; c++) {
+              int gg = gi[c];
+              ega_step_fast2_d(r[c], rn[c], nc[c], ep, us, gg, ec[c], dep[c], deu[c], slow[c]);
+              gi[c] = gg;
+            }
 #pragma unroll
-          for (int c = 0; c < 4; c++) {
-            if (slow[c]) ec[c] = ega_step_slow(kc[c], nc[c], ep, useg, gi[c], r[c]);
+            for (int c = 0; c < 4; c++) {
+              if (slow[c]) ec[c] = ega_corner_f32_d(kc[c], nc[c], ep, useg, gi[c], dep[c], deu[c]);
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++) ega_step_fast2(r[c], rn[c], nc[c], ep, us, useg, gi[c], ec[c], slow[c]);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              if (slow[c]) ec[c] = ega_step_slow(kc[c], nc[c], ep, useg, gi[c], r[c]);
+            }
           }
         }
         const float w0 = (tq - ta0) * dt0;
@@ -234,6 +253,23 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         else
           wp = (float)(pq - pa) * wp_scale;
         float en = a + (b - a) * wp;
+        if constexpr (PART) {
+          // the bilinear cell weights and their T and p slopes
+          const float q0 = 1.f - wp, q1 = wp;
+          const float cw[4] = {q0 * (1.f - w0), q0 * w0, q1 * (1.f - w1), q1 * w1};
+          const float tw[4] = {-q0 * dt0, q0 * dt0, -q1 * dt1, q1 * dt1};
+          const float pw[4] = {-(1.f - w0) * wp_scale, -w0 * wp_scale, (1.f - w1) * wp_scale, w1 * wp_scale};
+          float sdep = 0.f, sdu = 0.f, sdt = 0.f, sdp = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            sdep += cw[c] * dep[c];
+            sdu += cw[c] * deu[c];
+            sdt += tw[c] * ec[c];
+            sdp += pw[c] * ec[c];
+          }
+          // a clamped emissivity is constant
+          dpart = (en < 0.f || en > 1.f) ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(sdep, sdp, sdt, sdu);
+        }
         en = (en < 1.f) ? en : 1.f;
         en = (en > 0.f) ? en : 0.f;
         ep = en;
@@ -248,6 +284,7 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       un = LOSR(L.f_u + ig, ipn);
     }
     st32(outb, (unsigned)ip * WAVE + lane, ep);
+    if (PART) st32(partb, (unsigned)ip * WAVE + lane, dpart);
     if (ep >= 1.f) {
       // tau_path below float resolution: every later segment is opaque too
       for (ip++; ip < R.np; ip++) st32(outb, (unsigned)ip * WAVE + lane, 1.f);
@@ -264,9 +301,11 @@ void launch_lookup_f32(const ModelView &M, const BatchView &B, const LosView &L,
   if (M.nd * M.ng == 0) return;
   const dim3 grid(padded_blocks(nblk) * M.nd * M.ng), blk(WAVE);
   if (M.formod == 0)
-    hipLaunchKernelGGL(k_lookup_f32<true>, grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+    hipLaunchKernelGGL((k_lookup_f32<true, false>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+  else if (E.part)
+    hipLaunchKernelGGL((k_lookup_f32<false, true>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
   else
-    hipLaunchKernelGGL(k_lookup_f32<false>, grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
+    hipLaunchKernelGGL((k_lookup_f32<false, false>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
 }
 
 }  // namespace jur
