@@ -502,10 +502,13 @@ __device__ __forceinline__ int level_of(const atm_t *atm, double z, int lev, dou
 // wavefronts of the same row also update).
 // The pair's altitudes and pressures and the reciprocals the weights need are
 // kept while the pair lasts (a pair spans ~10 LOS points at RAYDZ = 0.1 km).
+// A path leaves a pair into a neighbouring one: that pair costs one round
+// trip for the new level's altitude and pressure, where a search of the
+// level grid (locate_irr) would take one per bisection step.
 template <int NQ, unsigned ATOM = 0>
 struct LevelWindow {
   int wa = -1;
-  double z0 = 0, z1 = 0, rz = 0, rp0 = 0, rp1 = 0;
+  double z0 = 0, z1 = 0, rz = 0, P0 = 0, P1 = 0, rp0 = 0, rp1 = 0;
   bool plog = false;
   double acc[NQ][2];
   __device__ __forceinline__ void flush(const int *xm, const int *qi, double *kt) {
@@ -527,21 +530,58 @@ struct LevelWindow {
   __device__ __forceinline__ void add(const atm_t *atm, double z, int &lev, const double *v, int qp, double p,
                                       const int *xm, const int *qi, double *kt) {
     if (!(wa >= 0 && z0 <= z && z < z1)) {
-      double w;
-      lev = level_of(atm, z, lev, w);
-      if (lev != wa) {
+      // the pair locate_irr returns: a neighbour of the current pair (with
+      // its clamps at the first and last pair), else a search
+      int nl = -1;
+      if (wa >= 0 && z0 < z1) {
+        if (z < z0) {
+          if (wa > 0) {
+            const double zl = atm->z[wa - 1], Pl = atm->p[wa - 1];
+            if (zl <= z || wa == 1) {
+              nl = wa - 1;
+              z1 = z0;
+              z0 = zl;
+              P1 = P0;
+              P0 = Pl;
+              rp1 = rp0;
+              rp0 = 1 / Pl;
+            }
+          }
+        } else {
+          const int n = atm->np;
+          if (wa + 2 <= n - 1) {
+            const double zh = atm->z[wa + 2], Ph = atm->p[wa + 2];
+            if (z < zh || wa + 2 == n - 1) {
+              nl = wa + 1;
+              z0 = z1;
+              z1 = zh;
+              P0 = P1;
+              P1 = Ph;
+              rp0 = rp1;
+              rp1 = 1 / Ph;
+            }
+          }
+        }
+      }
+      if (nl < 0) {
+        double w;
+        nl = level_of(atm, z, lev, w);
+        z0 = atm->z[nl];
+        z1 = atm->z[nl + 1];
+        P0 = atm->p[nl];
+        P1 = atm->p[nl + 1];
+        rp0 = 1 / P0;
+        rp1 = 1 / P1;
+      }
+      rz = 1 / (z1 - z0);
+      plog = P1 / P0 > 0;
+      if (nl != wa) {
         flush(xm, qi, kt);
-        wa = lev;
+        wa = nl;
 #pragma unroll
         for (int q = 0; q < NQ; q++) acc[q][0] = acc[q][1] = 0;
       }
-      z0 = atm->z[lev];
-      z1 = atm->z[lev + 1];
-      rz = 1 / (z1 - z0);
-      const double P0 = atm->p[lev], P1 = atm->p[lev + 1];
-      plog = P1 / P0 > 0;
-      rp0 = 1 / P0;
-      rp1 = 1 / P1;
+      lev = nl;
     }
     const double w = (z - z0) * rz;
 #pragma unroll
