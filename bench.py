@@ -53,7 +53,7 @@ HBM_PEAK = 8.0e12
 # 1 GiB device copy under the same counters), in
 # profiles/k_lookup_f32_limb_4096x32x12_hbm_traffic.md.  Other workload sizes
 # report null.
-LOOKUP_F32_TRAFFIC = (4096, 36.71e9)
+LOOKUP_F32_TRAFFIC = (4096, 37.84e9)
 NPROF, NRAY_PER, GASES = 4096, 32, ["CO2", "H2O", "O3"]
 
 
@@ -564,7 +564,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nprof", type=int, default=NPROF)
     ap.add_argument("--cpu-sample", type=int, default=None,
-                    help="profiles for the CPU baseline (default: 16 per thread)")
+                    help="profiles for the CPU baseline (default: 128 per thread, about 10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-jac", action="store_true")
@@ -702,7 +702,7 @@ def main():
     out.update(res)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ns = args.cpu_sample or max(16, 16 * threads)
+        ns = args.cpu_sample or max(128, 128 * threads)
         log("CPU baseline on %d profiles, %d threads" % (ns, threads))
         out["cpu_baseline"] = cpu_baseline(ns)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
