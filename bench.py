@@ -315,6 +315,54 @@ def boxcar_filters(nu, width=1.0, step=0.0005):
     return off, np.concatenate(xs), np.tile(w, len(nu))
 
 
+def tblgen_bench(args, gpu):
+    """Table production (tblgen, src/tblgen.c:44-140) for one pressure and
+    temperature: a broad-band filter over a 1,000,001-point optical-depth
+    spectrum, i.e. 10^6 points x ~600 column densities of exp terms.  Times
+    the whole C-ABI call (host selection of the points and weights, copies,
+    kernels, row selection) and the oracle's sequential loop on the same
+    input."""
+    import numpy as np
+    n, nu0, nu1 = 1000001, 600.0, 1100.0
+    rng = np.random.default_rng(7)
+    nu = np.linspace(nu0, nu1, n)
+    tau = np.full(n, 1e-5)
+    for c, st, w in zip(rng.uniform(nu0, nu1, 400), 10 ** rng.uniform(-3, 0, 400), rng.uniform(0.02, 0.2, 400)):
+        lo, hi = np.searchsorted(nu, [c - 50 * w, c + 50 * w])
+        tau[lo:hi] += st * w / np.pi / ((nu[lo:hi] - c) ** 2 + w * w)
+    nuf = np.linspace(610.0, 1090.0, 481)
+    filt = 0.5 + 0.5 * np.cos((nuf - 850.0) / 240.0 * np.pi / 2)
+    press, temp = 300.0, 240.0
+    gpu.tblgen(press, temp, tau, nu0, nu1, nuf, filt)  # warm
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        u, e = gpu.tblgen(press, temp, tau, nu0, nu1, nuf, filt)
+    dt = (time.perf_counter() - t0) / reps
+    npt = int(((nu >= nuf[0]) & (nu <= nuf[-1])).sum())
+    ku = 1
+    while 1.122 ** ku <= 1e30:
+        ku += 1
+    out = {"workload": "one (p, T): %d-point spectrum, %d points under the filter, %d column densities"
+                       % (n, npt, ku),
+           "rows": int(len(u)), "ms_per_call": dt * 1e3,
+           "terms_per_s": npt * ku / dt, "unit": "exp terms/s"}
+    log("tblgen: %.2f ms per (p, T), %d rows, %.3g terms/s" % (dt * 1e3, len(u), npt * ku / dt))
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        u0, e0 = O.tblgen(press, temp, tau, nu0, nu1, nuf, filt)
+        dc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / dc, "unit": "(p, T) rows per s", "cores": 1, "kind": "port",
+                               "sample": "the same call, oracle's sequential loop, %.1f s" % dc}
+        out["gpu_over_cpu"] = dc / dt
+        out["rows_match"] = bool(len(u0) == len(u) and np.array_equal(u0, u))
+        out["eps_max_abs_diff"] = float(np.abs(e0 - e).max()) if len(u0) == len(u) else None
+        log("tblgen CPU oracle: %.1f s (GPU / CPU = %.0f)" % (dc, dc / dt))
+    return out
+
+
 def wide_bench(args, torch, gpu, abi):
     """2048 channels x 8 emitters (CO2, H2O, O3, CH4, N2O, CO, HNO3, F11),
     nadir view angles (i mod 49) deg, continua and refraction on: forward
@@ -582,6 +630,8 @@ def main():
                     help="atmospheres per angle for the FP32-vs-FP64 adjoint check")
     ap.add_argument("--jac-cpu-sample", type=int, default=2)
     ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--no-tblgen", action="store_true")
+    ap.add_argument("--tblgen-only", action="store_true")
     ap.add_argument("--wide-only", action="store_true")
     ap.add_argument("--wide-nprof", type=int, default=4096)
     ap.add_argument("--copy-calib", action="store_true",
@@ -612,6 +662,9 @@ def main():
         return
     if args.wide_only:
         print(json.dumps({"wide": wide_bench(args, torch, gpu, abi)}))
+        return
+    if args.tblgen_only:
+        print(json.dumps({"tblgen": tblgen_bench(args, gpu)}))
         return
     ctl, atms, prof, geom = build_workload(args.nprof)
     nprof, nray, nd = args.nprof, len(prof), ctl.nd
@@ -712,6 +765,8 @@ def main():
         out["jacobian"] = jacobian_bench(args, torch, gpu, abi)
     if not args.no_serve:
         out["serving"] = serving_bench(args, gpu, abi)
+    if not args.no_tblgen:
+        out["tblgen"] = tblgen_bench(args, gpu)
     if not args.no_wide:
         try:
             torch.cuda.empty_cache()

@@ -243,6 +243,19 @@ int jur_gpu_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
    xmap[JUR_XMAP_LEN] and returns the state size n. */
 int jur_state_map(const ctl_t *ctl, const atm_t *atm, int *xmap);
 
+/* Emissivity rows of a lookup table from a monochromatic spectrum: replaces
+   the tblgen tool (src/tblgen.c:44-140, main(<press> <temp> <spec>
+   <filter>)).  tau[npts] is the optical-depth spectrum on the regular grid
+   nu0..nu1 (the values of the RFM file that tblgen reads, :84-102), and
+   (nuf, filt)[nf] the filter function (:67-74).  The sums over the spectrum
+   for the ~600 column densities 1.122^k run on the GPU.  Writes the rows
+   tblgen prints, u_out[r] and eps_out[r] (its third and fourth columns), up
+   to maxrow of them, and their number to *nrow (which may exceed maxrow).
+   Returns JUR_OK or JUR_ERR_ARG / JUR_ERR_MEM / JUR_ERR_HIP. */
+int jur_tblgen(double press, double temp, const double *tau, int npts, double nu0, double nu1,
+               const double *nuf, const double *filt, int nf, double *u_out, double *eps_out, int maxrow,
+               int *nrow);
+
 /* Retrieval settings: the part of the reference's ret_t
    (src/retrieval.c:31-113) that optimal_estimation reads. */
 typedef struct {

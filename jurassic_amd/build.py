@@ -38,6 +38,7 @@ SOURCES = [
     ("adjoint.hip", ["-ffp-contract=fast"]),
     ("lm_kernels.hip", ["-ffp-contract=off"]),
     ("retrieval.cpp", ["-x", "hip", "-ffp-contract=off"]),
+    ("tblgen.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -62,7 +62,7 @@ __device__ __forceinline__ void cart2geo(const double *x, double *z, double *lon
 
 // locate_irr (src/jurassic.c:4103-4129): bisection, either grid order.
 template <typename T>
-__device__ __forceinline__ int locate_irr(const T *xx, int n, double x) {
+__host__ __device__ __forceinline__ int locate_irr(const T *xx, int n, double x) {
   int ilo = 0, ihi = n - 1, i = (ihi + ilo) >> 1;
   if ((double)xx[i] < (double)xx[i + 1])
     while (ihi > ilo + 1) {

@@ -126,6 +126,9 @@ def lib():
         L.jur_server_stats.restype = I
         L.jur_server_profile.argtypes = [P, C.POINTER(D), I]
         L.jur_server_profile.restype = I
+        L.jur_tblgen.argtypes = [D, D, C.POINTER(D), I, D, D, C.POINTER(D), C.POINTER(D), I,
+                                 C.POINTER(D), C.POINTER(D), I, C.POINTER(I)]
+        L.jur_tblgen.restype = I
         if L.jur_gpu_abi_check(C.sizeof(abi.ctl_t), C.sizeof(abi.atm_t), C.sizeof(abi.obs_t)):
             raise OSError("ABI mismatch between jurassic_amd.abi and libjurassic_gpu.so")
         _LIB = L
@@ -213,6 +216,26 @@ def optimal_estimation(ret, ctl, obs_meas, atm_apr, method=abi.JUR_JAC_ADJOINT,
     if analysis:
         return out_a, out_o, out_h, out_n, res
     return out_a, out_o, out_h, out_n
+
+
+def tblgen(press, temp, tau, nu0, nu1, nuf, filt, maxrow=4096):
+    """The emissivity rows (u, eps) of tblgen (src/tblgen.c:44-140) for one
+    pressure [hPa] and temperature [K], from the optical-depth spectrum tau
+    on the regular grid nu0..nu1 and the filter function (nuf, filt); the
+    spectral sums run on the GPU."""
+    def arr(x):
+        return np.ascontiguousarray(x, dtype=np.float64)
+    tau, nuf, filt = arr(tau), arr(nuf), arr(filt)
+    u = np.zeros(maxrow)
+    e = np.zeros(maxrow)
+    nrow = C.c_int(0)
+    dp = C.POINTER(C.c_double)
+    _check(lib().jur_tblgen(press, temp, tau.ctypes.data_as(dp), len(tau), nu0, nu1, nuf.ctypes.data_as(dp),
+                            filt.ctypes.data_as(dp), len(nuf), u.ctypes.data_as(dp), e.ctypes.data_as(dp),
+                            maxrow, C.byref(nrow)))
+    if nrow.value > maxrow:
+        return tblgen(press, temp, tau, nu0, nu1, nuf, filt, maxrow=nrow.value)
+    return u[:nrow.value], e[:nrow.value]
 
 
 def state_map(ctl, atm):

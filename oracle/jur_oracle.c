@@ -1748,3 +1748,58 @@ done:
   free(y_m); free(iqa); free(ipa);
   return rc;
 }
+
+/* tblgen (src/tblgen.c:44-140).  The column density of 1 km at 1 ppmv
+   (:64) turns the optical depths into cross sections (:94, :99); the grid
+   spacing is recomputed from the end points (:106), and the integration
+   starts at the index (int) of the filter start's fractional position
+   times npts (:107-108), skipping points below the filter and stopping
+   above it (:117-120).  The filter bracket is kept from point to point and
+   from one column density to the next, and searched only when the point
+   leaves it (:122-123).  Column densities run 1, 1.122, 1.122^2, ... up to
+   1e30 by repeated multiplication (:111); a row is kept when its emissivity
+   lies in [1e-6, 0.999999] and exceeds the previous column density's
+   (:129-134), and the loop ends after the first emissivity above 0.999999
+   (:137-138). */
+int orc_tblgen(double press, double temp, const double *tau, int npts, double nu0, double nu1,
+               const double *nuf, const double *filt, int nf, double *u_out, double *eps_out,
+               int maxrow) {
+  if (nf < 2 || npts < 2) return -1;
+  const double uref = 1e-6 * press * 100 / (1.380658e-23 * temp) * 1000 / 1e4;
+  double *a = malloc(sizeof(double) * (size_t)npts);
+  if (!a) return -1;
+  for (int i = 0; i < npts; i++) a[i] = tau[i] / uref;
+  const double dnu = (nu1 - nu0) / ((double)npts - 1.0);
+  const double r0 = (nuf[0] - nu0) / (nu1 - nu0) * (double)npts;
+  const int i0 = (int)r0;
+  int idx = 0, nrow = 0;
+  double epsold = 0;
+  for (double u = 1.0; u <= 1e30; u *= 1.122) {
+    double epssum = 0, fsum = 0;
+    for (int i = i0; i < npts; i++) {
+      const double nu = nu0 + dnu * (double)i;
+      if (nu < nuf[0])
+        continue;
+      else if (nu > nuf[nf - 1])
+        break;
+      else {
+        if (nu < nuf[idx] || nu > nuf[idx + 1]) idx = orc_locate_irr(nuf, nf, nu);
+        const double f = LIN(nuf[idx], filt[idx], nuf[idx + 1], filt[idx + 1], nu);
+        fsum += f;
+        epssum += f * exp(-a[i] * u);
+      }
+    }
+    epssum = 1 - epssum / fsum;
+    if (epssum >= 1e-6 && epssum <= 0.999999 && epssum > epsold) {
+      if (nrow < maxrow) {
+        u_out[nrow] = u;
+        eps_out[nrow] = epssum;
+      }
+      nrow++;
+    }
+    epsold = epssum;
+    if (epssum > 0.999999) break;
+  }
+  free(a);
+  return nrow;
+}

@@ -176,6 +176,15 @@ int orc_trace_ray(const ctl_t *ctl, const atm_t *atm, obs_t *obs, int ir,
                   double *z, double *p, double *t, double *ds, double *u,
                   double *lon, double *lat, double *sft);
 
+/* tblgen (src/tblgen.c:44-140): emissivity rows of one pressure and
+   temperature from an optical-depth spectrum tau[npts] on the regular grid
+   nu0..nu1 and a filter function (nuf, filt)[nf].  Writes the rows the
+   reference prints, (u, eps), up to maxrow of them, and returns their
+   number (-1 for an empty filter range). */
+int orc_tblgen(double press, double temp, const double *tau, int npts, double nu0, double nu1,
+               const double *nuf, const double *filt, int nf, double *u_out, double *eps_out,
+               int maxrow);
+
 /* Drop cached tables (next call re-reads them). */
 void orc_reset(void);
 

@@ -79,6 +79,8 @@ def lib():
         L.orc_trace_ray.restype = I
         L.orc_hydrostatic.argtypes = [P, P]
         L.orc_reset.argtypes = []
+        L.orc_tblgen.argtypes = [C.c_double, C.c_double, dp, I, C.c_double, C.c_double, dp, dp, I, dp, dp, I]
+        L.orc_tblgen.restype = I
         L.orc_last_error.restype = C.c_char_p
         L.orc_optimal_estimation.argtypes = [P, P, P, P, P, P, dp, ip, P, P]
         L.orc_optimal_estimation.restype = I
@@ -200,3 +202,18 @@ def optimal_estimation(ret, ctl, obs_meas, atm_apr, fwd=None, analysis=False):
     if analysis:
         return atm_i, obs_i, hist[:nit.value + 1], nit.value, res
     return atm_i, obs_i, hist[:nit.value + 1], nit.value
+
+
+def tblgen(press, temp, tau, nu0, nu1, nuf, filt, maxrow=4096):
+    """The rows (u, eps) that tblgen prints for one pressure and temperature
+    (src/tblgen.c:44-140); tau is the optical-depth spectrum on nu0..nu1."""
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    nuf = np.ascontiguousarray(nuf, dtype=np.float64)
+    filt = np.ascontiguousarray(filt, dtype=np.float64)
+    u = np.zeros(maxrow)
+    e = np.zeros(maxrow)
+    n = lib().orc_tblgen(press, temp, _ptr(tau), len(tau), nu0, nu1, _ptr(nuf), _ptr(filt), len(nuf),
+                         _ptr(u), _ptr(e), maxrow)
+    if n < 0:
+        raise ValueError("orc_tblgen: empty spectrum or filter")
+    return u[:min(n, maxrow)], e[:min(n, maxrow)]
