@@ -246,7 +246,8 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     double ds = M.rayds;
     if (M.raydz > 0) {
       norm = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-      for (int i = 0; i < 3; i++) xh[i] = x[i] / norm;
+      const double rn = 1 / norm;
+      for (int i = 0; i < 3; i++) xh[i] = x[i] * rn;
       const double cosa = fabs(ex0[0] * xh[0] + ex0[1] * xh[1] + ex0[2] * xh[2]);
       if (cosa != 0) ds = (M.rayds < M.raydz / cosa) ? M.rayds : M.raydz / cosa;
     }
@@ -319,10 +320,11 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     LOSW(F_Z, np) = z;
     LOSW(F_P, np) = p;
     LOSW(F_T, np) = t;
+    const double ucol = 10 * p / (JUR_KB * t) * dtr;  // column density per unit vmr
     for (int ig = 0; ig < M.ng; ig++) {
       const double q = LV(6 + ig) + LV(6 + M.ng + ig) * dzl;
       LOSW(F_Q0 + ig, np) = q;
-      LOSW(L.f_u + ig, np) = 10 * q * p / (JUR_KB * t) * dtr;
+      LOSW(L.f_u + ig, np) = q * ucol;
     }
     for (int iw = 0; iw < M.nw; iw++)
       LOSW(L.f_k + iw, np) = LV(6 + 2 * M.ng + iw) + LV(6 + 2 * M.ng + M.nw + iw) * dzl;
@@ -398,13 +400,14 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
         for (int k = 0; k < 4; k++) intpol_pt(atm, anp, zdir, zg2, lc, zq[k], &pq[k], &tq[k]);
       }
       n = 7.753e-05 * pq[0] / tq[0];
-      for (int i = 0; i < 3; i++) ngr[i] = (7.753e-05 * pq[i + 1] / tq[i + 1] - n) / h;
+      for (int i = 0; i < 3; i++) ngr[i] = (7.753e-05 * pq[i + 1] / tq[i + 1] - n) * (1 / h);
     } else
       for (int i = 0; i < 3; i++) ngr[i] = 0;
 
     for (int i = 0; i < 3; i++) ex1[i] += ds * ngr[i];
     norm = sqrt(ex1[0] * ex1[0] + ex1[1] * ex1[1] + ex1[2] * ex1[2]);
-    for (int i = 0; i < 3; i++) ex1[i] /= norm;
+    const double rn1 = 1 / norm;
+    for (int i = 0; i < 3; i++) ex1[i] *= rn1;
     for (int i = 0; i < 3; i++) x[i] += 0.5 * ds * (ex0[i] + ex1[i]);
     for (int i = 0; i < 3; i++) ex0[i] = ex1[i];
   }
