@@ -348,6 +348,27 @@ def tblgen_bench(args, gpu):
            "rows": int(len(u)), "ms_per_call": dt * 1e3,
            "terms_per_s": npt * ku / dt, "unit": "exp terms/s"}
     log("tblgen: %.2f ms per (p, T), %d rows, %.3g terms/s" % (dt * 1e3, len(u), npt * ku / dt))
+    # a whole table: 40 pressures x 20 temperatures, each its own 4001-point
+    # spectrum of one channel (2 cm^-1 at 0.0005 cm^-1), in one batched call
+    nt_p, nt_t, m = 40, 20, 4001
+    tp = np.repeat(np.logspace(np.log10(1100.0), np.log10(0.01), nt_p), nt_t)
+    tt = np.tile(np.linspace(160.0, 320.0, nt_t), nt_p)
+    g0, g1 = 700.0, 702.0
+    gn = np.linspace(g0, g1, m)
+    base = np.full(m, 1e-6)
+    for c, st, w in zip(rng.uniform(g0, g1, 30), 10 ** rng.uniform(-3, 0, 30), rng.uniform(0.001, 0.01, 30)):
+        base += st * w / np.pi / ((gn - c) ** 2 + w * w)
+    ttau = base[None, :] * (tp / 300.0)[:, None] * (250.0 / tt)[:, None]
+    tnuf = np.linspace(700.2, 701.8, 33)
+    tfilt = np.exp(-0.5 * ((tnuf - 701.0) / 0.3) ** 2)
+    gpu.tblgen_batch(tp, tt, ttau, g0, g1, tnuf, tfilt)  # warm
+    t0 = time.perf_counter()
+    trows = gpu.tblgen_batch(tp, tt, ttau, g0, g1, tnuf, tfilt)
+    dtab = time.perf_counter() - t0
+    out["table"] = {"workload": "%d x %d (p, T) spectra of %d points, one batched call" % (nt_p, nt_t, m),
+                    "s_per_table": dtab, "rows_total": int(sum(len(r[0]) for r in trows)),
+                    "spectra_per_s": nt_p * nt_t / dtab}
+    log("tblgen table: %d spectra in %.1f ms (%.0f spectra/s)" % (nt_p * nt_t, dtab * 1e3, nt_p * nt_t / dtab))
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -360,6 +381,16 @@ def tblgen_bench(args, gpu):
         out["rows_match"] = bool(len(u0) == len(u) and np.array_equal(u0, u))
         out["eps_max_abs_diff"] = float(np.abs(e0 - e).max()) if len(u0) == len(u) else None
         log("tblgen CPU oracle: %.1f s (GPU / CPU = %.0f)" % (dc, dc / dt))
+        ns_cpu = 160
+        t0 = time.perf_counter()
+        for s_ in range(ns_cpu):
+            O.tblgen(tp[s_ * 5], tt[s_ * 5], ttau[s_ * 5], g0, g1, tnuf, tfilt)
+        dcs = (time.perf_counter() - t0) / ns_cpu
+        out["table"]["cpu_baseline"] = {"value": 1.0 / dcs, "unit": "spectra/s", "cores": 1, "kind": "port",
+                                        "sample": "%d of the table's spectra, oracle, %.1f s" % (ns_cpu, dcs * ns_cpu)}
+        out["table"]["gpu_over_cpu"] = (nt_p * nt_t / dtab) * dcs
+        log("tblgen table CPU oracle: %.1f ms per spectrum (GPU / CPU = %.0f)"
+            % (dcs * 1e3, (nt_p * nt_t / dtab) * dcs))
     return out
 
 

@@ -256,6 +256,14 @@ int jur_tblgen(double press, double temp, const double *tau, int npts, double nu
                const double *nuf, const double *filt, int nf, double *u_out, double *eps_out, int maxrow,
                int *nrow);
 
+/* jur_tblgen for nspec spectra on one grid and filter, e.g. the pressures
+   and temperatures of one table (the reference runs tblgen once per (p, T),
+   src/tblgen.c:44-140): tau is [nspec][npts]; spectrum s writes its rows
+   to u_out/eps_out[s * maxrow + r] and their number to nrow[s]. */
+int jur_tblgen_batch(int nspec, const double *press, const double *temp, const double *tau, int npts,
+                     double nu0, double nu1, const double *nuf, const double *filt, int nf, double *u_out,
+                     double *eps_out, int maxrow, int *nrow);
+
 /* Retrieval settings: the part of the reference's ret_t
    (src/retrieval.c:31-113) that optimal_estimation reads. */
 typedef struct {

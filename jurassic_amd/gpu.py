@@ -129,6 +129,9 @@ def lib():
         L.jur_tblgen.argtypes = [D, D, C.POINTER(D), I, D, D, C.POINTER(D), C.POINTER(D), I,
                                  C.POINTER(D), C.POINTER(D), I, C.POINTER(I)]
         L.jur_tblgen.restype = I
+        L.jur_tblgen_batch.argtypes = [I, C.POINTER(D), C.POINTER(D), C.POINTER(D), I, D, D, C.POINTER(D),
+                                       C.POINTER(D), I, C.POINTER(D), C.POINTER(D), I, C.POINTER(I)]
+        L.jur_tblgen_batch.restype = I
         if L.jur_gpu_abi_check(C.sizeof(abi.ctl_t), C.sizeof(abi.atm_t), C.sizeof(abi.obs_t)):
             raise OSError("ABI mismatch between jurassic_amd.abi and libjurassic_gpu.so")
         _LIB = L
@@ -236,6 +239,26 @@ def tblgen(press, temp, tau, nu0, nu1, nuf, filt, maxrow=4096):
     if nrow.value > maxrow:
         return tblgen(press, temp, tau, nu0, nu1, nuf, filt, maxrow=nrow.value)
     return u[:nrow.value], e[:nrow.value]
+
+
+def tblgen_batch(press, temp, tau, nu0, nu1, nuf, filt, maxrow=1024):
+    """tblgen for several spectra on one grid (tau[nspec][npts]) with their
+    pressures and temperatures: a list of (u, eps) per spectrum."""
+    def arr(x):
+        return np.ascontiguousarray(x, dtype=np.float64)
+    press, temp, tau, nuf, filt = arr(press), arr(temp), arr(tau), arr(nuf), arr(filt)
+    ns, npts = tau.shape
+    u = np.zeros((ns, maxrow))
+    e = np.zeros((ns, maxrow))
+    nrow = np.zeros(ns, dtype=np.int32)
+    dp = C.POINTER(C.c_double)
+    _check(lib().jur_tblgen_batch(ns, press.ctypes.data_as(dp), temp.ctypes.data_as(dp), tau.ctypes.data_as(dp),
+                                  npts, nu0, nu1, nuf.ctypes.data_as(dp), filt.ctypes.data_as(dp), len(nuf),
+                                  u.ctypes.data_as(dp), e.ctypes.data_as(dp), maxrow,
+                                  nrow.ctypes.data_as(C.POINTER(C.c_int))))
+    if ns and nrow.max() > maxrow:
+        return tblgen_batch(press, temp, tau, nu0, nu1, nuf, filt, maxrow=int(nrow.max()))
+    return [(u[s, :nrow[s]], e[s, :nrow[s]]) for s in range(ns)]
 
 
 def state_map(ctl, atm):

@@ -188,6 +188,24 @@ def test_gpu_tblgen_cli(gpu, oracle, tmp_path):
 
 
 @pytest.mark.gpu
+def test_gpu_tblgen_batch_matches_oracle(gpu, oracle):
+    # one table's (p, T) grid on one spectral grid, each spectrum its own
+    n, nu0, nu1 = 8001, 700.0, 704.0
+    press = np.repeat([1000.0, 300.0, 50.0, 2.0], 3)
+    temp = np.tile([200.0, 250.0, 300.0], 4)
+    tau = np.stack([line_spectrum(n, nu0, nu1, seed=10 + s) * (press[s] / 300.0) for s in range(len(press))])
+    nuf = np.linspace(700.5, 703.5, 31)
+    filt = 1.0 - np.abs(nuf - 702.0) / 1.6
+    rows = gpu.tblgen_batch(press, temp, tau, nu0, nu1, nuf, filt)
+    assert len(rows) == len(press)
+    for s, (u1, e1) in enumerate(rows):
+        u0, e0 = oracle.tblgen(press[s], temp[s], tau[s], nu0, nu1, nuf, filt)
+        assert len(u1) == len(u0) > 10, s
+        np.testing.assert_array_equal(u1, u0)
+        np.testing.assert_allclose(e1, e0, rtol=1e-10, atol=1e-15 * np.sqrt(n))
+
+
+@pytest.mark.gpu
 def test_gpu_tblgen_bad_args(gpu):
     with pytest.raises(gpu.JurassicError):
         gpu.tblgen(300.0, 250.0, np.ones(1), 600.0, 601.0, np.array([600.2, 600.8]), np.ones(2))
