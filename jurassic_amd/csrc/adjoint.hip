@@ -498,19 +498,25 @@ __device__ __forceinline__ int level_of(const atm_t *atm, double z, int lev, dou
 // Sums of point adjoints of the current level pair (wa, wa+1) for NQ level
 // quantities (JUR_XMAP_LEV quantity index qi[q], -1 = none); flushed into the
 // lane's tile row when the pair changes.
-// Quantities in the mask ATOM are added atomically (columns that other
-// wavefronts of the same row also update).
+// Quantities in the mask ATOM are added atomically: columns that other
+// wavefronts of the same row also update, and columns a wavefront owns,
+// where an atomic add without return saves the flush the read of the tile
+// (one wavefront's adds to one address are applied in its program order, so
+// the sums keep their order).  With CCOL the pair's K columns are loaded
+// when the pair is entered, so that its flush does not wait for them (at
+// the cost of registers, which the FP64 kernels cannot spare).
 // The pair's altitudes and pressures and the reciprocals the weights need are
 // kept while the pair lasts (a pair spans ~10 LOS points at RAYDZ = 0.1 km).
 // A path leaves a pair into a neighbouring one: that pair costs one round
 // trip for the new level's altitude and pressure, where a search of the
 // level grid (locate_irr) would take one per bisection step.
-template <int NQ, unsigned ATOM = 0>
+template <int NQ, unsigned ATOM = 0, bool CCOL = true>
 struct LevelWindow {
   int wa = -1;
   double z0 = 0, z1 = 0, rz = 0, P0 = 0, P1 = 0, rp0 = 0, rp1 = 0;
   bool plog = false;
   double acc[NQ][2];
+  int col[NQ][2];  // K columns of the pair's levels (loaded when the pair is entered)
   __device__ __forceinline__ void flush(const int *xm, const int *qi, double *kt) {
     if (wa < 0) return;
 #pragma unroll
@@ -518,7 +524,7 @@ struct LevelWindow {
 #pragma unroll
       for (int q = 0; q < NQ; q++) {
         if (qi[q] < 0) continue;
-        const int c = xm[JUR_XMAP_LEV(qi[q], wa + k)];
+        const int c = CCOL ? col[q][k] : xm[JUR_XMAP_LEV(qi[q], wa + k)];
         if (c < 0) continue;
         if (ATOM >> q & 1u) unsafeAtomicAdd(&kt[(size_t)c * WAVE], acc[q][k]);
         else kt[(size_t)c * WAVE] += acc[q][k];
@@ -579,7 +585,13 @@ struct LevelWindow {
         flush(xm, qi, kt);
         wa = nl;
 #pragma unroll
-        for (int q = 0; q < NQ; q++) acc[q][0] = acc[q][1] = 0;
+        for (int q = 0; q < NQ; q++) {
+          acc[q][0] = acc[q][1] = 0;
+          if (CCOL) {
+            col[q][0] = qi[q] >= 0 ? xm[JUR_XMAP_LEV(qi[q], nl)] : -1;
+            col[q][1] = qi[q] >= 0 ? xm[JUR_XMAP_LEV(qi[q], nl + 1)] : -1;
+          }
+        }
       }
       lev = nl;
     }
@@ -789,7 +801,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
   {
     const int qi[5] = {1, 0, has_k ? 2 + NG + C.window : -1, M.ig_co2 >= 0 ? 2 + M.ig_co2 : -1,
                        M.ig_h2o >= 0 ? 2 + M.ig_h2o : -1};
-    LevelWindow<5> W;
+    LevelWindow<5, 0x1Fu, F32> W;
     int lev = -1;
     double qi_cur = nef > 0 ? qprod(nef - 1) : 1;
     // point ip's inputs and scratch, and the emitters' values at ip - 1,
@@ -925,7 +937,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
       ct += u * LOSR(F_T, ip);
     }
   const int qi[3] = {1, 0, 2 + g};
-  LevelWindow<3, 3u> W;
+  LevelWindow<3, 7u, FAST> W;
   int lev = -1, ipr = -1;
   int gi[4];
 #pragma unroll
