@@ -236,10 +236,11 @@ def jacobian_bench(args, torch, gpu, abi):
     ctx.close()
     del k_d
 
-    # algorithmic bytes per Jacobian: forward lookups plus the recompute sweep
-    # (176 B per segment, channel, emitter; 16 B Planck bracket per segment
-    # and channel) and the K write
-    bytes_per_jac = 2 * m * mean_np * (len(GASES) * 176 + 16) + m * n * 4
+    # algorithmic bytes per Jacobian: the lookups (176 B per segment, channel
+    # and emitter) with their stored partials (16 B written, then read with
+    # the 4 B emissivity by the reverse chains), the Planck bracket (16 B per
+    # segment and channel, forward and reverse sweep) and the FP64 K write
+    bytes_per_jac = m * mean_np * (len(GASES) * (176 + 16 + 16 + 4) + 2 * 16) + m * n * 8
     out = {
         "value": jac_rate, "unit": "Jacobians/s", "ms_per_step": dt * 1e3, "steps": steps,
         "workload": "nadir sounder: %d atmospheres x %d view angles x %d channels, n = %d, m = %d "
