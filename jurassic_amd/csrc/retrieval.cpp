@@ -83,7 +83,10 @@ int kernel_fd_host(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t m
 
 namespace {
 
-enum Stage { ST_EVAL_F_OUT, ST_EVAL_FK_OUT, ST_EVAL_FK0, ST_ITER, ST_EVAL_FK, ST_TRIAL, ST_EVAL_F, ST_DONE };
+// ST_EVAL_FT: a trial step evaluated with its Jacobian, because the next
+// iteration recomputes K (kernel_recomp) and, if the step is accepted, that
+// K is the one at the new state.
+enum Stage { ST_EVAL_F_OUT, ST_EVAL_FK_OUT, ST_EVAL_FK0, ST_ITER, ST_EVAL_FK, ST_TRIAL, ST_EVAL_F, ST_EVAL_FT, ST_DONE };
 
 struct Job {
   int kind = JUR_REQ_RETRIEVAL;
@@ -108,6 +111,8 @@ struct Job {
   // S_a^-1, C, A, K, sig, dy, dx, b, step
   double *dp[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int dc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double *ktr = nullptr;  // K of a trial step (ST_EVAL_FT), m x n
+  int ktrc = 0;
   std::vector<double> sa_in;  // packed inputs of S_a (8 n)
   int ray0 = 0;
   size_t bytes = 0;        // device footprint of the job's own buffers
@@ -125,7 +130,8 @@ struct Job {
   double *g = nullptr;             // gain matrix during the analysis
   int gc = 0;
   LmDesc desc() {
-    return LmDesc{n, m, ray0, nr, dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], dp[8], g};
+    return LmDesc{n, m, ray0, nr, dp[0], dp[1], dp[2], st == ST_EVAL_FT ? ktr : dp[3], dp[4], dp[5], dp[6], dp[7],
+                  dp[8], g};
   }
   bool failed() const { return rc != JUR_OK; }
 };
@@ -268,6 +274,8 @@ struct Engine {
       pool.put(J->dp[b], J->dc[b]);
       J->dp[b] = nullptr;
     }
+    if (J->ktr) pool.put(J->ktr, J->ktrc);
+    J->ktr = nullptr;
   }
   // One round over all active jobs; finished jobs are moved to done.
   int round(std::vector<Job *> &done);
@@ -352,7 +360,14 @@ int Engine::add(Job *J) {
       return fail(J, set_err(JUR_ERR_MEM, "device allocation failed"));
     }
   }
-  J->bytes = (3 * nn + (size_t)J->m * J->n + 2 * J->m + 3 * J->n) * 8;
+  if (method == JUR_JAC_ADJOINT) {
+    J->ktr = pool.get((size_t)J->m * J->n, J->ktrc);
+    if (!J->ktr) {
+      release(J);
+      return fail(J, set_err(JUR_ERR_MEM, "device allocation failed"));
+    }
+  }
+  J->bytes = (3 * nn + (size_t)J->m * J->n * (J->ktr ? 2 : 1) + 2 * J->m + 3 * J->n) * 8;
   J->st = ST_EVAL_FK0;
   pending.push_back(J);
   return JUR_OK;
@@ -720,7 +735,7 @@ int Engine::round(std::vector<Job *> &done) {
   for (size_t r = 0; r < nj; r++) {
     const Stage s = jobs[r]->st;
     if (s == ST_EVAL_F || s == ST_EVAL_F_OUT) need_f.push_back((int)r);
-    if (s == ST_EVAL_FK0 || s == ST_EVAL_FK || s == ST_EVAL_FK_OUT) need_fk.push_back((int)r);
+    if (s == ST_EVAL_FK0 || s == ST_EVAL_FK || s == ST_EVAL_FK_OUT || s == ST_EVAL_FT) need_fk.push_back((int)r);
   }
   auto tnow = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
   const double t0 = tnow();
@@ -771,11 +786,14 @@ int Engine::round(std::vector<Job *> &done) {
   std::vector<int> trials;
   for (int r : need_f)
     if (jobs[r]->st == ST_EVAL_F) trials.push_back(r);
+  for (int r : need_fk)
+    if (jobs[r]->st == ST_EVAL_FT) trials.push_back(r);
   if ((rc = chisq(trials))) return rc;
   for (int r : trials) {
     Job &J = *jobs[r];
     if (J.failed()) continue;
-    bool end_iter;
+    const bool with_k = J.st == ST_EVAL_FT;
+    bool end_iter, accepted = false;
     if (J.chisq > J.chisq_old) {
       J.lmpar *= 10;
       for (int i = 0; i < J.n; i++) J.x_i[i] -= J.step[i];
@@ -785,6 +803,7 @@ int Engine::round(std::vector<Job *> &done) {
     } else {
       J.lmpar /= 10;
       end_iter = true;
+      accepted = true;
     }
     if (end_iter) {
       if (J.hist) J.hist[J.it] = J.chisq;
@@ -799,6 +818,12 @@ int Engine::round(std::vector<Job *> &done) {
       } else {
         J.fk_done = false;
         J.st = ST_ITER;
+        if (accepted && with_k && J.it % kr == 0) {
+          // the trial's K is the one at the accepted state
+          std::swap(J.dp[3], J.ktr);
+          std::swap(J.dc[3], J.ktrc);
+          J.fk_done = true;
+        }
       }
     }
   }
@@ -859,7 +884,8 @@ int Engine::round(std::vector<Job *> &done) {
       memcpy(J.atm_i, J.atm_apr, sizeof(atm_t));
       x2atm_h(J.sctl, J.x_i.data(), J.atm_i);
       clamp_atm(J.sctl, J.atm_i);
-      J.st = ST_EVAL_F;
+      // with its Jacobian when the next iteration would recompute K
+      J.st = (J.ktr && (J.it + 1) % kr == 0 && J.it + 1 <= itmax) ? ST_EVAL_FT : ST_EVAL_F;
     }
   }
   // ---- error analysis of finished retrievals ----
