@@ -53,7 +53,7 @@ HBM_PEAK = 8.0e12
 # 1 GiB device copy under the same counters), in
 # profiles/k_lookup_f32_limb_4096x32x12_hbm_traffic.md.  Other workload sizes
 # report null.
-LOOKUP_F32_TRAFFIC = (4096, 37.84e9)
+LOOKUP_F32_TRAFFIC = (4096, 38.85e9)
 NPROF, NRAY_PER, GASES = 4096, 32, ["CO2", "H2O", "O3"]
 
 
