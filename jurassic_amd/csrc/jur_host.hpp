@@ -137,6 +137,7 @@ struct jur_ctx {
   jur::DevBuf<double> d_fov_dz, d_fov_w;
   // work buffers
   jur::DevBuf<double> d_los;
+  jur::DevBuf<float> d_losf;  // FP32 forward pass: LOS fields rounded to float
   jur::DevBuf<char> d_eps;
   jur::DevBuf<float4> d_part;  // FP32 EGA Jacobian: per-step lookup partials
   jur::DevBuf<int> d_npb;

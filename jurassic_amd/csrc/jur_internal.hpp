@@ -75,6 +75,9 @@ struct LosView {
   int f_k;      // first window-extinction field
   int f_cl;     // cloud factor field, -1 if none
   int f_u;      // first column-density field
+  float *losf;  // FP32 forward pass: the fields rounded to float, same layout
+                // (the FP32 radiance kernel rounds every field it reads);
+                // null otherwise
 };
 
 // Per-segment emissivities of every (channel, emitter), written by the

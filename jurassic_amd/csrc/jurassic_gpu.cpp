@@ -666,7 +666,10 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     // Memory per ray block: the LOS at full capacity, and per LOS point of
     // the block's longest ray the emissivities (and for the Jacobian the
     // forward-sweep scratch), plus the Jacobian's K tile.
-    const size_t los_blk = (size_t)L.nfield * L.cap * WAVE * sizeof(double);
+    // the FP32 forward pass also keeps the fields rounded to float (what its
+    // radiance kernel reads)
+    const bool losf = precision == JUR_FP32 && !jac;
+    const size_t los_blk = (size_t)L.nfield * L.cap * WAVE * (sizeof(double) + (losf ? sizeof(float) : 0));
     const size_t pt_blk = (size_t)s.nd * s.ng * WAVE * (esz + (part ? sizeof(float4) : 0)) +
                           (jac ? (size_t)s.nd * nscr * WAVE * sizeof(double) : 0);
     const size_t tile_blk = jac ? (size_t)s.nd * jac->ncol * WAVE * sizeof(double) : 0;
@@ -676,7 +679,7 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
       // most 96 GiB
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
-      fr += c->d_los.n * sizeof(double) + c->d_eps.n + c->d_part.n * sizeof(float4) +
+      fr += c->d_los.n * sizeof(double) + c->d_losf.n * sizeof(float) + c->d_eps.n + c->d_part.n * sizeof(float4) +
             (c->d_kt.n + c->d_scr.n) * sizeof(double);
       budget = std::min(fr / 2, (size_t)96 << 30);
     }
@@ -686,10 +689,12 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, (budget / 2) / (los_blk + 1)));
     const size_t rest = budget > los_blk * chunk ? budget - los_blk * chunk : 0;
     if ((rc = c->d_los.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
+    if (losf && (rc = c->d_losf.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
     if ((rc = c->d_npb.reserve(chunk))) return rc;
     if ((rc = c->d_eoff.reserve(chunk))) return rc;
     if (jac && (rc = c->d_soff.reserve(chunk))) return rc;
     L.los = c->d_los.p;
+    L.losf = losf ? c->d_losf.p : nullptr;
     std::vector<int> npb(chunk);
     std::vector<size_t> eoff(chunk), soff(chunk);
     float ms[3] = {0, 0, 0};
@@ -728,6 +733,7 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
         HIPCHK(hipMemcpyAsync(c->d_eoff.p, eoff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
         LosView Ls = L;
         Ls.los = L.los + (size_t)s0 * L.cap * L.nfield * WAVE;
+        if (Ls.losf) Ls.losf = L.losf + (size_t)s0 * L.cap * L.nfield * WAVE;
         EpsView E{c->d_eps.p, c->d_eoff.p, c->d_npb.p + s0, part ? c->d_part.p : nullptr};
         const int bs = b0 + s0;
         HIPCHK(hipEventRecord(c->ev[1], st));

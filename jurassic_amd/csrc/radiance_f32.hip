@@ -44,7 +44,9 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   const ChanInfo C = M.chan[id];
   const double nu = C.nu;
   const float nuf = (float)nu;
-  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  // the LOS fields rounded to float by the ray tracer (this kernel rounds
+  // every field it reads, so the copy reads half the bytes for the same bits)
+  const float *los = L.losf + (size_t)bl * L.cap * L.nfield * WAVE + lane;
   const int npb = E.npb[bl];
   const float *eps_g = (const float *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
   const double *sr_col = M.sr + (size_t)id * TBLNS;
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   const int f_uh = h2o_on ? L.f_u + M.ig_h2o : F_DS;
   const int f_qh = h2o_on ? F_Q0 + M.ig_h2o : F_DS;
   float en[NG];
-  double pn = 0, tn = 0, dsn = 0, kn = 0, cln = 0, ucn = 0, uhn = 0, qhn = 0;
+  float pn = 0, tn = 0, dsn = 0, kn = 0, cln = 0, ucn = 0, uhn = 0, qhn = 0;
   auto load = [&](int ip) {
 #pragma unroll
     for (int g = 0; g < NG; g++)
@@ -149,8 +151,8 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
   // the source-function bracket of the next point (its T row of sr_col),
   // loaded at the end of the iteration before
   double s0n = 0, s1n = 0, stn = 0;
-  auto load_src = [&](double t) {
-    int it = (int)(((float)t - st0) * inv_dst);
+  auto load_src = [&](float t) {
+    int it = (int)((t - st0) * inv_dst);
     it = it < 0 ? 0 : (it > TBLNS - 2 ? TBLNS - 2 : it);
     s0n = sr_col[it];
     s1n = sr_col[it + 1];
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
     float ec[NG];
 #pragma unroll
     for (int g = 0; g < NG; g++) ec[g] = en[g];
-    const double pc = pn, tc = tn, dsc = dsn, kc = kn, clc = cln, ucc = ucn, uhc = uhn, qhc = qhn;
+    const float pc = pn, tc = tn, dsc = dsn, kc = kn, clc = cln, ucc = ucn, uhc = uhn, qhc = qhn;
     const double s0c = s0n, s1c = s1n, stc = stn;
     load(ip + 1 < R.np ? ip + 1 : ip);
 

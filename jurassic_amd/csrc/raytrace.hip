@@ -83,6 +83,16 @@ __device__ __forceinline__ double sza(double sec, double lon, double lat) {
 }
 
 #define LOSW(f, ip) LOS_AT(base, L, f, ip)
+// a LOS field and, for the FP32 forward pass, its float copy (of the
+// fields the FP32 radiance kernel reads: the column densities of CO2 and
+// H2O and the H2O mixing ratio among the per-gas fields)
+#define LOSW3(f, ip, v, copy)                                  \
+  do {                                                         \
+    const double v_ = (v);                                     \
+    LOSW(f, ip) = v_;                                          \
+    if (basef && (copy)) LOS_AT(basef, L, f, ip) = (float)v_;  \
+  } while (0)
+#define LOSW2(f, ip, v) LOSW3(f, ip, v, true)
 
 // First minimum of z along the LOS (gsl_stats_min_index semantics: a NaN
 // ends the search) and the positions of its two neighbours, for
@@ -137,6 +147,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
   const int ir = blk * WAVE + lane;
   if (ir >= B.nray) return;
   double *base = L.los + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane;
+  float *basef = L.losf ? L.losf + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane : nullptr;
 
   const double *g = B.ray_geom + (size_t)7 * ir;
   const int iprof = B.ray_prof[ir];
@@ -318,22 +329,22 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     // rewrites)
     const double dtr = np == 0 ? ds * 0.5 : 0.5 * (ds1 + ds);
     LOSW(F_Z, np) = z;
-    LOSW(F_P, np) = p;
-    LOSW(F_T, np) = t;
+    LOSW2(F_P, np, p);
+    LOSW2(F_T, np, t);
     const double ucol = 10 * p / (JUR_KB * t) * dtr;  // column density per unit vmr
     for (int ig = 0; ig < M.ng; ig++) {
       const double q = LV(6 + ig) + LV(6 + M.ng + ig) * dzl;
-      LOSW(F_Q0 + ig, np) = q;
-      LOSW(L.f_u + ig, np) = q * ucol;
+      LOSW3(F_Q0 + ig, np, q, ig == M.ig_h2o);
+      LOSW3(L.f_u + ig, np, q * ucol, ig == M.ig_co2 || ig == M.ig_h2o);
     }
     for (int iw = 0; iw < M.nw; iw++)
-      LOSW(L.f_k + iw, np) = LV(6 + 2 * M.ng + iw) + LV(6 + 2 * M.ng + M.nw + iw) * dzl;
-    LOSW(F_DS, np) = dtr;
+      LOSW2(L.f_k + iw, np, LV(6 + 2 * M.ng + iw) + LV(6 + 2 * M.ng + M.nw + iw) * dzl);
+    LOSW2(F_DS, np, dtr);
     if (L.f_cl >= 0) {
       const double aux = (M.ncl > 0 && atm->cldz > 0)
                              ? exp(-0.5 * (((z - atm->clz) / atm->cldz) * ((z - atm->clz) / atm->cldz)))
                              : 0.0;
-      LOSW(L.f_cl, np) = aux;
+      LOSW2(L.f_cl, np, aux);
     }
     // MinTrack::add, with the raw step lengths at the minimum and after it
     if (!nan_seen) {
@@ -447,6 +458,7 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
   RayInfo R = info[ir];
   if (R.status != JUR_OK || R.np == 0) return;
   double *base = L.los + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane;
+  float *basef = L.losf ? L.losf + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane : nullptr;
   const atm_t *atm = &B.atm[B.ray_prof[ir]];
   const RayTail T = tail[ir];
   MinTrack mt = T.mt;
@@ -485,33 +497,34 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
   // the march wrote those of the others
   const double dtr0 = npm == 1 ? clip * 0.5 : 0.5 * (T.ds2 + clip);
   const double dtr1 = 0.5 * (clip + 0.0);
-  LOSW(F_DS, npm - 1) = dtr0;
+  LOSW2(F_DS, npm - 1, dtr0);
   {
     const double p0 = LOSW(F_P, npm - 1), t0 = LOSW(F_T, npm - 1);
     for (int ig = 0; ig < M.ng; ig++)
-      LOSW(L.f_u + ig, npm - 1) = 10 * LOSW(F_Q0 + ig, npm - 1) * p0 / (JUR_KB * t0) * dtr0;
+      LOSW3(L.f_u + ig, npm - 1, 10 * LOSW(F_Q0 + ig, npm - 1) * p0 / (JUR_KB * t0) * dtr0,
+            ig == M.ig_co2 || ig == M.ig_h2o);
   }
   LOSW(F_Z, np) = z;
-  LOSW(F_P, np) = p;
-  LOSW(F_T, np) = t;
+  LOSW2(F_P, np, p);
+  LOSW2(F_T, np, t);
   for (int ig = 0; ig < M.ng; ig++) {
     const double q0 = atm->q[ig][ipa];
     const double qs = (atm->q[ig][ipa + 1] - atm->q[ig][ipa]) / (x1 - x0);
     const double q = q0 + qs * dzl;
-    LOSW(F_Q0 + ig, np) = q;
-    LOSW(L.f_u + ig, np) = 10 * q * p / (JUR_KB * t) * dtr1;
+    LOSW3(F_Q0 + ig, np, q, ig == M.ig_h2o);
+    LOSW3(L.f_u + ig, np, 10 * q * p / (JUR_KB * t) * dtr1, ig == M.ig_co2 || ig == M.ig_h2o);
   }
   for (int iw = 0; iw < M.nw; iw++) {
     const double k0 = atm->k[iw][ipa];
     const double ks = (atm->k[iw][ipa + 1] - atm->k[iw][ipa]) / (x1 - x0);
-    LOSW(L.f_k + iw, np) = k0 + ks * dzl;
+    LOSW2(L.f_k + iw, np, k0 + ks * dzl);
   }
-  LOSW(F_DS, np) = dtr1;
+  LOSW2(F_DS, np, dtr1);
   if (L.f_cl >= 0) {
     const double aux = (M.ncl > 0 && atm->cldz > 0)
                            ? exp(-0.5 * (((z - atm->clz) / atm->cldz) * ((z - atm->clz) / atm->cldz)))
                            : 0.0;
-    LOSW(L.f_cl, np) = aux;
+    LOSW2(L.f_cl, np, aux);
   }
   mt.add(np, z, x, T.xprev);
   np++;
