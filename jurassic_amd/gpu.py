@@ -44,7 +44,7 @@ EXPORTS = [
     "jur_ctx_last_timing", "jur_ctx_last_mean_np", "jur_state_map", "jur_ctx_jacobian_device",
     "jur_gpu_optimal_estimation", "jur_server_create", "jur_server_destroy", "jur_server_submit",
     "jur_server_wait", "jur_server_fetch", "jur_server_stats", "jur_server_profile",
-    "jur_ctx_create_device_tables", "jur_gpu_optimal_estimation_ana",
+    "jur_ctx_create_device_tables", "jur_gpu_optimal_estimation_ana", "jur_tblgen", "jur_tblgen_batch",
 ]
 
 ANA_MATS = ("cov", "corr", "gain", "avk")

@@ -115,6 +115,17 @@ def test_spectrum_file_roundtrip(tmp_path):
     np.testing.assert_array_equal(filt, [0.1, 1.0, 0.2])
 
 
+def test_tblgen_argument_checks_on_host():
+    # rejected or empty before any device call, so these run without a GPU
+    from jurassic_amd import gpu as G
+    with pytest.raises(G.JurassicError):
+        G.tblgen(300.0, 250.0, np.ones(1), 600.0, 601.0, np.array([600.2, 600.8]), np.ones(2))
+    with pytest.raises(G.JurassicError):
+        G.tblgen(300.0, 250.0, np.ones(100), 600.0, 601.0, np.array([600.2]), np.ones(1))
+    assert G.tblgen_batch(np.zeros(0), np.zeros(0), np.zeros((0, 100)), 600.0, 601.0,
+                          np.array([600.2, 600.8]), np.ones(2)) == []
+
+
 @pytest.fixture(scope="module")
 def gpu(hip_runtime):
     from jurassic_amd import gpu as G
@@ -203,6 +214,26 @@ def test_gpu_tblgen_batch_matches_oracle(gpu, oracle):
         assert len(u1) == len(u0) > 10, s
         np.testing.assert_array_equal(u1, u0)
         np.testing.assert_allclose(e1, e0, rtol=1e-10, atol=1e-15 * np.sqrt(n))
+
+
+@pytest.mark.gpu
+def test_gpu_tblgen_row_capacity(gpu):
+    # fewer output slots than rows: the first maxrow rows, and the count of all
+    import ctypes as C
+    tau = line_spectrum(10001, 700.0, 702.0, seed=4)
+    nuf = np.array([700.5, 701.0, 701.5])
+    filt = np.array([0.5, 1.0, 0.5])
+    u_all, e_all = gpu.tblgen(400.0, 260.0, tau, 700.0, 702.0, nuf, filt)
+    dp = C.POINTER(C.c_double)
+    u = np.zeros(5)
+    e = np.zeros(5)
+    n = C.c_int(0)
+    rc = gpu.lib().jur_tblgen(400.0, 260.0, tau.ctypes.data_as(dp), len(tau), 700.0, 702.0, nuf.ctypes.data_as(dp),
+                              filt.ctypes.data_as(dp), 3, u.ctypes.data_as(dp), e.ctypes.data_as(dp), 5,
+                              C.byref(n))
+    assert rc == 0 and n.value == len(u_all) > 5
+    np.testing.assert_array_equal(u, u_all[:5])
+    np.testing.assert_array_equal(e, e_all[:5])
 
 
 @pytest.mark.gpu
