@@ -98,54 +98,105 @@ __global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *
     }
 }
 
-// Largest n whose Cholesky column and triangular solves use the staged paths
-// below (the LDS column, the per-lane rows of one wavefront).
+// Largest n whose Cholesky and triangular solves use the staged paths below
+// (the LDS panel, the per-lane rows of one wavefront).
 constexpr int CHOL_NMAX = 512;
 
 // In-place Cholesky A = L L^T of an n x n row-major matrix (lower triangle
 // of a holds L; the upper triangle is left as it was).  Returns false if a
-// pivot is not positive.  Called by all threads of the workgroup.  For
-// n <= CHOL_NMAX, column j is staged in LDS for the trailing update, whose
-// element updates are independent and issued four at a time.  (Owning each
-// element by one thread for the whole factorisation, with LDS-only
-// barriers, measured slower: 4.0 against 3.3 ms at n = 324.)
+// pivot is not positive.  Called by all threads of the workgroup.
+//
+// For n <= CHOL_NMAX: right-looking by panels of CHOL_B columns.  The panel
+// (rows j0..n-1) is factored in LDS, column by column, then written back,
+// and the trailing matrix takes the panel's CHOL_B updates per element in
+// one read-modify-write.  Every element sees the same operations in the
+// same order as in the column-by-column algorithm (each update subtracted
+// in column order), so the factor is the same to the bit; the workgroup
+// waits on global memory twice per panel instead of three times per
+// column.  (Owning each element by one thread for the whole factorisation,
+// with LDS-only barriers, measured slower.)
+constexpr int CHOL_B = 8;
+
 __device__ bool chol_wg(double *a, int n, int *bad) {
-  __shared__ double colj[CHOL_NMAX];
-  const bool staged = n <= CHOL_NMAX;
-  for (int j = 0; j < n; j++) {
-    if (threadIdx.x == 0) {
-      const double s = a[(size_t)j * n + j];
-      if (!(s > 0)) *bad = 1;
-      a[(size_t)j * n + j] = sqrt(s);
+  if (n > CHOL_NMAX) {
+    for (int j = 0; j < n; j++) {
+      if (threadIdx.x == 0) {
+        const double s = a[(size_t)j * n + j];
+        if (!(s > 0)) *bad = 1;
+        a[(size_t)j * n + j] = sqrt(s);
+      }
+      __syncthreads();
+      if (*bad) return false;
+      const double l = a[(size_t)j * n + j];
+      for (int i = j + 1 + threadIdx.x; i < n; i += LM_T) a[(size_t)i * n + j] /= l;
+      __syncthreads();
+      // trailing update of the lower triangle, rows i > j, columns j < k <= i
+      const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+      for (int i = j + 1 + ty; i < n; i += 16) {
+        const double lij = a[(size_t)i * n + j];
+        for (int k = j + 1 + tx; k <= i; k += 16) a[(size_t)i * n + k] -= lij * a[(size_t)k * n + j];
+      }
+      __syncthreads();
+    }
+    return true;
+  }
+  __shared__ double P[CHOL_NMAX * CHOL_B];  // panel: P[(i - j0) * CHOL_B + c]
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int j0 = 0; j0 < n; j0 += CHOL_B) {
+    const int jb = n - j0 < CHOL_B ? n - j0 : CHOL_B;
+    const int m = n - j0;
+    for (int e = threadIdx.x; e < m * jb; e += LM_T) {
+      const int r = e / jb, c = e - r * jb;
+      P[r * CHOL_B + c] = a[(size_t)(j0 + r) * n + j0 + c];
     }
     __syncthreads();
-    if (*bad) return false;
-    const double l = a[(size_t)j * n + j];
-    for (int i = j + 1 + threadIdx.x; i < n; i += LM_T) {
-      const double v = a[(size_t)i * n + j] / l;
-      a[(size_t)i * n + j] = v;
-      if (staged) colj[i] = v;
+    for (int c = 0; c < jb; c++) {
+      const double s = P[c * CHOL_B + c];
+      if (!(s > 0)) {
+        if (threadIdx.x == 0) *bad = 1;
+        __syncthreads();
+        return false;
+      }
+      const double l = sqrt(s);
+      for (int r = c + 1 + threadIdx.x; r < m; r += LM_T) P[r * CHOL_B + c] /= l;
+      __syncthreads();
+      if (threadIdx.x == 0) P[c * CHOL_B + c] = l;
+      // the panel's later columns, rows r >= k
+      for (int e = threadIdx.x; e < m * (jb - c - 1); e += LM_T) {
+        const int r = e / (jb - c - 1), k = c + 1 + (e - r * (jb - c - 1));
+        if (r >= k) P[r * CHOL_B + k] -= P[r * CHOL_B + c] * P[k * CHOL_B + c];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    // trailing update of the lower triangle, rows i > j, columns j < k <= i,
-    // on a 16 x 16 thread tiling (each element once per step)
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    for (int i = j + 1 + ty; i < n; i += 16) {
+    // the panel's lower part back to a
+    for (int e = threadIdx.x; e < m * jb; e += LM_T) {
+      const int r = e / jb, c = e - r * jb;
+      if (r >= c) a[(size_t)(j0 + r) * n + j0 + c] = P[r * CHOL_B + c];
+    }
+    // trailing update of the lower triangle, rows i >= j0 + jb, columns
+    // j0 + jb <= k <= i, each element taking the panel's columns in order
+    const int t0 = j0 + jb;
+    for (int i = t0 + ty; i < n; i += 16) {
       double *ai = a + (size_t)i * n;
-      if (staged) {
-        const double lij = colj[i];
-        int k = j + 1 + tx;
-        for (; k + 48 <= i; k += 64) {
-          const double x0 = ai[k], x1 = ai[k + 16], x2 = ai[k + 32], x3 = ai[k + 48];
-          ai[k] = x0 - lij * colj[k];
-          ai[k + 16] = x1 - lij * colj[k + 16];
-          ai[k + 32] = x2 - lij * colj[k + 32];
-          ai[k + 48] = x3 - lij * colj[k + 48];
+      const double *pi = P + (size_t)(i - j0) * CHOL_B;
+      int k = t0 + tx;
+      for (; k + 48 <= i; k += 64) {
+        double x[4] = {ai[k], ai[k + 16], ai[k + 32], ai[k + 48]};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double *pk = P + (size_t)(k + 16 * q - j0) * CHOL_B;
+          for (int c = 0; c < jb; c++) x[q] -= pi[c] * pk[c];
         }
-        for (; k <= i; k += 16) ai[k] -= lij * colj[k];
-      } else {
-        const double lij = ai[j];
-        for (int k = j + 1 + tx; k <= i; k += 16) ai[k] -= lij * a[(size_t)k * n + j];
+        ai[k] = x[0];
+        ai[k + 16] = x[1];
+        ai[k + 32] = x[2];
+        ai[k + 48] = x[3];
+      }
+      for (; k <= i; k += 16) {
+        const double *pk = P + (size_t)(k - j0) * CHOL_B;
+        double x = ai[k];
+        for (int c = 0; c < jb; c++) x -= pi[c] * pk[c];
+        ai[k] = x;
       }
     }
     __syncthreads();
