@@ -1067,6 +1067,8 @@ static int formod_fov(const ctl_t *ctl, obs_t *obs) {
   return rc;
 }
 
+int orc_formod_fov(const ctl_t *ctl, obs_t *obs) { return formod_fov(ctl, obs); }
+
 /* formod (src/jurassic.c:3026-3071). */
 int orc_formod(const ctl_t *ctl, atm_t *atm, obs_t *obs) {
   if (ctl->nd < 0 || ctl->nd > ND || ctl->ng < 0 || ctl->ng > NG ||

@@ -84,6 +84,9 @@ double orc_intpol_col_u(const float *u, const float *eps, int nu, double x);
 /* Forward model (src/jurassic.c:3026-3071).  Returns 0 or a JUR_ERR_* code
    (the reference exits instead). */
 int orc_formod(const ctl_t *ctl, atm_t *atm, obs_t *obs);
+/* formod_fov alone on obs (src/jurassic.c:3121-3192), for its closed-form
+   pins; obs->rad and obs->tau are the pencil-beam values. */
+int orc_formod_fov(const ctl_t *ctl, obs_t *obs);
 
 /* The same over n independent packages, OpenMP across packages after one
    warm call (the CPU baseline). */

@@ -37,6 +37,8 @@ def lib():
         ip = C.POINTER(C.c_int)
         L.orc_formod.argtypes = [P, P, P]
         L.orc_formod.restype = I
+        L.orc_formod_fov.argtypes = [P, P]
+        L.orc_formod_fov.restype = I
         L.orc_formod_batch.argtypes = [P, P, P, I, I]
         L.orc_formod_batch.restype = I
         L.orc_kernel.argtypes = [P, P, P, dp, S, S, I]
