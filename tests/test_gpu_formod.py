@@ -331,3 +331,30 @@ def test_kernel_fd_matches_oracle(gpu, oracle):
     tol = 1e-9 * y[:, None] / h[None, :]
     assert np.all(np.abs(k1 - k0) <= tol)
     assert np.abs(k1 - k0).max() / np.abs(k0).max() < 1e-7
+
+
+def test_formod_gpu_and_kernel_gpu_dropins(gpu, oracle):
+    """The literal drop-ins: formod_gpu (void, ERRMSG-style errors) and
+    kernel_gpu (void, finite differences in FP64) on a valid nadir case,
+    against the oracle's formod and kernel()."""
+    L = gpu.lib()
+    L.formod_gpu.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.formod_gpu.restype = None
+    L.kernel_gpu.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t]
+    L.kernel_gpu.restype = None
+    ctl, atm, obs = scenes.nadir_co2()
+    ctl.write_bbt = 1
+    o_gpu, a_gpu = abi.copy_struct(obs), abi.copy_struct(atm)
+    L.formod_gpu(C.byref(ctl), C.byref(a_gpu), C.byref(o_gpu))
+    o_cpu = abi.copy_struct(obs)
+    oracle.formod(ctl, abi.copy_struct(atm), o_cpu)
+    _assert_obs(ctl, o_gpu, o_cpu)
+    ctl.rett_zmin, ctl.rett_zmax = -100, 100
+    m, n = oracle.sizes(ctl, atm, obs)
+    k0 = oracle.kernel(ctl, abi.copy_struct(atm), abi.copy_struct(obs))
+    k1 = np.zeros((m, n))
+    L.kernel_gpu(C.byref(ctl), C.byref(abi.copy_struct(atm)), C.byref(abi.copy_struct(obs)),
+                 k1.ctypes.data_as(C.POINTER(C.c_double)), m, n)
+    # BT differences over a 1 K step: entries agree to about 1e-10 BT / 1 K
+    assert np.abs(k1 - k0).max() <= 2e-7
+    assert np.abs(k0).max() > 0.01

@@ -1,5 +1,6 @@
 """Tables from device memory (jur_ctx_create_device_tables) against the same
-tables read from files (read_tbl, src/jurassic.c:4949-5110).
+tables read from files (read_tbl, src/jurassic.c:4949-5110) and against the
+oracle's formod on the same atmospheres and rays.
 
 The device path takes the arrays of the reference's tbl_t (channel-major,
 capacities TBLNP x TBLNT x TBLNU) as they are, as read_tbl takes a binary
@@ -96,18 +97,54 @@ def _limb_batch(ctl, torch, nprof=3):
     return _batch(ctl, nprof, lambda: scenes.limb_geometry(800.0, scenes.limb_tangent_heights()[::4]), torch)
 
 
+def _oracle_limb(ctl, nprof=3):
+    """The oracle's formod on the atmospheres and rays of _limb_batch, in
+    the batch's ray-major order: [nray][nd] radiance and transmittance."""
+    import oracle as O
+    geom = scenes.limb_geometry(800.0, scenes.limb_tangent_heights()[::4])
+    nr = len(geom["obsz"])
+    rad = np.zeros((nr * nprof, ctl.nd))
+    tau = np.zeros_like(rad)
+    for k in range(nprof):
+        atm = scenes.make_profile_atm([ctl.emitter[i].value.decode() for i in range(ctl.ng)], iprof=k)
+        obs = scenes.make_obs(geom)
+        O.formod(ctl, atm, obs)
+        r, t = abi.obs_arrays(obs, ctl)
+        rad[k::nprof] = r.T
+        tau[k::nprof] = t.T
+    return rad, tau
+
+
+def _bt(rad, nu):
+    c1, c2 = 1.19104259e-8, 1.43877506
+    nu = np.asarray(nu)[None, :]
+    return c2 * nu / np.log1p(c1 * nu ** 3 / rad)
+
+
 def _same_as_files(gpu, ctl, torch, filters=None, batch=None):
+    """Bit-identical to the context built from the files, and equal to the
+    oracle's formod on the same rays (FP64 to 1e-10 relative, FP32 to
+    0.01 K in brightness temperature)."""
     nu = [ctl.nu[i] for i in range(ctl.nd)]
     tabs = _tbl_arrays(ctl, torch)
+    own_batch = batch is None
     batch = batch or _limb_batch(ctl, torch)
     c_file = gpu.Context(ctl)
     c_dev = gpu.Context.from_device_tables(ctl, nu, tabs, filters=filters)
+    ref = _oracle_limb(ctl) if own_batch else None
     try:
         for prec in (abi.JUR_FP64, abi.JUR_FP32):
             r0, t0, p0 = _forward(c_file, ctl, batch, prec, torch)
             r1, t1, p1 = _forward(c_dev, ctl, batch, prec, torch)
             assert np.array_equal(r0, r1) and np.array_equal(t0, t1) and np.array_equal(p0, p1), prec
             assert np.isfinite(r0).all() and (r0 > 0).any()
+            if ref is None:
+                continue
+            if prec == abi.JUR_FP64:
+                np.testing.assert_allclose(r1, ref[0], rtol=1e-10, atol=0)
+                np.testing.assert_allclose(t1, ref[1], rtol=1e-10, atol=1e-300)
+            else:
+                assert float(np.max(np.abs(_bt(r1, nu) - _bt(ref[0], nu)))) < 0.01
     finally:
         c_file.close()
         c_dev.close()
