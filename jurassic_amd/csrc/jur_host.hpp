@@ -78,6 +78,33 @@ struct DevBuf {
   }
 };
 
+// Device work memory of one pass: the context's grow-only buffers (base ==
+// nullptr), or 256-byte aligned slices of a fixed scratch array (the
+// no-allocation entry point), handed out in order and released together by
+// resetting `used` to an earlier value.
+struct PassMem {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  template <typename T>
+  int get(DevBuf<T> &b, size_t n, T **out) {
+    if (!base) {
+      int rc = b.reserve(n);
+      if (rc) return rc;
+      *out = b.p;
+      return JUR_OK;
+    }
+    const size_t bytes = ((n ? n : 1) * sizeof(T) + 255) & ~(size_t)255;
+    if (bytes > cap - used)
+      return set_err(JUR_ERR_MEM,
+                     "the pass needs more than the %zu-byte scratch array of the no-allocation entry point "
+                     "(%zu bytes more at this step): use fewer rays per call or jur_ctx_formod_device",
+                     cap, bytes - (cap - used));
+    *out = (T *)(base + used);
+    used += bytes;
+    return JUR_OK;
+  }
+};
+
 // ------------------------------------------------------------------
 // Settings copied out of ctl_t (plus a channel list of any length).
 // ------------------------------------------------------------------
@@ -180,8 +207,10 @@ struct JacArgs {
 
 // One pass over a device batch: the forward model, or with jac the forward
 // model and its Jacobian (adjoint sweep).
+// mem: where the pass's work buffers live (nullptr: the context's buffers).
 int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned char *d_mask, double *d_rad,
-                 double *d_tau, double *d_tp, const FovPkg *fov, hipStream_t st, const JacArgs *jac = nullptr);
+                 double *d_tau, double *d_tp, const FovPkg *fov, hipStream_t st, const JacArgs *jac = nullptr,
+                 PassMem *mem = nullptr);
 
 // The cached device context of a ctl (tables read on first use).
 int ctx_for(const ctl_t *ctl, jur_ctx **out);

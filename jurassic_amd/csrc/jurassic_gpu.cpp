@@ -612,9 +612,12 @@ static int ctx_build(jur_ctx *c) {
 // One pass over a device batch: the forward model, or with jac the forward
 // model and its Jacobian (adjoint sweep).
 int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned char *d_mask, double *d_rad,
-                 double *d_tau, double *d_tp, const FovPkg *fov, hipStream_t st, const JacArgs *jac) {
+                 double *d_tau, double *d_tp, const FovPkg *fov, hipStream_t st, const JacArgs *jac,
+                 PassMem *mem) {
   const Setup &s = c->s;
   ModelView M = model_view(c);
+  PassMem own;
+  PassMem &pm = mem ? *mem : own;
   int rc;
   c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = 0;
   if (B.nray <= 0) return JUR_OK;
@@ -628,10 +631,11 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
   // of RAYDZ down and up plus horizontal steps of RAYDS along the longest
   // chord, with margin; rays that still exceed it are traced again with the
   // reference limit NLOS.
-  if ((rc = c->d_zr.reserve((size_t)2 * B.nprof))) return rc;
-  launch_zrange(B.atm, B.nprof, c->d_zr.p, st);
+  double *d_zr;
+  if ((rc = pm.get(c->d_zr, (size_t)2 * B.nprof, &d_zr))) return rc;
+  launch_zrange(B.atm, B.nprof, d_zr, st);
   std::vector<double> zr((size_t)2 * B.nprof);
-  HIPCHK(hipMemcpyAsync(zr.data(), c->d_zr.p, zr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(zr.data(), d_zr, zr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   double dz = 0, rmax = 0, rmin = 1e30;
   for (int i = 0; i < B.nprof; i++) {
@@ -646,11 +650,16 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
   int cap = (int)std::min((double)NLOS, est);
 
   const int nblk = (B.nray + WAVE - 1) / WAVE;
-  if ((rc = c->d_info.reserve(B.nray))) return rc;
-  if ((rc = c->d_tail.reserve((size_t)B.nray * ray_tail_bytes()))) return rc;
-  if ((rc = c->d_red.reserve(3))) return rc;
+  RayInfo *d_info;
+  char *d_tail;
+  unsigned long long *d_red;
+  if ((rc = pm.get(c->d_info, B.nray, &d_info))) return rc;
+  if ((rc = pm.get(c->d_tail, (size_t)B.nray * ray_tail_bytes(), &d_tail))) return rc;
+  if ((rc = pm.get(c->d_red, 3, &d_red))) return rc;
+  const size_t mark0 = pm.used;
 
   for (int attempt = 0; attempt < 2; attempt++) {
+    pm.used = mark0;
     LosView L{};
     L.cap = cap;
     L.f_k = F_Q0 + s.ng;
@@ -674,7 +683,12 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
                           (jac ? (size_t)s.nd * nscr * WAVE * sizeof(double) : 0);
     const size_t tile_blk = jac ? (size_t)s.nd * jac->ncol * WAVE * sizeof(double) : 0;
     size_t budget = c->los_budget;
-    if (budget == 0) {
+    if (pm.base) {
+      // the scratch array after the per-ray buffers, less the per-block
+      // offset arrays and the alignment of the slices
+      const size_t small = (size_t)nblk * (sizeof(int) + 2 * sizeof(size_t)) + 16 * 256;
+      budget = pm.cap - pm.used > small ? pm.cap - pm.used - small : 0;
+    } else if (budget == 0) {
       // half of the HBM free now (plus what these buffers already hold), at
       // most 96 GiB
       size_t fr = 0, tot = 0;
@@ -687,26 +701,32 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     // lookups and the radiance (or adjoint) pass then run over sub-chunks of
     // a chunk sized by the blocks' actual LOS lengths.
     int chunk = (int)std::max<size_t>(1, std::min<size_t>(nblk, (budget / 2) / (los_blk + 1)));
-    const size_t rest = budget > los_blk * chunk ? budget - los_blk * chunk : 0;
-    if ((rc = c->d_los.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
-    if (losf && (rc = c->d_losf.reserve((size_t)L.nfield * L.cap * WAVE * chunk))) return rc;
-    if ((rc = c->d_npb.reserve(chunk))) return rc;
-    if ((rc = c->d_eoff.reserve(chunk))) return rc;
-    if (jac && (rc = c->d_soff.reserve(chunk))) return rc;
-    L.los = c->d_los.p;
-    L.losf = losf ? c->d_losf.p : nullptr;
+    size_t rest = budget > los_blk * chunk ? budget - los_blk * chunk : 0;
+    double *d_los;
+    float *d_losf = nullptr;
+    int *d_npb;
+    size_t *d_eoff, *d_soff = nullptr;
+    if ((rc = pm.get(c->d_npb, chunk, &d_npb))) return rc;
+    if ((rc = pm.get(c->d_eoff, chunk, &d_eoff))) return rc;
+    if (jac && (rc = pm.get(c->d_soff, chunk, &d_soff))) return rc;
+    if ((rc = pm.get(c->d_los, (size_t)L.nfield * L.cap * WAVE * chunk, &d_los))) return rc;
+    if (losf && (rc = pm.get(c->d_losf, (size_t)L.nfield * L.cap * WAVE * chunk, &d_losf))) return rc;
+    if (pm.base) rest = pm.cap - pm.used;
+    const size_t mark1 = pm.used;
+    L.los = d_los;
+    L.losf = d_losf;
     std::vector<int> npb(chunk);
     std::vector<size_t> eoff(chunk), soff(chunk);
     float ms[3] = {0, 0, 0};
     for (int b0 = 0; b0 < nblk; b0 += chunk) {
       const int nb = std::min(chunk, nblk - b0);
       HIPCHK(hipEventRecord(c->ev[0], st));
-      launch_raytrace(M, B, L, c->d_info.p, c->d_tail.p, b0, nb, st);
+      launch_raytrace(M, B, L, d_info, d_tail, b0, nb, st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->ev[1], st));
-      launch_block_np(c->d_info.p, B.nray, b0, nb, c->d_npb.p, st);
+      launch_block_np(d_info, B.nray, b0, nb, d_npb, st);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(npb.data(), c->d_npb.p, nb * sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(npb.data(), d_npb, nb * sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       {
         float a = 0;
@@ -718,7 +738,8 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
         int s1 = s0;
         size_t tot = 0, stot = 0, mem = 0;
         while (s1 < nb) {
-          const size_t need = (size_t)npb[s1] * pt_blk + tile_blk;
+          // in scratch mode each slice is rounded up to 256 bytes
+          const size_t need = (size_t)npb[s1] * pt_blk + tile_blk + (pm.base ? 5 * 256 : 0);
           if (s1 > s0 && mem + need > rest) break;
           eoff[s1 - s0] = tot;
           tot += (size_t)s.nd * s.ng * npb[s1] * WAVE;
@@ -728,35 +749,39 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
           s1++;
         }
         const int ns = s1 - s0;
-        if ((rc = c->d_eps.reserve(tot * esz + 1))) return rc;
-        if (part && (rc = c->d_part.reserve(tot + 1))) return rc;
-        HIPCHK(hipMemcpyAsync(c->d_eoff.p, eoff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
+        pm.used = mark1;
+        char *d_eps;
+        float4 *d_part = nullptr;
+        if ((rc = pm.get(c->d_eps, tot * esz + 1, &d_eps))) return rc;
+        if (part && (rc = pm.get(c->d_part, tot + 1, &d_part))) return rc;
+        HIPCHK(hipMemcpyAsync(d_eoff, eoff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
         LosView Ls = L;
         Ls.los = L.los + (size_t)s0 * L.cap * L.nfield * WAVE;
         if (Ls.losf) Ls.losf = L.losf + (size_t)s0 * L.cap * L.nfield * WAVE;
-        EpsView E{c->d_eps.p, c->d_eoff.p, c->d_npb.p + s0, part ? c->d_part.p : nullptr};
+        EpsView E{d_eps, d_eoff, d_npb + s0, d_part};
         const int bs = b0 + s0;
         HIPCHK(hipEventRecord(c->ev[1], st));
-        if (precision == JUR_FP32) launch_lookup_f32(M, B, Ls, E, c->d_info.p, bs, ns, st);
-        else launch_lookup_f64(M, B, Ls, E, c->d_info.p, bs, ns, jac != nullptr, st);
+        if (precision == JUR_FP32) launch_lookup_f32(M, B, Ls, E, d_info, bs, ns, st);
+        else launch_lookup_f64(M, B, Ls, E, d_info, bs, ns, jac != nullptr, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[2], st));
         if (jac) {
-          if ((rc = c->d_scr.reserve(stot + 1))) return rc;
-          if ((rc = c->d_kt.reserve((size_t)ns * s.nd * jac->ncol * WAVE + 1))) return rc;
-          if ((rc = c->d_nef.reserve((size_t)ns * s.nd * WAVE + 1))) return rc;
-          HIPCHK(hipMemcpyAsync(c->d_soff.p, soff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
-          launch_zero(c->d_kt.p, (size_t)ns * s.nd * jac->ncol * WAVE, st);
-          JacView J{jac->xmap, jac->nmap, jac->prof_map, jac->ncol, c->d_kt.p, c->d_scr.p, c->d_soff.p, nscr,
-                    c->d_nef.p};
-          launch_adjoint(M, B, Ls, E, c->d_info.p, bs, ns, J, precision == JUR_FP32, d_rad, d_tau, st);
+          double *d_scr, *d_kt;
+          int *d_nef;
+          if ((rc = pm.get(c->d_scr, stot + 1, &d_scr))) return rc;
+          if ((rc = pm.get(c->d_kt, (size_t)ns * s.nd * jac->ncol * WAVE + 1, &d_kt))) return rc;
+          if ((rc = pm.get(c->d_nef, (size_t)ns * s.nd * WAVE + 1, &d_nef))) return rc;
+          HIPCHK(hipMemcpyAsync(d_soff, soff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
+          launch_zero(d_kt, (size_t)ns * s.nd * jac->ncol * WAVE, st);
+          JacView J{jac->xmap, jac->nmap, jac->prof_map, jac->ncol, d_kt, d_scr, d_soff, nscr, d_nef};
+          launch_adjoint(M, B, Ls, E, d_info, bs, ns, J, precision == JUR_FP32, d_rad, d_tau, st);
           HIPCHK(hipGetLastError());
-          launch_jac_store(c->d_kt.p, s.nd, jac->ncol, bs, ns, B.nray, d_rad, c->d_chan.p, s.write_bbt,
-                           d_mask, jac->k, st);
+          launch_jac_store(d_kt, s.nd, jac->ncol, bs, ns, B.nray, d_rad, c->d_chan.p, s.write_bbt, d_mask,
+                           jac->k, st);
         } else if (precision == JUR_FP32) {
-          launch_radiance_f32(M, B, Ls, E, c->d_info.p, bs, ns, d_rad, d_tau, st);
+          launch_radiance_f32(M, B, Ls, E, d_info, bs, ns, d_rad, d_tau, st);
         } else {
-          launch_radiance_f64(M, B, Ls, E, c->d_info.p, bs, ns, d_rad, d_tau, st);
+          launch_radiance_f64(M, B, Ls, E, d_info, bs, ns, d_rad, d_tau, st);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[3], st));
@@ -769,10 +794,10 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
         s0 = s1;
       }
     }
-    launch_reduce_info(c->d_info.p, B.nray, c->d_red.p, st);
+    launch_reduce_info(d_info, B.nray, d_red, st);
     HIPCHK(hipGetLastError());
     unsigned long long red[3];
-    HIPCHK(hipMemcpyAsync(red, c->d_red.p, sizeof red, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(red, d_red, sizeof red, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     for (int k = 0; k < 3; k++) c->last_ms[k] = ms[k];
     c->last_mean_np = (double)red[1] / B.nray;
@@ -785,28 +810,31 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     if (red[0] != 0) return set_err(JUR_ERR_ARG, "ray tracing failed (status %llu)", red[0]);
     break;
   }
+  pm.used = mark0;
 
   // tangent points
   if (d_tp) {
     // gather tpz/tplon/tplat from RayInfo with a strided device copy
-    HIPCHK(hipMemcpy2DAsync(d_tp, 3 * sizeof(double), (const char *)c->d_info.p + offsetof(RayInfo, tpz),
+    HIPCHK(hipMemcpy2DAsync(d_tp, 3 * sizeof(double), (const char *)d_info + offsetof(RayInfo, tpz),
                             sizeof(RayInfo), 3 * sizeof(double), B.nray, hipMemcpyDeviceToDevice, st));
   }
 
   // field-of-view convolution (before the brightness temperature)
   if (fov && fov->lo && c->fov_dz.size()) {
-    if ((rc = c->d_rad0.reserve((size_t)B.nray * s.nd))) return rc;
-    if ((rc = c->d_tau0.reserve((size_t)B.nray * s.nd))) return rc;
-    if ((rc = c->d_err.reserve(1))) return rc;
-    HIPCHK(hipMemcpyAsync(c->d_rad0.p, d_rad, (size_t)B.nray * s.nd * sizeof(double), hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(c->d_tau0.p, d_tau, (size_t)B.nray * s.nd * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double *d_rad0, *d_tau0;
+    int *d_err;
+    if ((rc = pm.get(c->d_rad0, (size_t)B.nray * s.nd, &d_rad0))) return rc;
+    if ((rc = pm.get(c->d_tau0, (size_t)B.nray * s.nd, &d_tau0))) return rc;
+    if ((rc = pm.get(c->d_err, 1, &d_err))) return rc;
+    HIPCHK(hipMemcpyAsync(d_rad0, d_rad, (size_t)B.nray * s.nd * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_tau0, d_tau, (size_t)B.nray * s.nd * sizeof(double), hipMemcpyDeviceToDevice, st));
     int zero = 0;
-    HIPCHK(hipMemcpyAsync(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice, st));
-    launch_fov(c->d_rad0.p, c->d_tau0.p, d_rad, d_tau, B.nray, s.nd, B.ray_geom, fov->lo, fov->hi,
-               c->d_fov_dz.p, c->d_fov_w.p, (int)c->fov_dz.size(), c->d_err.p, st);
+    HIPCHK(hipMemcpyAsync(d_err, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+    launch_fov(d_rad0, d_tau0, d_rad, d_tau, B.nray, s.nd, B.ray_geom, fov->lo, fov->hi, c->d_fov_dz.p,
+               c->d_fov_w.p, (int)c->fov_dz.size(), d_err, st);
     HIPCHK(hipGetLastError());
     int err = 0;
-    HIPCHK(hipMemcpyAsync(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&err, d_err, sizeof err, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (err) return set_err(JUR_ERR_FOV, "Cannot apply FOV convolution!");
   }
