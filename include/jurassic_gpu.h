@@ -500,6 +500,22 @@ int jur_ctx_last_timing(const jur_ctx_t *ctx, double *ms, int n);
 /* Mean number of LOS points per ray of the last pass. */
 double jur_ctx_last_mean_np(const jur_ctx_t *ctx);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): jur_ctx_formod_device with every work buffer (line of
+   sight, emissivities, per-ray state) taken from a fixed device array of
+   JUR_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until the
+   process ends; the call itself allocates no device memory.  Same inputs,
+   outputs and results as jur_ctx_formod_device.  Large batches are processed
+   in chunks of ray blocks; a batch whose per-ray buffers, or one block of
+   64 rays, do not fit the array is rejected with JUR_ERR_MEM and a message
+   (few channels and emitters fit: e.g. 12 channels x 3 emitters, limb).
+   The context must belong to the current device.  Calls that overlap on one
+   GPU (other threads or streams) share the array and race. */
+#define JUR_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int jur_ctx_formod_device_noalloc(jur_ctx_t *ctx, const jur_batch_t *batch,
+                                  int precision, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

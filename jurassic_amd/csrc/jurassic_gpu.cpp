@@ -845,6 +845,35 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
 }
 
 // ------------------------------------------------------------------
+// Scratch array of the no-allocation entry point
+// (jur_ctx_formod_device_noalloc).  Each GPU holds its own copy at its own
+// address; host code uses that address, looked up once per device, never
+// the array's name.
+// ------------------------------------------------------------------
+}  // namespace jur
+
+static __device__ __align__(256) unsigned char scratch[JUR_NOALLOC_SCRATCH_BYTES];
+
+namespace jur {
+
+static std::mutex g_scratch_mu;
+static char *g_scratch_dev[256];
+
+// Device address of the scratch array on device dev (the current device).
+static int noalloc_scratch(int dev, char **out) {
+  if (dev < 0 || dev >= 256) return set_err(JUR_ERR_ARG, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(JUR_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_dev[dev] = (char *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return JUR_OK;
+}
+
+// ------------------------------------------------------------------
 // Context cache for the ctl_t entry points.
 // ------------------------------------------------------------------
 static std::mutex g_mu;
@@ -1411,6 +1440,25 @@ int jur_ctx_formod_device(jur_ctx_t *ctx, const jur_batch_t *b, int precision, v
   BatchView B{b->nprof, b->atm, b->nray, b->ray_prof, b->ray_geom};
   return forward_pass(ctx, B, precision, b->mask, b->rad, b->tau, b->tp, nullptr,
                       (hipStream_t)stream);
+}
+
+int jur_ctx_formod_device_noalloc(jur_ctx_t *ctx, const jur_batch_t *b, int precision, void *stream) {
+  if (!ctx || !b) return set_err(JUR_ERR_ARG, "NULL argument");
+  if (precision != JUR_FP64 && precision != JUR_FP32) return set_err(JUR_ERR_ARG, "invalid precision");
+  if (b->nray < 0 || b->nprof < 0 || (b->nray > 0 && (!b->atm || !b->ray_prof || !b->ray_geom ||
+                                                      !b->rad || !b->tau)))
+    return set_err(JUR_ERR_ARG, "invalid batch");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != ctx->device)
+    return set_err(JUR_ERR_ARG, "the context belongs to device %d, the current device is %d", ctx->device, dev);
+  PassMem pm;
+  int rc = noalloc_scratch(dev, &pm.base);
+  if (rc) return rc;
+  pm.cap = JUR_NOALLOC_SCRATCH_BYTES;
+  BatchView B{b->nprof, b->atm, b->nray, b->ray_prof, b->ray_geom};
+  return forward_pass(ctx, B, precision, b->mask, b->rad, b->tau, b->tp, nullptr, (hipStream_t)stream, nullptr,
+                      &pm);
 }
 
 int jur_state_map(const ctl_t *ctl, const atm_t *atm, int *xmap) {

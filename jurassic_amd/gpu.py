@@ -101,6 +101,8 @@ def lib():
         L.jur_ctx_nd.restype = I
         L.jur_ctx_formod_device.argtypes = [P, C.POINTER(jur_batch_t), I, P]
         L.jur_ctx_formod_device.restype = I
+        L.jur_ctx_formod_device_noalloc.argtypes = [P, C.POINTER(jur_batch_t), I, P]
+        L.jur_ctx_formod_device_noalloc.restype = I
         L.jur_ctx_last_timing.argtypes = [P, C.POINTER(D), I]
         L.jur_ctx_last_timing.restype = I
         L.jur_ctx_last_mean_np.argtypes = [P]
@@ -322,6 +324,14 @@ class Context:
         b = jur_batch_t(nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, mask_ptr, rad_ptr,
                         tau_ptr, tp_ptr)
         _check(lib().jur_ctx_formod_device(self.ptr, C.byref(b), precision, stream))
+
+    def formod_device_noalloc(self, nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, rad_ptr, tau_ptr,
+                              tp_ptr=None, mask_ptr=None, precision=abi.JUR_FP64, stream=None):
+        """formod_device with the work buffers in the library's fixed
+        scratch array (no device allocation in the call)."""
+        b = jur_batch_t(nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, mask_ptr, rad_ptr,
+                        tau_ptr, tp_ptr)
+        _check(lib().jur_ctx_formod_device_noalloc(self.ptr, C.byref(b), precision, stream))
 
     def jacobian_device(self, nprof, atm_ptr, nray, ray_prof_ptr, ray_geom_ptr, rad_ptr, tau_ptr,
                         xmaps, prof_map_ptr, ncol, k_ptr, tp_ptr=None, mask_ptr=None,
