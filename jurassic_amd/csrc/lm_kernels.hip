@@ -9,7 +9,7 @@
 //                into a compact m x n matrix, rows scaled by 1/sigma_eps
 //                (rows of masked measurements are zero)
 //  k_lm_normal   C = (S^-1/2 K)^T (S^-1/2 K), lower triangle by 64x64 LDS
-//                tiles, mirrored
+//                tiles on the FP64 matrix cores, mirrored
 //  k_lm_cholinv  S_a^-1 (matrix_invert, src/retrieval.c:373-399): Cholesky,
 //                then one column of the inverse per thread
 //  k_lm_rhs      b = K^T S_eps^-1 dy - S_a^-1 dx
@@ -21,8 +21,9 @@
 //  k_ana_gain    gain matrix G = cov K^T S_eps^-1 = cov Kc^T diag(1/sigma)
 //  k_ana_avk     averaging kernels A = G K = cov C into the work buffer
 //
-// Cholesky is the right-looking column algorithm in global memory (the
-// matrices stay in L2: n <= 324 is 0.8 MB); every step is one barrier.
+// Cholesky is right-looking by 16-column panels: the panel factored in LDS,
+// the trailing matrix (in global memory, which L2 holds: n <= 324 is 0.8 MB)
+// updated by 16 x 16 tiles on the FP64 matrix cores.
 #include "jur_device.hpp"
 
 namespace jur {
@@ -42,8 +43,17 @@ __global__ __launch_bounds__(LM_T) void k_lm_gather(const LmDesc *D, const int *
   for (int j = threadIdx.x; j < d.n; j += LM_T) dst[j] = (s != 0) ? src[j] * s : 0.0;
 }
 
-// C = Kc^T Kc: one workgroup per 64x64 tile (ti >= tj), 256 threads with a
-// 4x4 block each, rows streamed through LDS 32 at a time.
+// FP64 matrix-core tile: D (16 x 16) += A (16 x 4) B (4 x 16) with
+// v_mfma_f64_16x16x4_f64.  Lane l holds A[l % 16][l / 16], B[l / 16][l % 16]
+// and D[l / 16 + 4 v][l % 16] in its v-th accumulator.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ dbl4 mfma_f64(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// C = Kc^T Kc: one workgroup per 64x64 tile (ti >= tj), rows of Kc streamed
+// through LDS 32 at a time; each of the four wavefronts forms a 32 x 32
+// quadrant as 2 x 2 tiles of 16 x 16 on the FP64 matrix cores.
 __global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *list) {
   __shared__ double a[32][65], b[32][65];
   const LmDesc d = D[list[blockIdx.y]];
@@ -56,12 +66,14 @@ __global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *
   }
   const int tj = t;
   if (ti >= nt) return;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const int ii = lane & 15, kq = lane >> 4;
+  const int qi = 32 * (wave >> 1), qj = 32 * (wave & 1);
+  dbl4 acc[2][2];
 #pragma unroll
-  for (int p = 0; p < 4; p++)
+  for (int p = 0; p < 2; p++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[p][q] = 0;
+    for (int q = 0; q < 2; q++) acc[p][q] = dbl4{0.0, 0.0, 0.0, 0.0};
   const double *K = d.k;
   const int i0 = ti * 64, j0 = tj * 64;
   for (int r0 = 0; r0 < d.m; r0 += 32) {
@@ -73,29 +85,29 @@ __global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *
       b[rr][cc] = (r < d.m && j0 + cc < d.n) ? K[(size_t)r * d.n + j0 + cc] : 0.0;
     }
     __syncthreads();
-    for (int rr = 0; rr < 32; rr++) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int p = 0; p < 4; p++) av[p] = a[rr][ty * 4 + p];
-#pragma unroll
-      for (int q = 0; q < 4; q++) bv[q] = b[rr][tx * 4 + q];
-#pragma unroll
-      for (int p = 0; p < 4; p++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) acc[p][q] += av[p] * bv[q];
+#pragma unroll 2
+    for (int k4 = 0; k4 < 32; k4 += 4) {
+      const double av0 = a[k4 + kq][qi + ii], av1 = a[k4 + kq][qi + 16 + ii];
+      const double bv0 = b[k4 + kq][qj + ii], bv1 = b[k4 + kq][qj + 16 + ii];
+      acc[0][0] = mfma_f64(av0, bv0, acc[0][0]);
+      acc[0][1] = mfma_f64(av0, bv1, acc[0][1]);
+      acc[1][0] = mfma_f64(av1, bv0, acc[1][0]);
+      acc[1][1] = mfma_f64(av1, bv1, acc[1][1]);
     }
   }
   double *C = d.cmat;
 #pragma unroll
-  for (int p = 0; p < 4; p++)
+  for (int p = 0; p < 2; p++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int i = i0 + ty * 4 + p, j = j0 + tx * 4 + q;
-      if (i < d.n && j < d.n) {
-        C[(size_t)i * d.n + j] = acc[p][q];
-        C[(size_t)j * d.n + i] = acc[p][q];
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = i0 + qi + 16 * p + kq + 4 * v, j = j0 + qj + 16 * q + ii;
+        if (i < d.n && j < d.n) {
+          C[(size_t)i * d.n + j] = acc[p][q][v];
+          C[(size_t)j * d.n + i] = acc[p][q][v];
+        }
       }
-    }
 }
 
 // Largest n whose Cholesky and triangular solves use the staged paths below
@@ -103,19 +115,19 @@ __global__ __launch_bounds__(LM_T) void k_lm_normal(const LmDesc *D, const int *
 constexpr int CHOL_NMAX = 512;
 
 // In-place Cholesky A = L L^T of an n x n row-major matrix (lower triangle
-// of a holds L; the upper triangle is left as it was).  Returns false if a
-// pivot is not positive.  Called by all threads of the workgroup.
+// of a holds L; of the upper triangle only the diagonal 16 x 16 blocks may
+// change).  Returns false if a pivot is not positive.  Called by all
+// threads of the workgroup.
 //
-// For n <= CHOL_NMAX: right-looking by panels of CHOL_B columns.  The panel
-// (rows j0..n-1) is factored in LDS, column by column, then written back,
-// and the trailing matrix takes the panel's CHOL_B updates per element in
-// one read-modify-write.  Every element sees the same operations in the
-// same order as in the column-by-column algorithm (each update subtracted
-// in column order), so the factor is the same to the bit; the workgroup
-// waits on global memory twice per panel instead of three times per
-// column.  (Owning each element by one thread for the whole factorisation,
-// with LDS-only barriers, measured slower.)
-constexpr int CHOL_B = 8;
+// For n <= CHOL_NMAX: right-looking by panels of 16 columns.  The panel
+// (rows j0..n-1) is staged in LDS column-major and factored there by the
+// first wavefront alone (no workgroup barrier per column); then the four
+// wavefronts update the trailing matrix by 16 x 16 tiles on the FP64 matrix
+// cores: A22 -= P P^T as four 16 x 16 x 4 products per tile, read and written
+// once per panel.  The sums of the 16 panel products per element are formed
+// in the matrix core's order, so L agrees with the column-by-column
+// algorithm to rounding.
+constexpr int CHOL_B = 16;
 
 __device__ bool chol_wg(double *a, int n, int *bad) {
   if (n > CHOL_NMAX) {
@@ -140,63 +152,77 @@ __device__ bool chol_wg(double *a, int n, int *bad) {
     }
     return true;
   }
-  __shared__ double P[CHOL_NMAX * CHOL_B];  // panel: P[(i - j0) * CHOL_B + c]
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  // panel P[c][r] = a[j0 + r][j0 + c], rows padded with zeros to whole tiles
+  __shared__ double P[CHOL_B][CHOL_NMAX];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   for (int j0 = 0; j0 < n; j0 += CHOL_B) {
     const int jb = n - j0 < CHOL_B ? n - j0 : CHOL_B;
     const int m = n - j0;
-    for (int e = threadIdx.x; e < m * jb; e += LM_T) {
-      const int r = e / jb, c = e - r * jb;
-      P[r * CHOL_B + c] = a[(size_t)(j0 + r) * n + j0 + c];
+    const int mp = (m + 15) & ~15;
+    for (int e = threadIdx.x; e < CHOL_B * mp; e += LM_T) {
+      const int c = e / mp, r = e - c * mp;
+      P[c][r] = (c < jb && r < m) ? a[(size_t)(j0 + r) * n + j0 + c] : 0.0;
     }
     __syncthreads();
-    for (int c = 0; c < jb; c++) {
-      const double s = P[c * CHOL_B + c];
-      if (!(s > 0)) {
-        if (threadIdx.x == 0) *bad = 1;
-        __syncthreads();
-        return false;
+    if (wave == 0) {
+      for (int c = 0; c < jb; c++) {
+        const double s = P[c][c];
+        if (!(s > 0)) {
+          if (lane == 0) *bad = 1;
+          break;
+        }
+        const double l = sqrt(s);
+        for (int r = c + 1 + lane; r < m; r += WAVE) P[c][r] /= l;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) P[c][c] = l;
+        // the panel's later columns, rows r >= k
+        for (int k = c + 1; k < jb; k++) {
+          const double pk = P[c][k];
+          for (int r = k + lane; r < m; r += WAVE) P[k][r] -= P[c][r] * pk;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      const double l = sqrt(s);
-      for (int r = c + 1 + threadIdx.x; r < m; r += LM_T) P[r * CHOL_B + c] /= l;
-      __syncthreads();
-      if (threadIdx.x == 0) P[c * CHOL_B + c] = l;
-      // the panel's later columns, rows r >= k
-      for (int e = threadIdx.x; e < m * (jb - c - 1); e += LM_T) {
-        const int r = e / (jb - c - 1), k = c + 1 + (e - r * (jb - c - 1));
-        if (r >= k) P[r * CHOL_B + k] -= P[r * CHOL_B + c] * P[k * CHOL_B + c];
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    if (*bad) return false;
     // the panel's lower part back to a
     for (int e = threadIdx.x; e < m * jb; e += LM_T) {
       const int r = e / jb, c = e - r * jb;
-      if (r >= c) a[(size_t)(j0 + r) * n + j0 + c] = P[r * CHOL_B + c];
+      if (r >= c) a[(size_t)(j0 + r) * n + j0 + c] = P[c][r];
     }
-    // trailing update of the lower triangle, rows i >= j0 + jb, columns
-    // j0 + jb <= k <= i, each element taking the panel's columns in order
+    // trailing update of the tiles (bi, bk), bi >= bk, of rows and columns
+    // t0 + 16 bi .., t0 + 16 bk ..
     const int t0 = j0 + jb;
-    for (int i = t0 + ty; i < n; i += 16) {
-      double *ai = a + (size_t)i * n;
-      const double *pi = P + (size_t)(i - j0) * CHOL_B;
-      int k = t0 + tx;
-      for (; k + 48 <= i; k += 64) {
-        double x[4] = {ai[k], ai[k + 16], ai[k + 32], ai[k + 48]};
+    if (t0 < n) {
+      const int nb = (n - t0 + 15) / 16;
+      const int ntile = nb * (nb + 1) / 2;
+      const int ii = lane & 15, kq = lane >> 4;
+      for (int t = wave; t < ntile; t += LM_T / WAVE) {
+        int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
+        while (bi * (bi + 1) / 2 > t) bi--;
+        const int bk = t - bi * (bi + 1) / 2;
+        const int ri = t0 + 16 * bi, ck = t0 + 16 * bk;
+        dbl4 acc;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const double *pk = P + (size_t)(k + 16 * q - j0) * CHOL_B;
-          for (int c = 0; c < jb; c++) x[q] -= pi[c] * pk[c];
+        for (int v = 0; v < 4; v++) {
+          const int i = ri + kq + 4 * v, k = ck + ii;
+          acc[v] = (i < n && k < n) ? a[(size_t)i * n + k] : 0.0;
         }
-        ai[k] = x[0];
-        ai[k + 16] = x[1];
-        ai[k + 32] = x[2];
-        ai[k + 48] = x[3];
-      }
-      for (; k <= i; k += 16) {
-        const double *pk = P + (size_t)(k - j0) * CHOL_B;
-        double x = ai[k];
-        for (int c = 0; c < jb; c++) x -= pi[c] * pk[c];
-        ai[k] = x;
+#pragma unroll
+        for (int s4 = 0; s4 < CHOL_B / 4; s4++) {
+          const int c = 4 * s4 + kq;
+          acc = mfma_f64(-P[c][ri - j0 + ii], P[c][ck - j0 + ii], acc);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int i = ri + kq + 4 * v, k = ck + ii;
+          if (i < n && k < n) a[(size_t)i * n + k] = acc[v];
+        }
       }
     }
     __syncthreads();

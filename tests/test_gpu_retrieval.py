@@ -164,3 +164,40 @@ def test_oe_error_analysis_adjoint(gpu, oracle, precision):
     assert np.all(a1["err_total"] > 0)
     assert np.all(a1["err_noise"] <= a1["err_total"] * (1 + 1e-9))
     assert 0 < np.trace(avk) < avk.shape[0]
+
+
+@pytest.mark.parametrize("gases_ret", [(0,), (0, 1)], ids=["n512", "n768"])
+def test_oe_fd_large_state_matches_oracle(gpu, oracle, gases_ret):
+    """State sizes at and above the staged dense-algebra limit (CHOL_NMAX =
+    512): T and one or two vmr profiles on 256 levels (n = 512 and 768), two
+    nadir rays; the FD iteration against the oracle's as above."""
+    gases, nus = ["CO2", "H2O"], [667.8, 700.0]
+    tb = scenes.write_scene_tables(scenes.scene_dir("oe_large"), gases, nus)
+    ctl = abi.make_ctl(gases, nus, tblbase=tb, write_bbt=1, rett_zmin=-100, rett_zmax=100)
+    for ig in gases_ret:
+        ctl.retq_zmin[ig], ctl.retq_zmax[ig] = -100, 100
+    z = np.linspace(0.0, 80.0, 256)
+    atm_true = scenes.make_profile_atm(gases, iprof=5, z=z)
+    atm_apr = scenes.make_profile_atm(gases, z=z, perturb=False)
+    obs = scenes.make_obs(scenes.nadir_geometry(705.0, [0.0, 30.0]))
+    ret = abi.ret_t()
+    ret.kernel_recomp = 3
+    ret.conv_itmax = 4
+    ret.conv_dmin = 1e-4
+    for i in range(ctl.nd):
+        ret.err_noise[i] = np.sqrt(0.1)
+    ret.err_temp, ret.err_temp_cz, ret.err_temp_ch = 2.0, 5.0, 500.0
+    for ig in range(2):
+        ret.err_q[ig], ret.err_q_cz[ig], ret.err_q_ch[ig] = 20.0, 5.0, 500.0
+    meas = _measure(oracle, ctl, atm_true, obs)
+    m, n = oracle.sizes(ctl, atm_apr, meas)
+    assert n == 256 * (1 + len(gases_ret))
+    a0, _, h0, n0 = oracle.optimal_estimation(ret, ctl, abi.copy_struct(meas), abi.copy_struct(atm_apr))
+    a1, _, h1, n1 = gpu.optimal_estimation(ret, ctl, abi.copy_struct(meas), abi.copy_struct(atm_apr),
+                                           method=abi.JUR_JAC_FD, precision=abi.JUR_FP64)
+    assert n0 >= 3
+    assert n1 == n0 and len(h1) == len(h0)
+    np.testing.assert_allclose(h1, h0, rtol=1e-6)
+    for v1, v0 in zip(_state(ctl, a1, gases_ret), _state(ctl, a0, gases_ret)):
+        np.testing.assert_allclose(v1, v0, rtol=1e-6, atol=1e-12)
+    assert h1[-1] < h1[0]
