@@ -9,19 +9,21 @@ H2O and O3), 32 tangent heights 5..67 km seen from 800 km, 12 channels
 with atmospheres and geometry already resident in HBM.
 
 The JSON line reports whole-job radiances/s of the FP32 path (FP64 ray
-geometry), the FP64 path's rate beside it, the radiance kernel's
-algorithmic bytes against the HBM roofline (HIP events on the library's
-stream), and the CPU FP64 oracle timed on this host's cores on a bounded
-sample of the same workload.
+geometry), the FP64 path's rate beside it, the roofline of the dominant
+kernel (the FP32 table lookups; its time from HIP events on the library's
+stream, its instruction and HBM byte counts from a committed rocprofv3
+counter profile), and the CPU FP64 oracle timed on this host's cores on a
+bounded sample of the same workload.
 
 Jacobians (the "jacobian" object): nadir sounder, 16,384 atmospheres x 8
 view angles 0..50 deg x 96 channels 650..1600 cm^-1 (CO2, H2O, O3; EGA,
 continua, refraction), state T and the three volume mixing ratios on 81
 levels (n = 324, m = 768 per atmosphere).  One step is one adjoint pass
 (forward model and all K rows, K written to HBM).  Beside it: the FD path
-that kernel() runs, n + 1 = 325 FP64 forward passes per Jacobian (its rate
-is timed as one FP64 forward pass over the same batch / 325), and the CPU
-oracle kernel() (OpenMP over the 324 columns) on a sample of atmospheres.
+that kernel() runs, n + 1 = 325 FP64 forward models per Jacobian, measured
+on 64 of the atmospheres in one device pass (perturbations and differences
+on the device), its K compared with the adjoint's, and the CPU oracle
+kernel() (OpenMP over the 324 columns) on a sample of atmospheres.
 The FP32 adjoint's error is stated against the FP64 adjoint on a subset:
 relative L2 per state block and the brightness temperature of the forward
 part.
@@ -48,12 +50,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "radiances/s fwd; Jacobians/s adjoint vs FD; p50/p99 retrieval latency"
 HBM_PEAK = 8.0e12
-# Measured HBM bytes per k_lookup_f32 launch at the default workload (4096
-# profiles): rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a
-# 1 GiB device copy under the same counters), in
-# profiles/k_lookup_f32_limb_4096x32x12_hbm_traffic.md.  Other workload sizes
-# report null.
-LOOKUP_F32_TRAFFIC = (4096, 38.85e9)
+# Vector-instruction issue peak of one MI355X: 256 CUs x 2.4 GHz, one wave64
+# VALU instruction per CU per cycle (four SIMD16 units, four cycles each).
+VALU_PEAK = 256 * 2.4e9
+# Counters of one k_lookup_f32 launch at the default workload (4096
+# profiles), from rocprofv3 --pmc runs of this script
+# (profiles/k_lookup_f32_limb_4096x32x12_counters.md): SQ_INSTS_VALU and
+# SQ_INSTS_VMEM_RD summed over the launch (wave instructions), and HBM bytes
+# = FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB x 1024 (factors from a 1 GiB
+# device copy under the same counters).  Other workload sizes report null.
+LOOKUP_F32_COUNTERS = {"nprof": 4096, "valu": 3.0198e10, "vmem_rd": 1.3640e9, "hbm_bytes": 39.37e9,
+                       "profile": "profiles/k_lookup_f32_limb_4096x32x12_counters.md"}
 NPROF, NRAY_PER, GASES = 4096, 32, ["CO2", "H2O", "O3"]
 
 
@@ -108,6 +115,45 @@ def cpu_baseline(nsample):
     return dict(value=nrad / dt, unit="radiances/s", cores=threads, kind="port",
                 sample="%d profiles x %d tangent heights x %d channels (%d radiances), FP64 oracle, %.1f s"
                 % (nsample, len(zt), ctl.nd, nrad, dt))
+
+
+def lookup_roofline(nprof, nray, nd, mean_np, kms):
+    """Roofline of the dominant forward kernel, k_lookup_f32 (one launch per
+    pass; its time from HIP events on the library's stream).
+
+    `bound`/`achieved`/`frac` are HBM: the bytes the kernel must move at least
+    (every LOS field it reads once, the tables once, the path-emissivity
+    write) per second of its time, against 8 TB/s; `traffic` is the HBM bytes
+    per launch of the committed counter profile.  The kernel's limiter is
+    vector-instruction issue, reported beside it (`issue`): SQ_INSTS_VALU of
+    the same profile per second of the measured time against VALU_PEAK.  The
+    per-lookup gather model (176 B per (segment, channel, emitter): 4 corners
+    x 2 float brackets of 16 B plus the p and two T brackets, 48 B) is mostly
+    served by L2, so its rate is not an HBM figure."""
+    t = kms[1] * 1e-3
+    n_chain = nray * mean_np * nd * len(GASES)         # (ray, point, channel, emitter) steps
+    unique = nray * mean_np * (2 + len(GASES)) * 8 + n_chain * 4 + 36 * 40 * 20 * 200 * 16
+    gather = n_chain * 176
+    cnt = LOOKUP_F32_COUNTERS if nprof == LOOKUP_F32_COUNTERS["nprof"] else None
+    r = {"kernel": "k_lookup_f32<false, false>", "kernel_ms": kms[1],
+         "kernel_ms_all": {"raytrace": kms[0], "lookup": kms[1], "accumulate": kms[2]},
+         "bound": "hbm", "achieved": unique / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+         "frac": unique / t / HBM_PEAK, "traffic": cnt["hbm_bytes"] if cnt else None}
+    if cnt:
+        ach = cnt["valu"] / t
+        r["issue"] = {"limiter": "VALU issue", "achieved": ach / 1e9, "peak": VALU_PEAK / 1e9,
+                      "unit": "G wave-instructions/s", "frac": ach / VALU_PEAK,
+                      "valu_per_launch": cnt["valu"], "vmem_rd_per_launch": cnt["vmem_rd"]}
+    r["hbm"] = {"unique_bytes": unique, "unique_GBps": unique / t / 1e9,
+                "unique_frac": unique / t / HBM_PEAK,
+                "measured_bytes": cnt["hbm_bytes"] if cnt else None,
+                "measured_GBps": cnt["hbm_bytes"] / t / 1e9 if cnt else None,
+                "peak_GBps": HBM_PEAK / 1e9}
+    r["gather_model"] = {"bytes_per_lookup": 176, "bytes_per_launch": gather, "GBps": gather / t / 1e9,
+                         "note": "algorithmic gather bytes; L2 serves most of them, so this rate is "
+                                 "not HBM traffic"}
+    r["counters_profile"] = cnt["profile"] if cnt else None
+    return r
 
 
 NJAC_PROF, NJAC_ANG, NJAC_CH = 16384, 8, 96
@@ -194,18 +240,53 @@ def jacobian_bench(args, torch, gpu, abi):
     log("adjoint FP32: %.1f ms/step (%.0f Jacobians/s); kernels: ray tracing %.1f, lookups %.1f, "
         "reverse sweep + K store %.1f ms; mean LOS points %.1f" % (dt * 1e3, jac_rate, kms[0], kms[1], kms[2], mean_np))
 
-    # the FD path: 325 FP64 forward passes per Jacobian
-    def fwd(precision):
-        ctx.formod_device(nprof, atm_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
-                          rad_d.data_ptr(), tau_d.data_ptr(), precision=precision)
-    fwd(abi.JUR_FP64)
+    # the FD path of kernel() (src/jurassic.c:4004-4099), measured: for the
+    # first nfd atmospheres, the n + 1 FP64 forward models of each Jacobian
+    # (one-sided steps of :4046-4068: 1 K for T, max(|0.01 x|, 1e-15) for
+    # vmr) in one device pass, the perturbed atmospheres built and the
+    # differences formed on the device, all inside the timed region
+    nfd = min(args.jac_fd_nprof, nprof)
+    asz = C.sizeof(abi.atm_t)
+    col_off = np.empty(n, dtype=np.int64)
+    is_t = np.zeros(n, dtype=bool)
+    off_t, off_q = abi.atm_t.t.offset // 8, abi.atm_t.q.offset // 8
+    for j in range(n):
+        blk, ip = divmod(j, 81)
+        col_off[j] = off_t + ip if blk == 0 else off_q + (blk - 1) * abi.NP + ip
+        is_t[j] = blk == 0
+    col_off_d = torch.from_numpy(col_off).to(dev)
+    is_t_d = torch.from_numpy(is_t).to(dev)
+    natm_fd = nfd * (n + 1)
+    nray_fd = NJAC_ANG * natm_fd
+    # rays: angle major, then atmosphere (profile major, perturbation minor)
+    prof_fd = torch.arange(natm_fd, dtype=torch.int32, device=dev).repeat(NJAC_ANG)
+    ang_rows = torch.from_numpy(np.ascontiguousarray(geom[np.arange(NJAC_ANG) * nprof])).to(dev)
+    geom_fd = ang_rows.repeat_interleave(natm_fd, dim=0).contiguous()
+    rad_fd = torch.empty(nray_fd * nd, dtype=torch.float64, device=dev)
+    tau_fd = torch.empty_like(rad_fd)
+    jcols = torch.arange(n, device=dev)
+
+    def fd_jacobians():
+        base = atm_d.view(nprof, asz)[:nfd]
+        pert = base[:, None, :].expand(nfd, n + 1, asz).contiguous()
+        pf = pert.view(torch.float64).view(nfd, n + 1, asz // 8)
+        x0 = pf[:, 0, col_off_d]
+        h = torch.where(is_t_d[None, :], torch.ones_like(x0), torch.clamp(torch.abs(0.01 * x0), min=1e-15))
+        pf[:, 1 + jcols, col_off_d] += h
+        ctx.formod_device(natm_fd, pert.data_ptr(), nray_fd, prof_fd.data_ptr(), geom_fd.data_ptr(),
+                          rad_fd.data_ptr(), tau_fd.data_ptr(), precision=abi.JUR_FP64)
+        radv = rad_fd.view(NJAC_ANG, nfd, n + 1, nd)
+        # K[profile][angle][channel][column]
+        return ((radv[:, :, 1:, :] - radv[:, :, :1, :]) / h[None, :, :, None]).permute(1, 0, 3, 2).contiguous()
+    fd_jacobians()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fwd(abi.JUR_FP64)
+    k_fd = fd_jacobians()
     torch.cuda.synchronize()
-    dt64 = time.perf_counter() - t0
-    fd_rate = nprof / (dt64 * (n + 1))
-    log("FP64 forward pass over the batch: %.1f ms -> FD Jacobians/s %.1f" % (dt64 * 1e3, fd_rate))
+    dt_fd = time.perf_counter() - t0
+    fd_rate = nfd / dt_fd
+    log("FD FP64 (kernel()): %d Jacobians = %d forward models x %d rays in %.1f ms -> %.1f Jacobians/s"
+        % (nfd, natm_fd, NJAC_ANG, dt_fd * 1e3, fd_rate))
 
     # FP32 adjoint against FP64 adjoint on the first atmospheres of every angle
     nsub = min(args.jac_check, nprof)
@@ -233,14 +314,30 @@ def jacobian_bench(args, torch, gpu, abi):
         s_ = slice(b * 81, (b + 1) * 81)
         blocks[name] = float(np.linalg.norm(k32[:, s_] - k64[:, s_]) / np.linalg.norm(k64[:, s_]))
     log("FP32 vs FP64 adjoint on %d atmospheres: BT %.2e K, K rel L2 %s" % (nsub, bt_err, blocks))
+    # the adjoint (path held fixed) against kernel()'s one-sided differences
+    # (which also move the refracted path) on the FD atmospheres
+    ncmp = min(nfd, nsub)
+    kf = k_fd[:ncmp].cpu().numpy()  # [atm][angle][channel][column]
+    fd_vs = {}
+    for p, k_adj in ((abi.JUR_FP32, k32), (abi.JUR_FP64, k64)):
+        ka = k_adj.reshape(NJAC_ANG, nsub, nd, n)[:, :ncmp].transpose(1, 0, 2, 3)
+        fd_vs["fp32" if p == abi.JUR_FP32 else "fp64"] = {
+            name: float(np.linalg.norm(ka[..., b * 81:(b + 1) * 81] - kf[..., b * 81:(b + 1) * 81])
+                        / np.linalg.norm(kf[..., b * 81:(b + 1) * 81]))
+            for b, name in enumerate(["T"] + GASES)}
+    log("adjoint vs kernel() FD on %d atmospheres (refraction on): %s" % (ncmp, fd_vs))
+    del k_fd, rad_fd, tau_fd
     ctx.close()
     del k_d
 
-    # algorithmic bytes per Jacobian: the lookups (176 B per segment, channel
-    # and emitter) with their stored partials (16 B written, then read with
-    # the 4 B emissivity by the reverse chains), the Planck bracket (16 B per
-    # segment and channel, forward and reverse sweep) and the FP64 K write
-    bytes_per_jac = m * mean_np * (len(GASES) * (176 + 16 + 16 + 4) + 2 * 16) + m * n * 8
+    # HBM bytes a Jacobian pass must move at least, per Jacobian: per (ray,
+    # LOS point, channel) and emitter the lookup's stored partials (16 B) and
+    # path emissivity (4 B), each written once and read once by the reverse
+    # chains, and the accumulation's per-point value H_i (8 B, written and
+    # read); per (ray, point) the LOS fields read by the lookups and the
+    # accumulation (10 FP64 fields, twice); the FP64 K written once
+    bytes_per_jac = (m * mean_np * (len(GASES) * 2 * (16 + 4) + 2 * 8) + NJAC_ANG * mean_np * 2 * 10 * 8
+                     + m * n * 8)
     out = {
         "value": jac_rate, "unit": "Jacobians/s", "ms_per_step": dt * 1e3, "steps": steps,
         "workload": "nadir sounder: %d atmospheres x %d view angles x %d channels, n = %d, m = %d "
@@ -250,13 +347,23 @@ def jacobian_bench(args, torch, gpu, abi):
         "mean_los_points": mean_np,
         "kernel_ms": {"raytrace": kms[0], "lookup": kms[1], "reverse_sweep_and_store": kms[2]},
         "fd_fp64_gpu": {"value": fd_rate, "unit": "Jacobians/s",
-                        "method": "n + 1 = %d FP64 forward passes per Jacobian (kernel())" % (n + 1),
-                        "forward_ms_batch": dt64 * 1e3},
+                        "method": "measured: kernel()'s n + 1 = %d one-sided FP64 forward models per "
+                                  "Jacobian for %d atmospheres x %d angles x %d channels (%d forward "
+                                  "models, %d rays) in one device pass, perturbations and differences "
+                                  "on the device" % (n + 1, nfd, NJAC_ANG, nd, natm_fd, nray_fd),
+                        "ms": dt_fd * 1e3},
         "adjoint_over_fd_fp64_gpu": jac_rate / fd_rate,
+        "adjoint_vs_kernel_fd_rel_l2": {"atmospheres": ncmp, "note": "difference to kernel()'s one-sided "
+                                        "finite differences (its own step sizes), mostly their "
+                                        "first-order truncation; the adjoint includes the "
+                                        "refracted path's derivative",
+                                        **fd_vs},
         "fp32_vs_fp64": {"atmospheres": nsub, "bt_max_abs_K": bt_err, "k_rel_l2": blocks},
         "roofline": {"bound": "hbm", "achieved": bytes_per_jac * jac_rate / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": bytes_per_jac * jac_rate / HBM_PEAK,
-                     "bytes_per_jacobian": bytes_per_jac},
+                     "bytes_per_jacobian": bytes_per_jac,
+                     "model": "unique bytes: partials and path emissivities written and read once, "
+                              "per-point accumulation values, LOS fields, K"},
     }
     if not args.no_cpu:
         ns_cpu = args.jac_cpu_sample
@@ -661,6 +768,8 @@ def main():
     ap.add_argument("--jac-check", type=int, default=64,
                     help="atmospheres per angle for the FP32-vs-FP64 adjoint check")
     ap.add_argument("--jac-cpu-sample", type=int, default=2)
+    ap.add_argument("--jac-fd-nprof", type=int, default=64,
+                    help="atmospheres of the measured FD Jacobian run")
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--no-tblgen", action="store_true")
     ap.add_argument("--tblgen-only", action="store_true")
@@ -729,25 +838,34 @@ def main():
         dt = time.perf_counter() - t0
         return dt, [k / steps for k in kms]
 
+    nrad_all = nray * nd
     dt32, k32 = timed(abi.JUR_FP32, args.steps, args.warmup)
     mean_np = ctx.last_mean_np()
     log("FP32: %.1f ms/step; kernels: ray tracing %.1f ms, lookups %.1f ms, accumulation %.1f ms; "
         "mean LOS points %.1f" % (dt32 * 1e3 / args.steps, k32[0], k32[1], k32[2], mean_np))
     res = {}
     if not args.no_fp64:
+        rad32 = rad_d.clone()
         dt64, k64 = timed(abi.JUR_FP64, max(1, args.steps // 2), 1)
         res["fp64_value"] = nray * nd * max(1, args.steps // 2) / dt64
         res["fp64_kernel_ms"] = {"raytrace": k64[0], "lookup": k64[1], "accumulate": k64[2]}
         log("FP64: %.1f ms/step; kernels: ray tracing %.1f ms, lookups %.1f ms, accumulation %.1f ms"
             % (dt64 * 1e3 / max(1, args.steps // 2), k64[0], k64[1], k64[2]))
+        # brightness-temperature error of the FP32 path against the FP64 path
+        # on every radiance of the workload
+        nu_d = torch.tensor([ctl.nu[i] for i in range(nd)], dtype=torch.float64, device=dev)
+        c1, c2 = 1.19104259e-8, 1.43877506
+
+        def bt(r):
+            return c2 * nu_d / torch.log1p(c1 * nu_d ** 3 / r.view(nray, nd))
+        err = torch.abs(bt(rad32) - bt(rad_d))
+        res["fp32_vs_fp64_bt_K"] = {"max": float(err.max().item()), "mean": float(err.mean().item()),
+                                    "radiances": nrad_all}
+        log("FP32 vs FP64 brightness temperature over %d radiances: max %.2e K, mean %.2e K"
+            % (nrad_all, res["fp32_vs_fp64_bt_K"]["max"], res["fp32_vs_fp64_bt_K"]["mean"]))
+        del rad32, err
     nrad = nray * nd
     value = nrad * args.steps / dt32
-    # algorithmic bytes: 176 B per (segment, channel, emitter) table lookup,
-    # all of them in the lookup kernel (the dominant kernel); the 16 B Planck
-    # bracket per (segment, channel) belongs to the accumulation kernel
-    bytes_per_rad = mean_np * (len(GASES) * 176 + 16)
-    lookup_bytes = mean_np * len(GASES) * 176 * nrad
-    achieved = lookup_bytes / (k32[1] * 1e-3)
     out = {
         "metric": METRIC,
         "value": value,
@@ -768,21 +886,7 @@ def main():
             "profiles": nprof, "rays": nray, "channels": nd, "emitters": len(GASES),
             "mean_los_points": mean_np, "precision": "FP32 lookups, FP64 ray geometry",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": LOOKUP_F32_TRAFFIC[1] if nprof == LOOKUP_F32_TRAFFIC[0] else None,
-            "traffic_hbm_rate_GBps": (LOOKUP_F32_TRAFFIC[1] / (k32[1] * 1e-3) / 1e9
-                                      if nprof == LOOKUP_F32_TRAFFIC[0] else None),
-            "kernel": "k_lookup_f32<false>",
-            "kernel_ms": k32[1],
-            "bytes_per_launch_step": lookup_bytes,
-            "bytes_per_radiance": bytes_per_rad,
-            "kernel_ms_all": {"raytrace": k32[0], "lookup": k32[1], "accumulate": k32[2]},
-        },
+        "roofline": lookup_roofline(nprof, nray, nd, mean_np, k32),
     }
     out.update(res)
     if not args.no_cpu:

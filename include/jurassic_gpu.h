@@ -239,6 +239,16 @@ int jur_gpu_formod_batch(const ctl_t *ctl, atm_t *atm, obs_t *obs, int n,
 int jur_gpu_kernel(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k,
                    size_t m, size_t n, int method, int precision);
 
+/* Replaces write_matrix (src/jurassic.c:5492-5666): writes matrix (row-major,
+   rows and columns in measurement space "y" = obs2y order or state space
+   "x" = atm2x order, as rowspace/colspace say) to <dirname>/<filename> in
+   the reference's text format (12 metadata columns, then the element;
+   sort "r" row by row, else column by column) when ctl->write_matrix is
+   set.  Returns JUR_OK, JUR_ERR_ARG or JUR_ERR_IO ("Cannot create file!"). */
+int jur_write_matrix(const char *dirname, const char *filename, const ctl_t *ctl, const double *matrix,
+                     const atm_t *atm, const obs_t *obs, const char *rowspace, const char *colspace,
+                     const char *sort);
+
 /* The state map of atm2x (src/jurassic.c:29-83) for ctl and atm: fills
    xmap[JUR_XMAP_LEN] and returns the state size n. */
 int jur_state_map(const ctl_t *ctl, const atm_t *atm, int *xmap);

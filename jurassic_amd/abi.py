@@ -22,6 +22,7 @@ JUR_FP32 = 1
 JUR_JAC_FD = 0
 JUR_JAC_ADJOINT = 1
 JUR_ERR_ARG = 1
+JUR_ERR_IO = 2
 JUR_ERR_MEM = 7
 
 # State map layout of the device Jacobian (JUR_XMAP_* in the C header).

@@ -510,12 +510,20 @@ __device__ __forceinline__ int level_of(const atm_t *atm, double z, int lev, dou
 // A path leaves a pair into a neighbouring one: that pair costs one round
 // trip for the new level's altitude and pressure, where a search of the
 // level grid (locate_irr) would take one per bisection step.
-template <int NQ, unsigned ATOM = 0, bool CCOL = true>
+// With PATH, add() also returns the point adjoints projected on the altitude
+// of the point, sum_q v[q] dQ_q/dz (intpol_atm's slopes within the pair),
+// which the adjoint of the refracted path needs.
+__device__ __forceinline__ double level_val(const atm_t *atm, int qi, int l) {
+  return qi == 0 ? atm->p[l] : (qi == 1 ? atm->t[l] : (qi < 2 + NG ? atm->q[qi - 2][l] : atm->k[qi - 2 - NG][l]));
+}
+
+template <int NQ, unsigned ATOM = 0, bool CCOL = true, bool PATH = false>
 struct LevelWindow {
   int wa = -1;
   double z0 = 0, z1 = 0, rz = 0, P0 = 0, P1 = 0, rp0 = 0, rp1 = 0;
   bool plog = false;
   double acc[NQ][2];
+  double sl[NQ];   // PATH: slope of each quantity in the pair (p: of log p, or of p)
   int col[NQ][2];  // K columns of the pair's levels (loaded when the pair is entered)
   __device__ __forceinline__ void flush(const int *xm, const int *qi, double *kt) {
     if (wa < 0) return;
@@ -533,8 +541,8 @@ struct LevelWindow {
   // Point adjoints v[q] at altitude z; quantity q == qp is pressure (LOGY
   // weights p/p_level, src/jurassic.c:3685-3706), the others use the LIN
   // weights.
-  __device__ __forceinline__ void add(const atm_t *atm, double z, int &lev, const double *v, int qp, double p,
-                                      const int *xm, const int *qi, double *kt) {
+  __device__ __forceinline__ double add(const atm_t *atm, double z, int &lev, const double *v, int qp, double p,
+                                        const int *xm, const int *qi, double *kt) {
     if (!(wa >= 0 && z0 <= z && z < z1)) {
       // the pair locate_irr returns: a neighbour of the current pair (with
       // its clamps at the first and last pair), else a search
@@ -581,6 +589,14 @@ struct LevelWindow {
       }
       rz = 1 / (z1 - z0);
       plog = P1 / P0 > 0;
+      if (PATH) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          if (qi[q] < 0) sl[q] = 0;
+          else if (q == qp) sl[q] = plog ? log(P1 / P0) * rz : (P1 - P0) * rz;
+          else sl[q] = (level_val(atm, qi[q], nl + 1) - level_val(atm, qi[q], nl)) * rz;
+        }
+      }
       if (nl != wa) {
         flush(xm, qi, kt);
         wa = nl;
@@ -596,6 +612,7 @@ struct LevelWindow {
       lev = nl;
     }
     const double w = (z - z0) * rz;
+    double dz = 0;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       if (q == qp && plog) {
@@ -605,7 +622,9 @@ struct LevelWindow {
         acc[q][0] += (1 - w) * v[q];
         acc[q][1] += w * v[q];
       }
+      if (PATH) dz += v[q] * ((q == qp && plog) ? p * sl[q] : sl[q]);
     }
+    return dz;
   }
 };
 
@@ -613,7 +632,7 @@ struct LevelWindow {
 // rad, tau, H_i (scratch), the number of points before the first opaque
 // segment, and the K entries of continua, source function, window
 // extinction, clouds and surface.
-template <typename SF>
+template <typename SF, bool PATH>
 __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
                                                   const RayInfo *info, int blk0, int nblk, JacView J,
                                                   double *rad_out, double *tau_out) {
@@ -675,8 +694,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
   // function and its slope, and s = tg exp(-beta ds); rt = 1/t, rp = 1/p.
   auto segment = [&](double p, double t, double ds, double q_h2o, double u_co2, double u_h2o, double kwin,
                      double clx, double tg, BetaGrad &bg, double &src, double &dsrc, double &rt,
-                     double &rp) {
-    double beta;
+                     double &rp, double &beta) {
     if constexpr (F32) {
       rt = 1 / t;
       rp = 1 / p;
@@ -753,10 +771,10 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       const double tg = q / qp;
       qp = q;
       BetaGrad bg;
-      double src, dsrc, rt, rp;
+      double src, dsrc, rt, rp, beta;
       const double s = segment(cu.p, cu.t, cu.ds, fq_h2o >= 0 ? cu.qh : 0.0, fu_co2 >= 0 ? cu.uc : 0.0,
                                fu_h2o >= 0 ? cu.uh : 0.0, has_k ? cu.kw : 0.0, cloud ? cu.cl * kcl : 0.0, tg,
-                               bg, src, dsrc, rt, rp);
+                               bg, src, dsrc, rt, rp, beta);
       SCR(0, ip) = tau;
       if (J.nscr > 1) SCR(1, ip) = dr;
       rad += src * (1 - s) * tau;
@@ -801,7 +819,11 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
   {
     const int qi[5] = {1, 0, has_k ? 2 + NG + C.window : -1, M.ig_co2 >= 0 ? 2 + M.ig_co2 : -1,
                        M.ig_h2o >= 0 ? 2 + M.ig_h2o : -1};
-    LevelWindow<5, 0x1Fu, F32> W;
+    LevelWindow<5, 0x1Fu, F32, PATH> W;
+    // the path adjoint's point values d rad / d z_i, d rad / d ds_i: zero
+    // beyond the first opaque segment, the emitter chains add theirs
+    if (PATH)
+      for (int ip = nef; ip < R.np; ip++) SCR(J.kz, ip) = SCR(J.kz + 1, ip) = 0.0;
     int lev = -1;
     double qi_cur = nef > 0 ? qprod(nef - 1) : 1;
     // point ip's inputs and scratch, and the emitters' values at ip - 1,
@@ -827,9 +849,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       const double u_h2o = fu_h2o >= 0 ? cu.uh : 0.0;
       const double aux = cloud ? cu.cl : 0.0;
       BetaGrad bg;
-      double src, dsrc, rt, rp;
+      double src, dsrc, rt, rp, beta;
       const double s = segment(p, t, ds, q_h2o, u_co2, u_h2o, has_k ? cu.kw : 0.0, aux * kcl, tg, bg, src, dsrc,
-                               rt, rp);
+                               rt, rp, beta);
       const double G = tau_i * (tb - src) + db * (dr_i - src);
       const double src_bar = (1 - s) * (tau_i + db);
       tb = src * (1 - s) + tb * s;
@@ -848,7 +870,17 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       v[2] = beta_bar;                                                                  // k
       v[3] = uc_bar * ucoef;                                                            // q_co2
       v[4] = uh_bar * ucoef + beta_bar * bg.dq_h2o;                                     // q_h2o
-      W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+      const double gz = W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+      if (PATH) {
+        // altitude: the level quantities' slopes and the cloud profile's;
+        // segment length: exp(-beta ds) (the continua's beta does not
+        // depend on ds, their u ~ ds cancels their 1/ds), the column
+        // densities' part comes from the emitter chains
+        double gzc = gz;
+        if (cloud && atm->cldz > 0) gzc -= beta_bar * kcl * aux * (z - atm->clz) / (atm->cldz * atm->cldz);
+        SCR(J.kz, ip) = gzc;
+        SCR(J.kz + 1, ip) = -H * beta;
+      }
 
       if (cloud) {
         clk_bar += beta_bar * aux;
@@ -875,6 +907,8 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       int c;
       if ((c = xm[JUR_XMAP_LEV(1, lv)]) >= 0) KT(c) += (1 - w) * sft_bar;
       if ((c = xm[JUR_XMAP_LEV(1, lv + 1)]) >= 0) KT(c) += w * sft_bar;
+      if (PATH && R.np - 1 < nef)
+        SCR(J.kz, R.np - 1) += sft_bar * (atm->t[lv + 1] - atm->t[lv]) / (atm->z[lv + 1] - atm->z[lv]);
     }
     if (M.nsf > 0) {
       int c;
@@ -897,7 +931,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
 // Reverse sweep of one emitter chain per wavefront: L_g,i from H_i and the
 // recomputed table partials; T and p entries are shared with the other
 // emitters of the row (atomic adds), q_g entries belong to this chain.
-template <bool CGA, typename SF, bool FAST>
+template <bool CGA, typename SF, bool FAST, bool PATH>
 __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, LosView L, EpsView E,
                                                     const RayInfo *info, int blk0, int nblk, JacView J) {
   const int nd = M.nd, ng = M.ng;
@@ -922,10 +956,18 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
   const int npb = E.npb[bl];
   const SF *epsb = (const SF *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
 #define EPS_AT(g, ip) epsb[((size_t)(g) * npb + (ip)) * WAVE]
-  const double *sc = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE + lane;
+  double *sc = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE + lane;
 #define SCR(k, ip) sc[((size_t)(ip) * J.nscr + (k)) * WAVE]
   double *kt = J.kt + ((size_t)bl * nd + id) * J.ncol * WAVE + lane;
   const int nef = J.nef[((size_t)bl * nd + id) * WAVE + lane];
+  // the path adjoint's point values: this emitter's part of d rad / d z_i
+  // and d rad / d ds_i (the other emitters of the row add theirs)
+  auto path_add = [&](int ip, double gz, double gd) {
+    if (PATH) {
+      unsafeAtomicAdd(&SCR(J.kz, ip), gz);
+      unsafeAtomicAdd(&SCR(J.kz + 1, ip), gd);
+    }
+  };
   const int mono = M.tab.mono[tix], pdir = M.tab.pdir[tix];
   const int fu = L.f_u + g;
   double cu = 0, cp = 0, ct = 0;  // Curtis-Godson sums up to the current point
@@ -937,7 +979,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
       ct += u * LOSR(F_T, ip);
     }
   const int qi[3] = {1, 0, 2 + g};
-  LevelWindow<3, 7u, FAST> W;
+  LevelWindow<3, 7u, FAST, PATH> W;
   int lev = -1, ipr = -1;
   int gi[4];
 #pragma unroll
@@ -994,7 +1036,8 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
         v[0] = t_bar - u_bar * useg * rt;
         v[1] = p_bar + u_bar * useg * rp;
         v[2] = u_bar * ucoef;
-        W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+        const double gz = W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+        path_add(ip, gz, u_bar * useg / ds);
         Hn = H;
         trc = ip > 0 ? trp : 1.0;
         p = pn;
@@ -1059,7 +1102,8 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
     v[0] = t_bar - u_bar * useg / t;
     v[1] = p_bar + u_bar * useg / p;
     v[2] = u_bar * ucoef;
-    W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+    const double gz = W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+    path_add(ip, gz, u_bar * useg / ds);
     Hn = H;
     trc = trp;
   }
@@ -1067,6 +1111,316 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
   }
 #undef EPS_AT
 #undef SCR
+}
+
+// ------------------------------------------------------------------
+// Adjoint of the refracted path (raytrace, src/jurassic.c:4201-4438).
+//
+// Below 60 km the march bends the ray by the refractivity N = 7.753e-5 p/T
+// (jurassic.h:197-198) at the point (n = 1 + N) and at four probes (the
+// step's midpoint and three points 0.02 km off it, for the gradient), so
+// every LOS point after the first depends on p and T of the levels the
+// path passed.  The reverse sweep runs each step of the march backwards,
+// step k taking the state (x_k, e_k) to
+//   x_k+1 = x_k + ds_k/2 (e_k + e'),  e' = v/|v|,  v = n e_k + ds_k grad N,
+//   ds_k = min(RAYDS, RAYDZ / |e_k . x_k/|x_k||),
+// with the adjoint (X, E) of (x_k+1, e_k+1) and the point values
+// d rad / d z_k and d rad / d ds_k that the accumulation and the emitter
+// chains stored (ds_k entering through the trapezoid rule, the last point's
+// through the clip of the step that left the altitude range, k_ray_finish).
+// Each probe's and point's N goes to its two levels with intpol_atm's
+// weights (LOGY in p, LIN in T).  A lane is a ray and carries the adjoints
+// of CH channels, whose steps share the recomputed march; level sums are
+// kept for a window of four levels per channel and added to the K tile
+// when the window moves.
+template <int CH>
+struct PathWin {
+  int wb = -1;
+  double at[CH][4], ap[CH][4];
+  __device__ __forceinline__ void flush(const int *xm, double *kt0, size_t ktstride, int nc) {
+    if (wb < 0) return;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int ct = xm[JUR_XMAP_LEV(1, wb + j)], cp = xm[JUR_XMAP_LEV(0, wb + j)];
+#pragma unroll
+      for (int c = 0; c < CH; c++) {
+        if (c >= nc) continue;
+        double *kt = kt0 + (size_t)c * ktstride;
+        if (ct >= 0) kt[(size_t)ct * WAVE] += at[c][j];
+        if (cp >= 0) kt[(size_t)cp * WAVE] += ap[c][j];
+      }
+    }
+  }
+  // levels lo and lo + 1 inside the window (moving it when needed)
+  __device__ __forceinline__ void cover(int lo, int nlev, const int *xm, double *kt0, size_t ktstride, int nc) {
+    if (wb >= 0 && lo >= wb && lo + 1 <= wb + 3) return;
+    flush(xm, kt0, ktstride, nc);
+    int b = lo - 1;
+    if (b > nlev - 4) b = nlev - 4;
+    if (b < 0) b = 0;
+    wb = b;
+#pragma unroll
+    for (int c = 0; c < CH; c++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) at[c][j] = ap[c][j] = 0.0;
+  }
+  // g[c] d N / d (levels) of N = CR p/t at a point of pair lo, weight w
+  // (fp = p/P_lo (1-w), fq = p/P_hi w for LOGY, else the LIN weights)
+  __device__ __forceinline__ void add(int lo, double w, double fp0, double fp1, const double *gt,
+                                      const double *gp) {
+    const int d = lo - wb;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double wt = j == d ? 1 - w : (j == d + 1 ? w : 0.0);
+      const double wp = j == d ? fp0 : (j == d + 1 ? fp1 : 0.0);
+#pragma unroll
+      for (int c = 0; c < CH; c++) {
+        at[c][j] += wt * gt[c];
+        ap[c][j] += wp * gp[c];
+      }
+    }
+  }
+};
+
+// p, t and their altitude slopes at z in level pair lo (weight w), and the
+// LOGY weights of p on the two levels
+struct PtAt {
+  int lo;
+  double w, p, t, dp, dt, fp0, fp1;
+};
+__device__ __forceinline__ PtAt pt_at(const atm_t *atm, double z, int hint) {
+  PtAt r;
+  r.lo = level_of(atm, z, hint, r.w);
+  const double z0 = atm->z[r.lo], z1 = atm->z[r.lo + 1];
+  const double P0 = atm->p[r.lo], P1 = atm->p[r.lo + 1];
+  const double T0 = atm->t[r.lo], T1 = atm->t[r.lo + 1];
+  const double rz = 1 / (z1 - z0);
+  if (P1 / P0 > 0) {
+    const double lq = log(P1 / P0) * rz;
+    r.p = P0 * exp(lq * (z - z0));
+    r.dp = r.p * lq;
+    r.fp0 = r.p / P0 * (1 - r.w);
+    r.fp1 = r.p / P1 * r.w;
+  } else {
+    r.p = P0 + (P1 - P0) * rz * (z - z0);
+    r.dp = (P1 - P0) * rz;
+    r.fp0 = 1 - r.w;
+    r.fp1 = r.w;
+  }
+  r.t = T0 + (T1 - T0) * rz * (z - z0);
+  r.dt = (T1 - T0) * rz;
+  return r;
+}
+
+template <int CH>
+__global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, LosView L, EpsView E,
+                                                   const RayInfo *info, const RayTail *tail, int blk0, int nblk,
+                                                   JacView J) {
+  const int nd = M.nd;
+  const int ngrp = (nd + CH - 1) / CH;
+  int bl, grp;
+  if (!xcd_block_map(blockIdx.x, ngrp, nblk, bl, grp)) return;
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + bl) * WAVE + lane;
+  if (ir >= B.nray) return;
+  const RayInfo R = info[ir];
+  if (R.status != JUR_OK || R.np < 2) return;
+  const int ipf = B.ray_prof[ir];
+  const int im = J.prof_map[ipf];
+  if (im < 0 || im >= J.nmap) return;
+  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
+  const atm_t *atm = &B.atm[ipf];
+  const int nlev = atm->np;
+  const int id0 = grp * CH;
+  const int nc = nd - id0 < CH ? nd - id0 : CH;
+  const int npb = E.npb[bl];
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const double *sc0 = J.scr + J.soff[bl] + (size_t)id0 * npb * J.nscr * WAVE + lane;
+  const size_t scstride = (size_t)npb * J.nscr * WAVE;
+#define GZ(c, ip) sc0[(c) * scstride + ((size_t)(ip) * J.nscr + J.kz) * WAVE]
+#define GD(c, ip) sc0[(c) * scstride + ((size_t)(ip) * J.nscr + J.kz + 1) * WAVE]
+  double *kt0 = J.kt + ((size_t)bl * nd + id0) * J.ncol * WAVE + lane;
+  const size_t ktstride = (size_t)J.ncol * WAVE;
+  const double CR = 7.753e-05, HP = 0.02, ZREF = 60;
+  const RayTail T = tail[ir];
+  const int N = R.np;
+  auto xe = [&](int ip, int i) { return LOSR(L.f_xe + i, ip); };
+
+  // d ds / d (x, e) of the step length at (x, e), applied to a scalar g
+  auto ds_grad = [&](const double *x, const double *e, double gds, double *gx, double *ge) {
+    if (!(M.raydz > 0)) return;
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double xh[3] = {x[0] / r, x[1] / r, x[2] / r};
+    const double cc = e[0] * xh[0] + e[1] * xh[1] + e[2] * xh[2];
+    if (cc == 0 || !(M.raydz / fabs(cc) < M.rayds)) return;
+    const double d = -M.raydz * (cc > 0 ? 1.0 : -1.0) / (cc * cc) * gds;
+    for (int i = 0; i < 3; i++) {
+      ge[i] += d * xh[i];
+      gx[i] += d * (e[i] - cc * xh[i]) / r;
+    }
+  };
+  auto step_ds = [&](const double *x, const double *e) {
+    double ds = M.rayds;
+    if (M.raydz > 0) {
+      const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+      const double ca = fabs((e[0] * x[0] + e[1] * x[1] + e[2] * x[2]) / r);
+      if (ca != 0) ds = (M.rayds < M.raydz / ca) ? M.rayds : M.raydz / ca;
+    }
+    return ds;
+  };
+
+  double X[CH][3], Ev[CH][3], Xp[CH][3], gzp[CH], gdn[CH];
+  // ---- the clipped last point: x_l = x_p + frac (x_cut - x_p), its raw
+  // step replacing point N-2's: ds_cut frac ----
+  {
+    const double zp = T.zprev, zc = T.zcut;
+    const double zb = zc < T.zmin ? T.zmin : T.zmax;
+    const double frac = (zb - zp) / (zc - zp);
+    double xl[3];
+    for (int i = 0; i < 3; i++) xl[i] = xe(N - 1, i);
+    const double rl = sqrt(xl[0] * xl[0] + xl[1] * xl[1] + xl[2] * xl[2]);
+    const double rc = sqrt(T.xcut[0] * T.xcut[0] + T.xcut[1] * T.xcut[1] + T.xcut[2] * T.xcut[2]);
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const double gz = c < nc ? GZ(c, N - 1) : 0.0;
+      const double gdl = c < nc ? GD(c, N - 1) : 0.0;
+      const double gdm = c < nc ? GD(c, N - 2) : 0.0;
+      gdn[c] = gdm;
+      const double gs = 0.5 * (gdm + gdl);  // d rad / d (clipped raw ds of point N-2)
+      double gfrac = T.dscut * gs, xk[3], ek[3] = {0, 0, 0};
+      for (int i = 0; i < 3; i++) {
+        const double xli = gz * xl[i] / rl;
+        Xp[c][i] = (1 - frac) * xli;
+        xk[i] = frac * xli;
+        gfrac += xli * (T.xcut[i] - T.xprev[i]);
+      }
+      gzp[c] = gfrac * (frac - 1) / (zc - zp);
+      const double gzc = -gfrac * frac / (zc - zp);
+      for (int i = 0; i < 3; i++) xk[i] += gzc * T.xcut[i] / rc;
+      ds_grad(T.xcut, T.ecut, frac * gs, xk, ek);
+      for (int i = 0; i < 3; i++) {
+        X[c][i] = xk[i];
+        Ev[c][i] = ek[i];
+      }
+    }
+  }
+
+  PathWin<CH> W;
+  int hint = -1;
+  for (int k = N - 2; k >= 0; k--) {
+    double x[3], e[3];
+    for (int i = 0; i < 3; i++) {
+      x[i] = xe(k, i);
+      e[i] = xe(k, 3 + i);
+    }
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double z = r - JUR_RE;
+    const double ds = step_ds(x, e);
+    const bool rf = M.refrac && z <= ZREF;
+    // the forward step, recomputed
+    double n = 1, ng[3] = {0, 0, 0}, xq[4][3];
+    PtAt pk{}, pq[4];
+    if (rf) {
+      pk = pt_at(atm, z, hint);
+      hint = pk.lo;
+      n = 1 + CR * pk.p / pk.t;
+      for (int i = 0; i < 3; i++) xq[0][i] = x[i] + 0.5 * ds * e[i];
+      for (int q = 1; q < 4; q++)
+        for (int i = 0; i < 3; i++) xq[q][i] = xq[0][i] + (i == q - 1 ? HP : 0.0);
+      for (int q = 0; q < 4; q++) {
+        const double zq = sqrt(xq[q][0] * xq[q][0] + xq[q][1] * xq[q][1] + xq[q][2] * xq[q][2]) - JUR_RE;
+        pq[q] = pt_at(atm, zq, pk.lo);
+      }
+      const double N0 = CR * pq[0].p / pq[0].t;
+      for (int i = 0; i < 3; i++) ng[i] = (CR * pq[i + 1].p / pq[i + 1].t - N0) / HP;
+    }
+    double v[3];
+    for (int i = 0; i < 3; i++) v[i] = e[i] * n + ds * ng[i];
+    const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double e1[3];
+    for (int i = 0; i < 3; i++) e1[i] = v[i] / nv;
+    // this point's own values
+    double gzk[CH], gsk[CH];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      gzk[c] = c < nc ? GZ(c, k) : 0.0;
+      const double gdk = c < nc ? GD(c, k) : 0.0;
+      gsk[c] = (k == N - 2) ? 0.0 : 0.5 * (gdk + gdn[c]);  // point N-2's raw ds was clipped
+      gdn[c] = gdk;
+    }
+    // reverse of the step, per channel; the refractivities' adjoints (of the
+    // four probes and the point) go to the levels after the channel loop
+    double gNq[4][CH], gnk[CH];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      double gx[3], ge[3], ge1[3], gds = 0;
+      for (int i = 0; i < 3; i++) {
+        gx[i] = X[c][i];
+        ge[i] = 0.5 * ds * X[c][i];
+        ge1[i] = Ev[c][i] + 0.5 * ds * X[c][i];
+        gds += 0.5 * X[c][i] * (e[i] + e1[i]);
+      }
+      const double pe = e1[0] * ge1[0] + e1[1] * ge1[1] + e1[2] * ge1[2];
+      double V[3];
+      for (int i = 0; i < 3; i++) V[i] = (ge1[i] - e1[i] * pe) / nv;
+      const double gn = e[0] * V[0] + e[1] * V[1] + e[2] * V[2];
+      for (int i = 0; i < 3; i++) {
+        ge[i] += n * V[i];
+        gds += ng[i] * V[i];
+      }
+      if (rf) {
+        double gN[4];
+        gN[0] = 0;
+        for (int i = 0; i < 3; i++) {
+          gN[i + 1] = ds * V[i] / HP;
+          gN[0] -= ds * V[i] / HP;
+        }
+        double gxh[3] = {0, 0, 0};
+        for (int q = 0; q < 4; q++) {
+          const double dNdp = CR / pq[q].t, dNdt = -CR * pq[q].p / (pq[q].t * pq[q].t);
+          gNq[q][c] = gN[q];
+          const double gzq = gN[q] * (dNdp * pq[q].dp + dNdt * pq[q].dt);
+          const double rq = sqrt(xq[q][0] * xq[q][0] + xq[q][1] * xq[q][1] + xq[q][2] * xq[q][2]);
+          for (int i = 0; i < 3; i++) gxh[i] += gzq * xq[q][i] / rq;
+        }
+        for (int i = 0; i < 3; i++) {
+          gx[i] += gxh[i];
+          ge[i] += 0.5 * ds * gxh[i];
+          gds += 0.5 * e[i] * gxh[i];
+        }
+        const double dndp = CR / pk.t, dndt = -CR * pk.p / (pk.t * pk.t);
+        gnk[c] = gn;
+        gzk[c] += gn * (dndp * pk.dp + dndt * pk.dt);
+      }
+      for (int i = 0; i < 3; i++) gx[i] += gzk[c] * x[i] / r;
+      gds += gsk[c];
+      if (k == N - 2)
+        for (int i = 0; i < 3; i++) gx[i] += Xp[c][i] + gzp[c] * x[i] / r;
+      ds_grad(x, e, gds, gx, ge);
+      for (int i = 0; i < 3; i++) {
+        X[c][i] = gx[i];
+        Ev[c][i] = ge[i];
+      }
+    }
+    if (rf) {
+      double gt[CH], gp[CH];
+      for (int q = -1; q < 4; q++) {
+        const PtAt &P = q < 0 ? pk : pq[q];
+        const double dNdp = CR / P.t, dNdt = -CR * P.p / (P.t * P.t);
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+          const double g = q < 0 ? gnk[c] : gNq[q][c];
+          gt[c] = g * dNdt;
+          gp[c] = g * dNdp;
+        }
+        W.cover(P.lo, nlev, xm, kt0, ktstride, nc);
+        W.add(P.lo, P.w, P.fp0, P.fp1, gt, gp);
+      }
+    }
+  }
+  W.flush(xm, kt0, ktstride, nc);
+#undef GZ
+#undef GD
 }
 
 // Tile -> row-major K[ray][channel][col], with d BT / d rad (BRIGHT,
@@ -1108,35 +1462,51 @@ __global__ __launch_bounds__(WAVE) void k_jac_store(const double *kt, int nd, in
   }
 }
 
-template <bool CGA, typename SF>
+constexpr int PATH_CH = 2;  // channels per lane of k_adj_path
+
+template <bool CGA, typename SF, bool PATH>
 static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
-                       const RayInfo *info, int blk0, int nblk, const JacView &J, double *rad,
+                       const RayInfo *info, const void *tail, int blk0, int nblk, const JacView &J, double *rad,
                        double *tau, hipStream_t s) {
   const int nb8 = padded_blocks(nblk);
-  hipLaunchKernelGGL((k_adj_acc<SF>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0, nblk, J,
-                     rad, tau);
-  if (M.ng == 0) return;
-  const dim3 grid(nb8 * M.nd * M.ng), blk(WAVE);
-  if constexpr (!CGA && sizeof(SF) == sizeof(float)) {
-    if (E.part) {
-      hipLaunchKernelGGL((k_adj_chain<false, float, true>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
-      return;
+  hipLaunchKernelGGL((k_adj_acc<SF, PATH>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0, nblk,
+                     J, rad, tau);
+  if (M.ng > 0) {
+    const dim3 grid(nb8 * M.nd * M.ng), blk(WAVE);
+    bool fast = false;
+    if constexpr (!CGA && sizeof(SF) == sizeof(float)) {
+      if (E.part) {
+        hipLaunchKernelGGL((k_adj_chain<false, float, true, PATH>), grid, blk, 0, s, M, B, L, E, info, blk0,
+                           nblk, J);
+        fast = true;
+      }
     }
+    if (!fast)
+      hipLaunchKernelGGL((k_adj_chain<CGA, SF, false, PATH>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
   }
-  hipLaunchKernelGGL((k_adj_chain<CGA, SF, false>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
+  if (PATH) {
+    const int ngrp = (M.nd + PATH_CH - 1) / PATH_CH;
+    hipLaunchKernelGGL((k_adj_path<PATH_CH>), dim3(nb8 * ngrp), dim3(WAVE), 0, s, M, B, L, E, info,
+                       (const RayTail *)tail, blk0, nblk, J);
+  }
 }
 
 void launch_adjoint(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
-                    const RayInfo *info, int blk0, int nblk, const JacView &J, bool fp32, double *rad,
-                    double *tau, hipStream_t s) {
+                    const RayInfo *info, const void *tail, int blk0, int nblk, const JacView &J, bool fp32,
+                    double *rad, double *tau, hipStream_t s) {
   if (M.nd == 0) return;
+  const bool path = J.kz >= 0;
+#define JUR_ADJ(CGA, SF)                                                                  \
+  (path ? launch_adj<CGA, SF, true>(M, B, L, E, info, tail, blk0, nblk, J, rad, tau, s)  \
+        : launch_adj<CGA, SF, false>(M, B, L, E, info, tail, blk0, nblk, J, rad, tau, s))
   if (M.formod == 0) {
-    if (fp32) launch_adj<true, float>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
-    else launch_adj<true, double>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
+    if (fp32) JUR_ADJ(true, float);
+    else JUR_ADJ(true, double);
   } else {
-    if (fp32) launch_adj<false, float>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
-    else launch_adj<false, double>(M, B, L, E, info, blk0, nblk, J, rad, tau, s);
+    if (fp32) JUR_ADJ(false, float);
+    else JUR_ADJ(false, double);
   }
+#undef JUR_ADJ
 }
 
 void launch_jac_store(const double *kt, int nd, int ncol, int blk0, int nblk, int nray, const double *rad,

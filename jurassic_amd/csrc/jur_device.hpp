@@ -840,4 +840,54 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
   return x * per + x + k;
 }
 
+// ------------------------------------------------------------------
+// Ray state handed from the march (raytrace.hip) to its finish and to the
+// adjoint of the refracted path (adjoint.hip).
+// ------------------------------------------------------------------
+// First minimum of z along the LOS (gsl_stats_min_index semantics: a NaN
+// ends the search) and the positions of its two neighbours, for
+// tangent_point.
+struct MinTrack {
+  double zlow = 0;
+  double xbefore[3] = {0, 0, 0}, xafter[3] = {0, 0, 0};
+  int imin = 0;
+  bool after_pending = false, nan_seen = false;
+  // point np at z, position x; xprev is the position of point np - 1
+  __device__ __forceinline__ void add(int np, double z, const double *x, const double *xprev) {
+    if (!nan_seen) {
+      const bool lower = (np == 0) || z < zlow;
+      if (lower) zlow = z;
+      if (lower || isnan(z)) {
+        imin = np;
+        if (np > 0)
+          for (int i = 0; i < 3; i++) xbefore[i] = xprev[i];
+        after_pending = true;
+        nan_seen = isnan(z);
+      } else if (after_pending) {
+        for (int i = 0; i < 3; i++) xafter[i] = x[i];
+        after_pending = false;
+      }
+    } else if (after_pending) {
+      for (int i = 0; i < 3; i++) xafter[i] = x[i];
+      after_pending = false;
+    }
+  }
+};
+
+// What the march (k_raytrace) hands to k_ray_finish for a ray with LOS
+// points: the step that left the altitude range, which is clipped to it
+// there, and the state for the tangent point.
+struct RayTail {
+  double xprev[3];  // the last point written
+  double xcut[3];   // the step beyond the altitude range
+  double xfirst[3];
+  double zprev, zcut, dscut;  // altitudes of both, step length
+  double zmin, zmax;
+  double ds1, ds2;            // raw step lengths of the last two points written
+  double ds_imin, ds_imin1;   // raw step lengths at the first minimum and after it
+  double ecut[3];             // the march direction at the step beyond the range
+  MinTrack mt;
+  int zdir;
+};
+
 }  // namespace jur

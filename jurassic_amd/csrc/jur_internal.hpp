@@ -78,6 +78,9 @@ struct LosView {
   float *losf;  // FP32 forward pass: the fields rounded to float, same layout
                 // (the FP32 radiance kernel rounds every field it reads);
                 // null otherwise
+  int f_xe;     // Jacobian pass with refraction: first of six fields holding
+                // the point's Cartesian position and the march direction
+                // leaving it (x0 x1 x2 e0 e1 e2); -1 otherwise
 };
 
 // Per-segment emissivities of every (channel, emitter), written by the
@@ -108,8 +111,12 @@ struct JacView {
   double *kt;
   double *scr;
   const size_t *soff;   // [nblk]
-  int nscr;             // 1, or 2 with a reflecting surface
+  int nscr;             // values per point: tau_i/H_i, D_i with a reflecting
+                        // surface, then d rad / d z_i and d rad / d ds_i when
+                        // the path is differentiated
   int *nef;             // [blk][nd][WAVE] points before the first opaque segment
+  int kz;               // scratch index of d rad / d z_i (d rad / d ds_i at
+                        // kz + 1): the refracted path is differentiated; -1: not
 };
 
 #define LOS_AT(base, L, f, ip) (base)[((size_t)(ip) * (L).nfield + (size_t)(f)) * WAVE]

@@ -37,7 +37,8 @@ void launch_hydrostatic(atm_t *, int, double, int, hipStream_t);
 void launch_lookup_f64(const ModelView &, const BatchView &, const LosView &, const EpsView &,
                        const RayInfo *, int, int, bool, hipStream_t);
 void launch_adjoint(const ModelView &, const BatchView &, const LosView &, const EpsView &,
-                    const RayInfo *, int, int, const JacView &, bool, double *, double *, hipStream_t);
+                    const RayInfo *, const void *, int, int, const JacView &, bool, double *, double *,
+                    hipStream_t);
 void launch_jac_store(const double *, int, int, int, int, int, const double *, const ChanInfo *, int,
                       const unsigned char *, double *, hipStream_t);
 void launch_lookup_f32(const ModelView &, const BatchView &, const LosView &, const EpsView &,
@@ -666,9 +667,15 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     L.f_cl = s.ncl > 0 ? L.f_k + s.nw : -1;
     L.f_u = L.f_k + s.nw + (s.ncl > 0 ? 1 : 0);
     L.nfield = L.f_u + s.ng;
+    // with refraction the Jacobian follows the path's dependence on p and T:
+    // the march keeps each point's position and direction
+    const bool dpath = jac && s.refrac;
+    L.f_xe = dpath ? L.nfield : -1;
+    if (dpath) L.nfield += 6;
     const size_t esz = (precision == JUR_FP32) ? sizeof(float) : sizeof(double);
     // D_i is needed only where the surface reflects (sfeps < 1 needs NSF > 0)
-    const int nscr = (s.sftype >= 2 && s.nsf > 0) ? 2 : 1;
+    const int nscr0 = (s.sftype >= 2 && s.nsf > 0) ? 2 : 1;
+    const int nscr = nscr0 + (dpath ? 2 : 0);
     // FP32 EGA Jacobian over monotone tables: the lookups store each step's
     // partials (16 B per point, channel and emitter) for the reverse chains
     const bool part = jac && precision == JUR_FP32 && M.formod != 0 && M.tab.all_mono;
@@ -773,8 +780,9 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
           if ((rc = pm.get(c->d_nef, (size_t)ns * s.nd * WAVE + 1, &d_nef))) return rc;
           HIPCHK(hipMemcpyAsync(d_soff, soff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
           launch_zero(d_kt, (size_t)ns * s.nd * jac->ncol * WAVE, st);
-          JacView J{jac->xmap, jac->nmap, jac->prof_map, jac->ncol, d_kt, d_scr, d_soff, nscr, d_nef};
-          launch_adjoint(M, B, Ls, E, d_info, bs, ns, J, precision == JUR_FP32, d_rad, d_tau, st);
+          JacView J{jac->xmap, jac->nmap, jac->prof_map, jac->ncol, d_kt, d_scr, d_soff, nscr, d_nef,
+                    dpath ? nscr0 : -1};
+          launch_adjoint(M, B, Ls, E, d_info, d_tail, bs, ns, J, precision == JUR_FP32, d_rad, d_tau, st);
           HIPCHK(hipGetLastError());
           launch_jac_store(d_kt, s.nd, jac->ncol, bs, ns, B.nray, d_rad, c->d_chan.p, s.write_bbt, d_mask,
                            jac->k, st);
@@ -1459,6 +1467,92 @@ int jur_ctx_formod_device_noalloc(jur_ctx_t *ctx, const jur_batch_t *b, int prec
   BatchView B{b->nprof, b->atm, b->nray, b->ray_prof, b->ray_geom};
   return forward_pass(ctx, B, precision, b->mask, b->rad, b->tau, b->tp, nullptr, (hipStream_t)stream, nullptr,
                       &pm);
+}
+
+// Name of state quantity idx (idx2name, src/jurassic.c:3586-3627).
+static void quantity_name(const ctl_t *ctl, int idx, char *q, size_t len) {
+  const int ng = ctl->ng, nw = ctl->nw, ncl = ctl->ncl;
+  q[0] = 0;
+  if (idx == IDXP) snprintf(q, len, "PRESSURE");
+  else if (idx == IDXT) snprintf(q, len, "TEMPERATURE");
+  else if (idx < 2 + ng) snprintf(q, len, "%s", ctl->emitter[idx - 2]);
+  else if (idx < 2 + ng + nw) snprintf(q, len, "EXTINCT_WINDOW_%d", idx - 2 - ng);
+  else if (idx == 2 + ng + nw) snprintf(q, len, "CLOUD_HEIGHT");
+  else if (idx == 3 + ng + nw) snprintf(q, len, "CLOUD_DEPTH");
+  else if (idx < 4 + ng + nw + ncl) snprintf(q, len, "CLOUD_EXTINCT_%.4f", ctl->clnu[idx - 4 - ng - nw]);
+  else if (idx == 4 + ng + nw + ncl) snprintf(q, len, "SURFACE_HEIGHT");
+  else if (idx == 5 + ng + nw + ncl) snprintf(q, len, "SURFACE_PRESSURE");
+  else if (idx == 6 + ng + nw + ncl) snprintf(q, len, "SURFACE_TEMPERATURE");
+  else if (idx < 7 + ng + nw + ncl + ctl->nsf)
+    snprintf(q, len, "SURFACE_EMISSIVITY_%.4f", ctl->sfnu[idx - 7 - ng - nw - ncl]);
+}
+
+int jur_write_matrix(const char *dirname, const char *filename, const ctl_t *ctl, const double *matrix,
+                     const atm_t *atm, const obs_t *obs, const char *rowspace, const char *colspace,
+                     const char *sort) {
+  if (!filename || !ctl || !matrix || !atm || !obs || !rowspace || !colspace || !sort)
+    return set_err(JUR_ERR_ARG, "NULL argument");
+  if (!ctl->write_matrix) return JUR_OK;
+  if (obs->nr < 0 || obs->nr > NR || ctl->nd < 0 || ctl->nd > ND || atm->np < 0 || atm->np > NP)
+    return set_err(JUR_ERR_ARG, "Dimension out of range!");
+  // measurement space: rows of obs2y (ray-major, finite radiances);
+  // state space: elements of atm2x
+  std::vector<int> yd, yr, xq(JUR_XMAP_LEN), xp(JUR_XMAP_LEN);
+  for (int ir = 0; ir < obs->nr; ir++)
+    for (int id = 0; id < ctl->nd; id++)
+      if (isfinite(obs->rad[id][ir])) {
+        yd.push_back(id);
+        yr.push_back(ir);
+      }
+  const size_t nx = atm2x_h(ctl, atm, nullptr, xq.data(), xp.data());
+  const bool ry = rowspace[0] == 'y', cy = colspace[0] == 'y';
+  const size_t nr = ry ? yd.size() : nx, nc = cy ? yd.size() : nx;
+  std::string file = dirname ? std::string(dirname) + "/" + filename : std::string(filename);
+  FILE *out = fopen(file.c_str(), "w");
+  if (!out) return set_err(JUR_ERR_IO, "Cannot create file!");
+  static const char *yhead[6] = {"index (measurement space)", "channel wavenumber [cm^-1]",
+                                 "time (seconds since 2000-01-01T00:00Z)", "view point altitude [km]",
+                                 "view point longitude [deg]", "view point latitude [deg]"};
+  static const char *xhead[6] = {"index (state space)", "name of quantity",
+                                 "time (seconds since 2000-01-01T00:00Z)", "altitude [km]", "longitude [deg]",
+                                 "latitude [deg]"};
+  for (int k = 0; k < 6; k++) fprintf(out, "# $%d = Row: %s\n", k + 1, ry ? yhead[k] : xhead[k]);
+  for (int k = 0; k < 6; k++) fprintf(out, "# $%d = Col: %s\n", k + 7, cy ? yhead[k] : xhead[k]);
+  fprintf(out, "# $13 = Matrix element\n\n");
+  char q[LEN];
+  auto meta = [&](bool y, size_t i, const char *sep) {
+    if (y) {
+      const int id = yd[i], ir = yr[i];
+      fprintf(out, "%s%d %.4f %.2f %g %g %g", sep, (int)i, ctl->nu[id], obs->time[ir], obs->vpz[ir],
+              obs->vplon[ir], obs->vplat[ir]);
+    } else {
+      const int ip = xp[i];
+      quantity_name(ctl, xq[i], q, sizeof q);
+      fprintf(out, "%s%d %s %.2f %g %g %g", sep, (int)i, q, atm->time[ip], atm->z[ip], atm->lon[ip], atm->lat[ip]);
+    }
+  };
+  size_t i = 0, j = 0;
+  while (i < nr && j < nc) {
+    meta(ry, i, "");
+    meta(cy, j, " ");
+    fprintf(out, " %g\n", matrix[i * nc + j]);
+    if (sort[0] == 'r') {
+      if (++j >= nc) {
+        j = 0;
+        i++;
+        fprintf(out, "\n");
+      }
+    } else {
+      if (++i >= nr) {
+        i = 0;
+        j++;
+        fprintf(out, "\n");
+      }
+    }
+  }
+  const bool bad = ferror(out) != 0;
+  if (fclose(out) != 0 || bad) return set_err(JUR_ERR_IO, "Cannot write file!");
+  return JUR_OK;
 }
 
 int jur_state_map(const ctl_t *ctl, const atm_t *atm, int *xmap) {

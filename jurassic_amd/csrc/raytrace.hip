@@ -94,51 +94,6 @@ __device__ __forceinline__ double sza(double sec, double lon, double lat) {
   } while (0)
 #define LOSW2(f, ip, v) LOSW3(f, ip, v, true)
 
-// First minimum of z along the LOS (gsl_stats_min_index semantics: a NaN
-// ends the search) and the positions of its two neighbours, for
-// tangent_point.
-struct MinTrack {
-  double zlow = 0;
-  double xbefore[3] = {0, 0, 0}, xafter[3] = {0, 0, 0};
-  int imin = 0;
-  bool after_pending = false, nan_seen = false;
-  // point np at z, position x; xprev is the position of point np - 1
-  __device__ __forceinline__ void add(int np, double z, const double *x, const double *xprev) {
-    if (!nan_seen) {
-      const bool lower = (np == 0) || z < zlow;
-      if (lower) zlow = z;
-      if (lower || isnan(z)) {
-        imin = np;
-        if (np > 0)
-          for (int i = 0; i < 3; i++) xbefore[i] = xprev[i];
-        after_pending = true;
-        nan_seen = isnan(z);
-      } else if (after_pending) {
-        for (int i = 0; i < 3; i++) xafter[i] = x[i];
-        after_pending = false;
-      }
-    } else if (after_pending) {
-      for (int i = 0; i < 3; i++) xafter[i] = x[i];
-      after_pending = false;
-    }
-  }
-};
-
-// What the march (k_raytrace) hands to k_ray_finish for a ray with LOS
-// points: the step that left the altitude range, which is clipped to it
-// there, and the state for the tangent point.
-struct RayTail {
-  double xprev[3];  // the last point written
-  double xcut[3];   // the step beyond the altitude range
-  double xfirst[3];
-  double zprev, zcut, dscut;  // altitudes of both, step length
-  double zmin, zmax;
-  double ds1, ds2;            // raw step lengths of the last two points written
-  double ds_imin, ds_imin1;   // raw step lengths at the first minimum and after it
-  MinTrack mt;
-  int zdir;
-};
-
 __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, LosView L,
                                                    RayInfo *info, RayTail *tail, int blk0) {
   JUR_FP_EXACT
@@ -346,6 +301,11 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
                              : 0.0;
       LOSW2(L.f_cl, np, aux);
     }
+    if (L.f_xe >= 0)
+      for (int i = 0; i < 3; i++) {
+        LOSW(L.f_xe + i, np) = x[i];
+        LOSW(L.f_xe + 3 + i, np) = ex0[i];
+      }
     // MinTrack::add, with the raw step lengths at the minimum and after it
     if (!nan_seen) {
       const bool lower = (np == 0) || z < zlow;
@@ -427,6 +387,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     for (int i = 0; i < 3; i++) {
       tl->xprev[i] = xprev[i];
       tl->xcut[i] = x[i];
+      tl->ecut[i] = ex0[i];
     }
     tl->zprev = zprev;
     tl->zcut = z;
@@ -526,6 +487,11 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
                            : 0.0;
     LOSW2(L.f_cl, np, aux);
   }
+  if (L.f_xe >= 0)
+    for (int i = 0; i < 3; i++) {
+      LOSW(L.f_xe + i, np) = x[i];
+      LOSW(L.f_xe + 3 + i, np) = 0.0;
+    }
   mt.add(np, z, x, T.xprev);
   np++;
   if (M.nsf > 0 && atm->sft > 0) t = atm->sft;

@@ -9,8 +9,9 @@ the same scene and the GPU's wide pass is compared slice by slice:
   * forward model, EGA and CGA: FP64 within 1e-10 relative, FP32 within
     0.01 K brightness temperature;
   * adjoint Jacobian of T, H2O and O3 (n = 243; refraction off, so the path
-    is fixed) against the oracle's central differences: directional
-    derivatives along each block and single columns.
+    is fixed) against the oracle's central differences along directions in
+    each block, and against the Jacobian of a context of the first ND
+    channels (the path the other adjoint tests cover) to rounding.
 All GPU calls go through the C-ABI.
 """
 import ctypes as C
@@ -179,17 +180,6 @@ def _state_cols(atm):
     return {"T": np.arange(atm.np), "H2O": atm.np + np.arange(atm.np), "O3": 2 * atm.np + np.arange(atm.np)}
 
 
-def _perturbed(atm, block, levels, h, sign):
-    a = abi.copy_struct(atm)
-    for ip in levels:
-        if block == "T":
-            a.t[ip] += sign * h
-        else:
-            ig = GASES.index(block)
-            a.q[ig][ip] += sign * h * atm.q[ig][ip]
-    return a
-
-
 def test_wide_adjoint_t_h2o_o3_matches_oracle_central_fd(gpu, oracle, scene):
     """K of the wide pass (n = 243) against the oracle's central differences
     on both channel slices: along each block's all-level direction
@@ -221,28 +211,78 @@ def test_wide_adjoint_t_h2o_o3_matches_oracle_central_fd(gpu, oracle, scene):
     finally:
         ctx.close()
     # central differences of the oracle on channels at both sides of the ND
-    # boundary and at the top of the band, two view angles: (block, levels, step)
+    # boundary and at the top of the band, two view angles, along each
+    # block's all-level direction (T: 1 K, vmr: the profile itself) and a
+    # seeded random one (weights 0.5..1.5), the better of two steps.
+    # The forward model is continuous only to the float slopes of its table
+    # brackets (a bracket change moves the emissivity by ~1e-7 of the
+    # bracket's rise), so a central difference whose step straddles a knot
+    # picks up jump / 2h; the expected error does not shrink with h and
+    # sits near 1e-4 of the derivative for T at 2200 cm^-1 and, for the
+    # optically thin O3, near 1e-3.  The bounds are set above those floors;
+    # the narrow-context comparison below pins the wide pass exactly.
     chans = list(range(118, 138)) + list(range(150, ND_WIDE))
     angles = [0, 3]
     rows = [j * NPROF + k for j in angles for k in range(NPROF)]
-    probes = [("T", range(81), 1e-2), ("H2O", range(81), 1e-3), ("O3", range(81), 1e-3),
-              ("T", [3], 1e-2), ("T", [30], 1e-2), ("H2O", [2], 1e-3), ("O3", [28], 1e-3)]
-    pert = [_perturbed(a, block, levels, h, sgn) for block, levels, h in probes for sgn in (1, -1) for a in atms]
+    rng = np.random.default_rng(11)
+    dirs = []
+    for block, steps, tol64, tol32 in (("T", (1e-4, 1e-5), 3e-4, 1e-3), ("H2O", (1e-4, 1e-5), 3e-5, 1e-3),
+                                       ("O3", (1e-5, 1e-6), 3e-3, 3e-3)):
+        for wname, wts in (("all", np.ones(81)), ("random", rng.uniform(0.5, 1.5, 81))):
+            dirs.append((block, wname, wts, steps, tol64, tol32))
+
+    def perturbed(atm, block, wts, h):
+        a = abi.copy_struct(atm)
+        for ip in range(81):
+            if block == "T":
+                a.t[ip] += h * wts[ip]
+            else:
+                ig = GASES.index(block)
+                a.q[ig][ip] += h * wts[ip] * atm.q[ig][ip]
+        return a
+
+    pert = [perturbed(a, block, wts, sgn * h) for block, _, wts, steps, _, _ in dirs for h in steps
+            for sgn in (1, -1) for a in atms]
     y = _oracle_forward(oracle, scene, pert, 1, refrac=0, chans=chans, angles=angles)[0]
     na = len(pert)
-    for ipr, (block, levels, h) in enumerate(probes):
-        # rows of y: angle-major, then perturbed atmosphere
-        sel_p = [ia * na + (2 * ipr) * NPROF + k for ia in range(len(angles)) for k in range(NPROF)]
-        sel_m = [ia * na + (2 * ipr + 1) * NPROF + k for ia in range(len(angles)) for k in range(NPROF)]
-        fd = (y[sel_p] - y[sel_m]) / (2 * h)
-        # the same direction through K: T columns times 1 K, vmr columns
-        # times the vmr (the same for both profiles)
+    errs = []
+    ia0 = 0
+    for block, wname, wts, steps, tol64, tol32 in dirs:
         v = np.zeros(n)
-        for ip in levels:
-            c = cols[block][ip]
-            v[c] = 1.0 if block == "T" else atms[0].q[GASES.index(block)][ip]
-        for prec, tol in ((abi.JUR_FP64, 3e-5), (abi.JUR_FP32, 1e-3)):
+        for ip in range(81):
+            v[cols[block][ip]] = wts[ip] * (1.0 if block == "T" else atms[0].q[GASES.index(block)][ip])
+        fds = []
+        for h in steps:
+            sel_p = [ia * na + ia0 * NPROF + k for ia in range(len(angles)) for k in range(NPROF)]
+            sel_m = [ia * na + (ia0 + 1) * NPROF + k for ia in range(len(angles)) for k in range(NPROF)]
+            fds.append((y[sel_p] - y[sel_m]) / (2 * h))
+            ia0 += 2
+        for prec, tol in ((abi.JUR_FP64, tol64), (abi.JUR_FP32, tol32)):
             kv = ks[prec][rows][:, chans] @ v
-            err = float(np.max(np.abs(kv - fd))) / float(np.max(np.abs(fd)))
-            assert err < tol, (block, list(levels)[:3], prec, err)
-        assert float(np.max(np.abs(fd))) > 0
+            best = min(float(np.max(np.abs(kv - fd))) / float(np.max(np.abs(fd))) for fd in fds)
+            errs.append((best < tol, block, wname, prec, best, tol))
+    print("\n".join(str(e) for e in errs))
+    assert all(e[0] for e in errs), [e for e in errs if not e[0]]
+    # the wide pass's K equals that of a context of the first ND channels
+    # (the path every other adjoint test covers) on those channels
+    ctl_n = _ctl(scene, NU[:abi.ND], 1, refrac=0)
+    ctl_n.rett_zmin, ctl_n.rett_zmax = -100, 100
+    for g in ("H2O", "O3"):
+        ig = GASES.index(g)
+        ctl_n.retq_zmin[ig], ctl_n.retq_zmax[ig] = -100, 100
+    ctx = gpu.Context(ctl_n)
+    try:
+        rad = torch.zeros(nray, abi.ND, dtype=torch.float64, device="cuda:0")
+        tau = torch.zeros_like(rad)
+        kd = torch.zeros(nray, abi.ND, n, dtype=torch.float64, device="cuda:0")
+        pmap = torch.zeros(NPROF, dtype=torch.int32, device="cuda:0")
+        ctx.jacobian_device(NPROF, a_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(), rad.data_ptr(),
+                            tau.data_ptr(), xm[None, :], pmap.data_ptr(), n, kd.data_ptr(),
+                            precision=abi.JUR_FP64)
+        torch.cuda.synchronize()
+        kn = kd.cpu().numpy()
+    finally:
+        ctx.close()
+    kw = ks[abi.JUR_FP64][:, :abi.ND]
+    # T columns are summed with atomics by the emitter chains (last-bit order)
+    np.testing.assert_allclose(kw, kn, rtol=1e-12, atol=1e-12 * np.abs(kn).max())
