@@ -1113,6 +1113,155 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
 #undef SCR
 }
 
+// The FP32 EGA chains of all emitters of a row in one wavefront (EM >= ng,
+// at most four; k_lookup_f32 stored each step's table partials).  The emitters
+// share the point's LOS values, H and level pair: their T and p adjoints are
+// summed per point before they enter the level window, and with PATH their
+// parts of d rad / d z_i and d rad / d ds_i are added to the accumulation's
+// in one read-modify-write per point.
+template <int EM, bool PATH>
+__global__ __launch_bounds__(WAVE) void k_adj_chains_f32(ModelView M, BatchView B, LosView L, EpsView E,
+                                                         const RayInfo *info, int blk0, int nblk, JacView J) {
+  const int nd = M.nd, ng = M.ng;
+  int bl, id;
+  if (!xcd_block_map(blockIdx.x, nd, nblk, bl, id)) return;
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + bl) * WAVE + lane;
+  if (ir >= B.nray) return;
+  const RayInfo R = info[ir];
+  if (R.status != JUR_OK) return;
+  const int ipf = B.ray_prof[ir];
+  const int im = J.prof_map[ipf];
+  if (im < 0 || im >= J.nmap) return;
+  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
+  const atm_t *atm = &B.atm[ipf];
+  const int npb = E.npb[bl];
+  const int nef = J.nef[((size_t)bl * nd + id) * WAVE + lane];
+  double *kt = J.kt + ((size_t)bl * nd + id) * J.ncol * WAVE + lane;
+  const double *losb = L.los + (size_t)bl * L.cap * L.nfield * WAVE;
+  const unsigned nfield = L.nfield;
+  const float *eps0 = (const float *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE;
+  const float4 *pab0 = E.part + E.off[bl] + (size_t)id * ng * npb * WAVE;
+  double *scb = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE;
+  const unsigned nscr = J.nscr;
+#define LOSQ(f, ip) ld32(losb, ((unsigned)(ip) * nfield + (unsigned)(f)) * WAVE + lane)
+#define EPSQ(g, ip) ld32(eps0, ((unsigned)(g) * npb + (unsigned)(ip)) * WAVE + lane)
+#define PARQ(g, ip) ld32(pab0, ((unsigned)(g) * npb + (unsigned)(ip)) * WAVE + lane)
+#define SCQ(k, ip) scb[((size_t)(ip) * nscr + (k)) * WAVE + lane]
+  // quantities of the level window: T, p, then q of each emitter
+  int qi[2 + EM];
+  bool on[EM];
+  qi[0] = 1;
+  qi[1] = 0;
+#pragma unroll
+  for (int g = 0; g < EM; g++) {
+    on[g] = g < ng && M.tab.np[g * nd + id] >= 30;
+    qi[2 + g] = on[g] ? 2 + g : -1;
+  }
+  LevelWindow<2 + EM, (1u << (2 + EM)) - 1u, true, PATH> W;
+  int lev = -1;
+  double carry[EM], trc[EM], useg[EM];
+  float epf[EM];
+  float4 dc[EM];
+  double p = 0, t = 0, Hc = 0, zc = 0, dsc = 0, gzc = 0, gdc = 0, Hn = 0;
+#pragma unroll
+  for (int g = 0; g < EM; g++) {
+    carry[g] = 0;
+    trc[g] = 1;
+    useg[g] = 0;
+    epf[g] = 0.f;
+    dc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (nef > 0) {
+    const int ip = nef - 1;
+    p = LOSQ(F_P, ip);
+    t = LOSQ(F_T, ip);
+    zc = LOSQ(F_Z, ip);
+    dsc = LOSQ(F_DS, ip);
+    Hc = SCQ(0, ip);
+    if (PATH) {
+      gzc = SCQ(J.kz, ip);
+      gdc = SCQ(J.kz + 1, ip);
+    }
+#pragma unroll
+    for (int g = 0; g < EM; g++) {
+      if (!on[g]) continue;
+      trc[g] = 1.0 - (double)EPSQ(g, ip);
+      dc[g] = PARQ(g, ip);
+      useg[g] = LOSQ(L.f_u + g, ip);
+      epf[g] = ip > 0 ? EPSQ(g, ip - 1) : 0.f;
+    }
+  }
+  for (int ip = nef - 1; ip >= 0; ip--) {
+    // the next point's inputs
+    const int ipn = ip > 0 ? ip - 1 : 0;
+    const double pn = LOSQ(F_P, ipn), tn = LOSQ(F_T, ipn);
+    const double zx = LOSQ(F_Z, ipn), dsx = LOSQ(F_DS, ipn);
+    const double Hx = SCQ(0, ipn);
+    double gzx = 0, gdx = 0;
+    if (PATH) {
+      gzx = SCQ(J.kz, ipn);
+      gdx = SCQ(J.kz + 1, ipn);
+    }
+    float4 dx[EM];
+    double un[EM];
+    float en[EM];
+#pragma unroll
+    for (int g = 0; g < EM; g++) {
+      dx[g] = on[g] ? PARQ(g, ipn) : make_float4(0.f, 0.f, 0.f, 0.f);
+      un[g] = on[g] ? LOSQ(L.f_u + g, ipn) : 0.0;
+      en[g] = (on[g] && ipn > 0) ? EPSQ(g, ipn - 1) : 0.f;
+    }
+    const double H = Hc, ds = dsc;
+    // the chain rule's divisions by the transmittance, T and p as float
+    // reciprocals (relative error 1e-7, far below the FP32 lookups')
+    const double rt = (double)frcp((float)t), rp = (double)frcp((float)p);
+    const double ucoef = 10 * p * rt * ds * (1 / JUR_KB);
+    double v[2 + EM];
+    double vt = 0, vp = 0, gu = 0;
+#pragma unroll
+    for (int g = 0; g < EM; g++) {
+      v[2 + g] = 0;
+      if (!on[g]) continue;
+      const double lam = (Hn - H) * (double)frcp((float)trc[g]) + carry[g];
+      // the stored partials are (d eps_prev, d p, d T, d u), as LkPart
+      const double u_bar = lam * dc[g].w;
+      carry[g] = lam * dc[g].x;
+      vt += lam * dc[g].z - u_bar * useg[g] * rt;
+      vp += lam * dc[g].y + u_bar * useg[g] * rp;
+      v[2 + g] = u_bar * ucoef;
+      gu += u_bar * useg[g];
+      trc[g] = ip > 0 ? 1.0 - (double)epf[g] : 1.0;
+    }
+    v[0] = vt;
+    v[1] = vp;
+    const double gz = W.add(atm, zc, lev, v, 1, p, xm, qi, kt);
+    if (PATH) {
+      SCQ(J.kz, ip) = gzc + gz;
+      SCQ(J.kz + 1, ip) = gdc + gu / ds;
+    }
+    Hn = H;
+    p = pn;
+    t = tn;
+    zc = zx;
+    dsc = dsx;
+    Hc = Hx;
+    gzc = gzx;
+    gdc = gdx;
+#pragma unroll
+    for (int g = 0; g < EM; g++) {
+      dc[g] = dx[g];
+      useg[g] = un[g];
+      epf[g] = en[g];
+    }
+  }
+  W.flush(xm, qi, kt);
+#undef LOSQ
+#undef EPSQ
+#undef PARQ
+#undef SCQ
+}
+
 // ------------------------------------------------------------------
 // Adjoint of the refracted path (raytrace, src/jurassic.c:4201-4438).
 //
@@ -1129,58 +1278,13 @@ __global__ __launch_bounds__(WAVE) void k_adj_chain(ModelView M, BatchView B, Lo
 // chains stored (ds_k entering through the trapezoid rule, the last point's
 // through the clip of the step that left the altitude range, k_ray_finish).
 // Each probe's and point's N goes to its two levels with intpol_atm's
-// weights (LOGY in p, LIN in T).  A lane is a ray and carries the adjoints
-// of CH channels, whose steps share the recomputed march; level sums are
-// kept for a window of four levels per channel and added to the K tile
-// when the window moves.
-template <int CH>
-struct PathWin {
-  int wb = -1;
-  double at[CH][4], ap[CH][4];
-  __device__ __forceinline__ void flush(const int *xm, double *kt0, size_t ktstride, int nc) {
-    if (wb < 0) return;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int ct = xm[JUR_XMAP_LEV(1, wb + j)], cp = xm[JUR_XMAP_LEV(0, wb + j)];
-#pragma unroll
-      for (int c = 0; c < CH; c++) {
-        if (c >= nc) continue;
-        double *kt = kt0 + (size_t)c * ktstride;
-        if (ct >= 0) kt[(size_t)ct * WAVE] += at[c][j];
-        if (cp >= 0) kt[(size_t)cp * WAVE] += ap[c][j];
-      }
-    }
-  }
-  // levels lo and lo + 1 inside the window (moving it when needed)
-  __device__ __forceinline__ void cover(int lo, int nlev, const int *xm, double *kt0, size_t ktstride, int nc) {
-    if (wb >= 0 && lo >= wb && lo + 1 <= wb + 3) return;
-    flush(xm, kt0, ktstride, nc);
-    int b = lo - 1;
-    if (b > nlev - 4) b = nlev - 4;
-    if (b < 0) b = 0;
-    wb = b;
-#pragma unroll
-    for (int c = 0; c < CH; c++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) at[c][j] = ap[c][j] = 0.0;
-  }
-  // g[c] d N / d (levels) of N = CR p/t at a point of pair lo, weight w
-  // (fp = p/P_lo (1-w), fq = p/P_hi w for LOGY, else the LIN weights)
-  __device__ __forceinline__ void add(int lo, double w, double fp0, double fp1, const double *gt,
-                                      const double *gp) {
-    const int d = lo - wb;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const double wt = j == d ? 1 - w : (j == d + 1 ? w : 0.0);
-      const double wp = j == d ? fp0 : (j == d + 1 ? fp1 : 0.0);
-#pragma unroll
-      for (int c = 0; c < CH; c++) {
-        at[c][j] += wt * gt[c];
-        ap[c][j] += wp * gp[c];
-      }
-    }
-  }
-};
+// weights (LOGY in p, LIN in T).
+//
+// The reverse step is linear in (X, E) and its coefficients do not depend
+// on the channel: k_path_coef recomputes each forward step once (in
+// parallel over steps) and stores the coefficients; k_adj_path applies them
+// to the adjoints of CH channels per lane.
+// ------------------------------------------------------------------
 
 // p, t and their altitude slopes at z in level pair lo (weight w), and the
 // LOGY weights of p on the two levels
@@ -1212,6 +1316,211 @@ __device__ __forceinline__ PtAt pt_at(const atm_t *atm, double z, int hint) {
   return r;
 }
 
+constexpr double PATH_CR = 7.753e-05, PATH_HP = 0.02, PATH_ZREF = 60;
+
+// One forward step of the march from (x, e), recomputed: its length, the
+// refractive index and gradient, the new direction e1, and the p and T of
+// the point (pk) and of the four probes (pq, at xq).
+struct MarchStep {
+  double r, xh[3], ds, n, ng[3], nv, e1[3];
+  bool rf;
+  PtAt pk, pq[4];
+  double xq[4][3];
+};
+__device__ __forceinline__ void march_step(const ModelView &M, const atm_t *atm, const double *x, const double *e,
+                                           int &hint, MarchStep &S) {
+  S.r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  for (int i = 0; i < 3; i++) S.xh[i] = x[i] / S.r;
+  const double z = S.r - JUR_RE;
+  S.ds = M.rayds;
+  if (M.raydz > 0) {
+    const double ca = fabs((e[0] * x[0] + e[1] * x[1] + e[2] * x[2]) / S.r);
+    if (ca != 0) S.ds = (M.rayds < M.raydz / ca) ? M.rayds : M.raydz / ca;
+  }
+  S.rf = M.refrac && z <= PATH_ZREF;
+  S.n = 1;
+  S.ng[0] = S.ng[1] = S.ng[2] = 0;
+  if (S.rf) {
+    S.pk = pt_at(atm, z, hint);
+    hint = S.pk.lo;
+    S.n = 1 + PATH_CR * S.pk.p / S.pk.t;
+    for (int i = 0; i < 3; i++) S.xq[0][i] = x[i] + 0.5 * S.ds * e[i];
+#pragma unroll
+    for (int q = 1; q < 4; q++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) S.xq[q][i] = S.xq[0][i] + (i == q - 1 ? PATH_HP : 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double zq =
+          sqrt(S.xq[q][0] * S.xq[q][0] + S.xq[q][1] * S.xq[q][1] + S.xq[q][2] * S.xq[q][2]) - JUR_RE;
+      S.pq[q] = pt_at(atm, zq, S.pk.lo);
+    }
+    const double N0 = PATH_CR * S.pq[0].p / S.pq[0].t;
+    for (int i = 0; i < 3; i++) S.ng[i] = (PATH_CR * S.pq[i + 1].p / S.pq[i + 1].t - N0) / PATH_HP;
+  }
+  double v[3];
+  for (int i = 0; i < 3; i++) v[i] = e[i] * S.n + S.ds * S.ng[i];
+  S.nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int i = 0; i < 3; i++) S.e1[i] = v[i] / S.nv;
+}
+
+// Coefficients of the reverse of step k, per channel with the adjoint (X, E)
+// of (x_k+1, e_k+1) and the point values gz = d rad / d z_k, gs = the
+// step-length share of d rad / d ds:
+//   V   = (G - e1 (e1 . G)) / |v|,  G = E + ds/2 X
+//   gds = (e + e1)/2 . X + NGM . V + gs
+//   X'  = X + MX V + (gz + SK e . V) XH + gds A
+//   E'  = ds/2 (X + MX V) + n V + gds B
+// MX V is the probes' share (d x_k of the four refractivities), NGM = grad N
+// + MX^T e / 2, A and B the step length's dependence on (x_k, e_k).  The
+// refractivities' adjoints e . V (point) and ds/HP (-sum V, V_0, V_1, V_2)
+// (probes) go to the levels LB, LB + 1, LB + 2 as TAU_j . V (temperature)
+// and PI_j . V (pressure); LB = -1: above 60 km, no level terms.  LB = -2:
+// the five points span more than three levels; then slots PC_W + 4q hold
+// point q's factors dN/dT (1 - w), dN/dT w, dN/dp f_lo, dN/dp f_hi and
+// PC_LO its level pair (8 bits per point).
+constexpr int PC_HDS = 0, PC_E = 1, PC_E1 = 4, PC_RNV = 7, PC_N = 8, PC_NGM = 9, PC_MX = 12, PC_SK = 21,
+              PC_XH = 22, PC_A = 25, PC_B = 28, PC_LB = 31, PC_TAU = 32, PC_PI = 41, PC_W = 32, PC_LO = 52;
+static_assert(PC_LO + 1 == JUR_NPCO && NP <= 256, "reverse-step coefficient layout");
+constexpr int PCO_STEPS = 16;  // steps per lane of k_path_coef
+
+// The point's and the probes' refractivity adjoints as linear forms of V:
+// point e . V, probe 0 -f (V_0 + V_1 + V_2), probe q f V_q-1 (f = ds/HP).
+__device__ __forceinline__ void point_forms(const double *e, double f, double c[5][3]) {
+  for (int i = 0; i < 3; i++) {
+    c[0][i] = e[i];
+    c[1][i] = -f;
+    for (int q = 0; q < 3; q++) c[2 + q][i] = i == q ? f : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(WAVE) void k_path_coef(ModelView M, BatchView B, LosView L, const RayInfo *info,
+                                                    int blk0, int nblk, int nchunk, JacView J) {
+  int bl, chk;
+  if (!xcd_block_map(blockIdx.x, nchunk, nblk, bl, chk)) return;
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + bl) * WAVE + lane;
+  if (ir >= B.nray) return;
+  const RayInfo R = info[ir];
+  if (R.status != JUR_OK || R.np < 2) return;
+  const int k0 = chk * PCO_STEPS;
+  const int k1 = k0 + PCO_STEPS < R.np - 1 ? k0 + PCO_STEPS : R.np - 1;
+  if (k0 >= k1) return;
+  const atm_t *atm = &B.atm[B.ray_prof[ir]];
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  double *pc = J.pco + J.soff[bl] / ((size_t)M.nd * J.nscr) * JUR_NPCO + lane;
+  int hint = -1;
+  for (int k = k0; k < k1; k++) {
+    double x[3], e[3];
+    for (int i = 0; i < 3; i++) {
+      x[i] = LOSR(L.f_xe + i, k);
+      e[i] = LOSR(L.f_xe + 3 + i, k);
+    }
+    MarchStep S;
+    march_step(M, atm, x, e, hint, S);
+    double *o = pc + (size_t)k * JUR_NPCO * WAVE;
+#define PCW(f, v) o[(size_t)(f) * WAVE] = (v)
+    PCW(PC_HDS, 0.5 * S.ds);
+    PCW(PC_RNV, 1 / S.nv);
+    PCW(PC_N, S.n);
+    double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    if (M.raydz > 0) {
+      const double cc = e[0] * S.xh[0] + e[1] * S.xh[1] + e[2] * S.xh[2];
+      if (cc != 0 && M.raydz / fabs(cc) < M.rayds) {
+        const double cdz = -M.raydz * (cc > 0 ? 1.0 : -1.0) / (cc * cc);
+        for (int i = 0; i < 3; i++) {
+          b[i] = cdz * S.xh[i];
+          a[i] = cdz * (e[i] - cc * S.xh[i]) / S.r;
+        }
+      }
+    }
+    double mx[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, ngm[3], sk = 0;
+    double tau[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, pi[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int i = 0; i < 3; i++) ngm[i] = S.ng[i];
+    int lb = -1;
+    if (S.rf) {
+      const double f = S.ds / PATH_HP;
+      double u[4][3];
+      for (int q = 0; q < 4; q++) {
+        const PtAt &P = S.pq[q];
+        const double sq = PATH_CR / P.t * P.dp - PATH_CR * P.p / (P.t * P.t) * P.dt;
+        const double rq = sqrt(S.xq[q][0] * S.xq[q][0] + S.xq[q][1] * S.xq[q][1] + S.xq[q][2] * S.xq[q][2]);
+        for (int i = 0; i < 3; i++) u[q][i] = sq * S.xq[q][i] / rq;
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) mx[i][j] = f * (u[j + 1][i] - u[0][i]);
+      for (int j = 0; j < 3; j++) ngm[j] += 0.5 * (e[0] * mx[0][j] + e[1] * mx[1][j] + e[2] * mx[2][j]);
+      sk = PATH_CR / S.pk.t * S.pk.dp - PATH_CR * S.pk.p / (S.pk.t * S.pk.t) * S.pk.dt;
+      const int lo = min(min(min(S.pk.lo, S.pq[0].lo), min(S.pq[1].lo, S.pq[2].lo)), S.pq[3].lo);
+      const int hi = max(max(max(S.pk.lo, S.pq[0].lo), max(S.pq[1].lo, S.pq[2].lo)), S.pq[3].lo);
+      if (hi > lo + 1) {
+        lb = -2;
+        auto wide = [&](int q, const PtAt &P) {
+          const double dNdt = -PATH_CR * P.p / (P.t * P.t), dNdp = PATH_CR / P.t;
+          PCW(PC_W + 4 * q, dNdt * (1 - P.w));
+          PCW(PC_W + 4 * q + 1, dNdt * P.w);
+          PCW(PC_W + 4 * q + 2, dNdp * P.fp0);
+          PCW(PC_W + 4 * q + 3, dNdp * P.fp1);
+        };
+        wide(0, S.pk);
+        wide(1, S.pq[0]);
+        wide(2, S.pq[1]);
+        wide(3, S.pq[2]);
+        wide(4, S.pq[3]);
+        const unsigned long long los5 = (unsigned long long)S.pk.lo | (unsigned long long)S.pq[0].lo << 8 |
+                                        (unsigned long long)S.pq[1].lo << 16 |
+                                        (unsigned long long)S.pq[2].lo << 24 | (unsigned long long)S.pq[3].lo << 32;
+        PCW(PC_LO, (double)los5);
+      } else {
+        lb = lo;
+        double c[5][3];
+        point_forms(e, f, c);
+        // point P with form c: its two levels' shares (P.lo is lb or lb + 1)
+        auto addpt = [&](const PtAt &P, const double *cq) {
+          const double dNdt = -PATH_CR * P.p / (P.t * P.t), dNdp = PATH_CR / P.t;
+          const double t0 = dNdt * (1 - P.w), t1 = dNdt * P.w, p0 = dNdp * P.fp0, p1 = dNdp * P.fp1;
+          const bool up = P.lo != lb;
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            tau[0][i] += up ? 0.0 : t0 * cq[i];
+            tau[1][i] += (up ? t0 : t1) * cq[i];
+            tau[2][i] += up ? t1 * cq[i] : 0.0;
+            pi[0][i] += up ? 0.0 : p0 * cq[i];
+            pi[1][i] += (up ? p0 : p1) * cq[i];
+            pi[2][i] += up ? p1 * cq[i] : 0.0;
+          }
+        };
+        addpt(S.pk, c[0]);
+        addpt(S.pq[0], c[1]);
+        addpt(S.pq[1], c[2]);
+        addpt(S.pq[2], c[3]);
+        addpt(S.pq[3], c[4]);
+      }
+    }
+    for (int i = 0; i < 3; i++) {
+      PCW(PC_E + i, e[i]);
+      PCW(PC_E1 + i, S.e1[i]);
+      PCW(PC_NGM + i, ngm[i]);
+      PCW(PC_XH + i, S.xh[i]);
+      PCW(PC_A + i, a[i]);
+      PCW(PC_B + i, b[i]);
+      for (int j = 0; j < 3; j++) {
+        PCW(PC_MX + 3 * i + j, mx[i][j]);
+        if (lb != -2) {
+          PCW(PC_TAU + 3 * i + j, tau[i][j]);
+          PCW(PC_PI + 3 * i + j, pi[i][j]);
+        }
+      }
+    }
+    PCW(PC_SK, sk);
+    PCW(PC_LB, (double)lb);
+#undef PCW
+  }
+}
+
+// The reverse sweep over the steps, CH channels per lane (one ray).  Level
+// sums are kept for a window of three levels (the step's LB and the two
+// above) per channel and added to the K tile when the window moves.
 template <int CH>
 __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, LosView L, EpsView E,
                                                    const RayInfo *info, const RayTail *tail, int blk0, int nblk,
@@ -1241,33 +1550,10 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
 #define GD(c, ip) sc0[(c) * scstride + ((size_t)(ip) * J.nscr + J.kz + 1) * WAVE]
   double *kt0 = J.kt + ((size_t)bl * nd + id0) * J.ncol * WAVE + lane;
   const size_t ktstride = (size_t)J.ncol * WAVE;
-  const double CR = 7.753e-05, HP = 0.02, ZREF = 60;
+  const double *pc = J.pco + J.soff[bl] / ((size_t)nd * J.nscr) * JUR_NPCO + lane;
   const RayTail T = tail[ir];
   const int N = R.np;
   auto xe = [&](int ip, int i) { return LOSR(L.f_xe + i, ip); };
-
-  // d ds / d (x, e) of the step length at (x, e), applied to a scalar g
-  auto ds_grad = [&](const double *x, const double *e, double gds, double *gx, double *ge) {
-    if (!(M.raydz > 0)) return;
-    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-    const double xh[3] = {x[0] / r, x[1] / r, x[2] / r};
-    const double cc = e[0] * xh[0] + e[1] * xh[1] + e[2] * xh[2];
-    if (cc == 0 || !(M.raydz / fabs(cc) < M.rayds)) return;
-    const double d = -M.raydz * (cc > 0 ? 1.0 : -1.0) / (cc * cc) * gds;
-    for (int i = 0; i < 3; i++) {
-      ge[i] += d * xh[i];
-      gx[i] += d * (e[i] - cc * xh[i]) / r;
-    }
-  };
-  auto step_ds = [&](const double *x, const double *e) {
-    double ds = M.rayds;
-    if (M.raydz > 0) {
-      const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-      const double ca = fabs((e[0] * x[0] + e[1] * x[1] + e[2] * x[2]) / r);
-      if (ca != 0) ds = (M.rayds < M.raydz / ca) ? M.rayds : M.raydz / ca;
-    }
-    return ds;
-  };
 
   double X[CH][3], Ev[CH][3], Xp[CH][3], gzp[CH], gdn[CH];
   // ---- the clipped last point: x_l = x_p + frac (x_cut - x_p), its raw
@@ -1280,6 +1566,20 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
     for (int i = 0; i < 3; i++) xl[i] = xe(N - 1, i);
     const double rl = sqrt(xl[0] * xl[0] + xl[1] * xl[1] + xl[2] * xl[2]);
     const double rc = sqrt(T.xcut[0] * T.xcut[0] + T.xcut[1] * T.xcut[1] + T.xcut[2] * T.xcut[2]);
+    // d ds / d (x, e) at the cut point
+    double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    if (M.raydz > 0) {
+      double xh[3];
+      for (int i = 0; i < 3; i++) xh[i] = T.xcut[i] / rc;
+      const double cc = T.ecut[0] * xh[0] + T.ecut[1] * xh[1] + T.ecut[2] * xh[2];
+      if (cc != 0 && M.raydz / fabs(cc) < M.rayds) {
+        const double cdz = -M.raydz * (cc > 0 ? 1.0 : -1.0) / (cc * cc);
+        for (int i = 0; i < 3; i++) {
+          b[i] = cdz * xh[i];
+          a[i] = cdz * (T.ecut[i] - cc * xh[i]) / rc;
+        }
+      }
+    }
 #pragma unroll
     for (int c = 0; c < CH; c++) {
       const double gz = c < nc ? GZ(c, N - 1) : 0.0;
@@ -1287,138 +1587,159 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
       const double gdm = c < nc ? GD(c, N - 2) : 0.0;
       gdn[c] = gdm;
       const double gs = 0.5 * (gdm + gdl);  // d rad / d (clipped raw ds of point N-2)
-      double gfrac = T.dscut * gs, xk[3], ek[3] = {0, 0, 0};
+      double gfrac = T.dscut * gs;
       for (int i = 0; i < 3; i++) {
         const double xli = gz * xl[i] / rl;
         Xp[c][i] = (1 - frac) * xli;
-        xk[i] = frac * xli;
+        X[c][i] = frac * xli;
         gfrac += xli * (T.xcut[i] - T.xprev[i]);
       }
       gzp[c] = gfrac * (frac - 1) / (zc - zp);
       const double gzc = -gfrac * frac / (zc - zp);
-      for (int i = 0; i < 3; i++) xk[i] += gzc * T.xcut[i] / rc;
-      ds_grad(T.xcut, T.ecut, frac * gs, xk, ek);
       for (int i = 0; i < 3; i++) {
-        X[c][i] = xk[i];
-        Ev[c][i] = ek[i];
+        X[c][i] += gzc * T.xcut[i] / rc + frac * gs * a[i];
+        Ev[c][i] = frac * gs * b[i];
       }
     }
   }
 
-  PathWin<CH> W;
-  int hint = -1;
-  for (int k = N - 2; k >= 0; k--) {
-    double x[3], e[3];
+  double Tw[CH][3], Pw[CH][3];
+  int wb = -1;
+  auto flush_level = [&](int j) {
+    const int l = wb + j;
+    if (l >= nlev) return;
+    const int ct = xm[JUR_XMAP_LEV(1, l)], cp = xm[JUR_XMAP_LEV(0, l)];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      if (c >= nc) continue;
+      double *kt = kt0 + (size_t)c * ktstride;
+      if (ct >= 0) kt[(size_t)ct * WAVE] += Tw[c][j];
+      if (cp >= 0) kt[(size_t)cp * WAVE] += Pw[c][j];
+    }
+  };
+  auto step = [&](int k) {
+    const double *o = pc + (size_t)k * JUR_NPCO * WAVE;
+#define PCR(f) o[(size_t)(f) * WAVE]
+    const double hds = PCR(PC_HDS), rnv = PCR(PC_RNV), n = PCR(PC_N), sk = PCR(PC_SK);
+    double e[3], e1[3], ngm[3], xh[3], a[3], b[3], mx[9];
     for (int i = 0; i < 3; i++) {
-      x[i] = xe(k, i);
-      e[i] = xe(k, 3 + i);
+      e[i] = PCR(PC_E + i);
+      e1[i] = PCR(PC_E1 + i);
+      ngm[i] = PCR(PC_NGM + i);
+      xh[i] = PCR(PC_XH + i);
+      a[i] = PCR(PC_A + i);
+      b[i] = PCR(PC_B + i);
     }
-    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-    const double z = r - JUR_RE;
-    const double ds = step_ds(x, e);
-    const bool rf = M.refrac && z <= ZREF;
-    // the forward step, recomputed
-    double n = 1, ng[3] = {0, 0, 0}, xq[4][3];
-    PtAt pk{}, pq[4];
-    if (rf) {
-      pk = pt_at(atm, z, hint);
-      hint = pk.lo;
-      n = 1 + CR * pk.p / pk.t;
-      for (int i = 0; i < 3; i++) xq[0][i] = x[i] + 0.5 * ds * e[i];
-      for (int q = 1; q < 4; q++)
-        for (int i = 0; i < 3; i++) xq[q][i] = xq[0][i] + (i == q - 1 ? HP : 0.0);
-      for (int q = 0; q < 4; q++) {
-        const double zq = sqrt(xq[q][0] * xq[q][0] + xq[q][1] * xq[q][1] + xq[q][2] * xq[q][2]) - JUR_RE;
-        pq[q] = pt_at(atm, zq, pk.lo);
-      }
-      const double N0 = CR * pq[0].p / pq[0].t;
-      for (int i = 0; i < 3; i++) ng[i] = (CR * pq[i + 1].p / pq[i + 1].t - N0) / HP;
-    }
-    double v[3];
-    for (int i = 0; i < 3; i++) v[i] = e[i] * n + ds * ng[i];
-    const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    double e1[3];
-    for (int i = 0; i < 3; i++) e1[i] = v[i] / nv;
-    // this point's own values
-    double gzk[CH], gsk[CH];
+    for (int i = 0; i < 9; i++) mx[i] = PCR(PC_MX + i);
+    const int lb = (int)PCR(PC_LB);
+    double V[CH][3];
 #pragma unroll
     for (int c = 0; c < CH; c++) {
-      gzk[c] = c < nc ? GZ(c, k) : 0.0;
-      const double gdk = c < nc ? GD(c, k) : 0.0;
-      gsk[c] = (k == N - 2) ? 0.0 : 0.5 * (gdk + gdn[c]);  // point N-2's raw ds was clipped
-      gdn[c] = gdk;
+      const double gz = c < nc ? GZ(c, k) : 0.0;
+      const double gd = c < nc ? GD(c, k) : 0.0;
+      const double gs = (k == N - 2) ? 0.0 : 0.5 * (gd + gdn[c]);  // point N-2's raw ds was clipped
+      gdn[c] = gd;
+      double G[3];
+      for (int i = 0; i < 3; i++) G[i] = Ev[c][i] + hds * X[c][i];
+      const double pe = e1[0] * G[0] + e1[1] * G[1] + e1[2] * G[2];
+      double v[3];
+      for (int i = 0; i < 3; i++) v[i] = (G[i] - e1[i] * pe) * rnv;
+      const double gn = e[0] * v[0] + e[1] * v[1] + e[2] * v[2];
+      double gds = gs;
+      for (int i = 0; i < 3; i++) gds += 0.5 * (e[i] + e1[i]) * X[c][i] + ngm[i] * v[i];
+      const double gzk = gz + sk * gn;
+      for (int i = 0; i < 3; i++) {
+        const double xg = X[c][i] + mx[3 * i] * v[0] + mx[3 * i + 1] * v[1] + mx[3 * i + 2] * v[2];
+        X[c][i] = xg + gzk * xh[i] + gds * a[i];
+        Ev[c][i] = hds * xg + n * v[i] + gds * b[i];
+        V[c][i] = v[i];
+      }
     }
-    // reverse of the step, per channel; the refractivities' adjoints (of the
-    // four probes and the point) go to the levels after the channel loop
-    double gNq[4][CH], gnk[CH];
+    if (lb >= 0) {
+      if (lb != wb) {
+        if (wb >= 0 && lb == wb + 1) {
+          flush_level(0);
 #pragma unroll
-    for (int c = 0; c < CH; c++) {
-      double gx[3], ge[3], ge1[3], gds = 0;
-      for (int i = 0; i < 3; i++) {
-        gx[i] = X[c][i];
-        ge[i] = 0.5 * ds * X[c][i];
-        ge1[i] = Ev[c][i] + 0.5 * ds * X[c][i];
-        gds += 0.5 * X[c][i] * (e[i] + e1[i]);
+          for (int c = 0; c < CH; c++) {
+            Tw[c][0] = Tw[c][1];
+            Tw[c][1] = Tw[c][2];
+            Tw[c][2] = 0;
+            Pw[c][0] = Pw[c][1];
+            Pw[c][1] = Pw[c][2];
+            Pw[c][2] = 0;
+          }
+        } else if (wb >= 0 && lb == wb - 1) {
+          flush_level(2);
+#pragma unroll
+          for (int c = 0; c < CH; c++) {
+            Tw[c][2] = Tw[c][1];
+            Tw[c][1] = Tw[c][0];
+            Tw[c][0] = 0;
+            Pw[c][2] = Pw[c][1];
+            Pw[c][1] = Pw[c][0];
+            Pw[c][0] = 0;
+          }
+        } else {
+          if (wb >= 0)
+            for (int j = 0; j < 3; j++) flush_level(j);
+#pragma unroll
+          for (int c = 0; c < CH; c++)
+            for (int j = 0; j < 3; j++) Tw[c][j] = Pw[c][j] = 0;
+        }
+        wb = lb;
       }
-      const double pe = e1[0] * ge1[0] + e1[1] * ge1[1] + e1[2] * ge1[2];
-      double V[3];
-      for (int i = 0; i < 3; i++) V[i] = (ge1[i] - e1[i] * pe) / nv;
-      const double gn = e[0] * V[0] + e[1] * V[1] + e[2] * V[2];
-      for (int i = 0; i < 3; i++) {
-        ge[i] += n * V[i];
-        gds += ng[i] * V[i];
-      }
-      if (rf) {
-        double gN[4];
-        gN[0] = 0;
+      for (int j = 0; j < 3; j++) {
+        double tj[3], pj[3];
         for (int i = 0; i < 3; i++) {
-          gN[i + 1] = ds * V[i] / HP;
-          gN[0] -= ds * V[i] / HP;
+          tj[i] = PCR(PC_TAU + 3 * j + i);
+          pj[i] = PCR(PC_PI + 3 * j + i);
         }
-        double gxh[3] = {0, 0, 0};
-        for (int q = 0; q < 4; q++) {
-          const double dNdp = CR / pq[q].t, dNdt = -CR * pq[q].p / (pq[q].t * pq[q].t);
-          gNq[q][c] = gN[q];
-          const double gzq = gN[q] * (dNdp * pq[q].dp + dNdt * pq[q].dt);
-          const double rq = sqrt(xq[q][0] * xq[q][0] + xq[q][1] * xq[q][1] + xq[q][2] * xq[q][2]);
-          for (int i = 0; i < 3; i++) gxh[i] += gzq * xq[q][i] / rq;
-        }
-        for (int i = 0; i < 3; i++) {
-          gx[i] += gxh[i];
-          ge[i] += 0.5 * ds * gxh[i];
-          gds += 0.5 * e[i] * gxh[i];
-        }
-        const double dndp = CR / pk.t, dndt = -CR * pk.p / (pk.t * pk.t);
-        gnk[c] = gn;
-        gzk[c] += gn * (dndp * pk.dp + dndt * pk.dt);
-      }
-      for (int i = 0; i < 3; i++) gx[i] += gzk[c] * x[i] / r;
-      gds += gsk[c];
-      if (k == N - 2)
-        for (int i = 0; i < 3; i++) gx[i] += Xp[c][i] + gzp[c] * x[i] / r;
-      ds_grad(x, e, gds, gx, ge);
-      for (int i = 0; i < 3; i++) {
-        X[c][i] = gx[i];
-        Ev[c][i] = ge[i];
-      }
-    }
-    if (rf) {
-      double gt[CH], gp[CH];
-      for (int q = -1; q < 4; q++) {
-        const PtAt &P = q < 0 ? pk : pq[q];
-        const double dNdp = CR / P.t, dNdt = -CR * P.p / (P.t * P.t);
 #pragma unroll
         for (int c = 0; c < CH; c++) {
-          const double g = q < 0 ? gnk[c] : gNq[q][c];
-          gt[c] = g * dNdt;
-          gp[c] = g * dNdp;
+          Tw[c][j] += tj[0] * V[c][0] + tj[1] * V[c][1] + tj[2] * V[c][2];
+          Pw[c][j] += pj[0] * V[c][0] + pj[1] * V[c][1] + pj[2] * V[c][2];
         }
-        W.cover(P.lo, nlev, xm, kt0, ktstride, nc);
-        W.add(P.lo, P.w, P.fp0, P.fp1, gt, gp);
+      }
+    } else if (lb == -2) {
+      // points over more than three levels: straight to the tile
+      const unsigned long long los5 = (unsigned long long)PCR(PC_LO);
+      const double f = 2 * hds / PATH_HP;
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        const int lo = (int)(los5 >> (8 * q) & 255u);
+        const double w0 = PCR(PC_W + 4 * q), w1 = PCR(PC_W + 4 * q + 1);
+        const double w2 = PCR(PC_W + 4 * q + 2), w3 = PCR(PC_W + 4 * q + 3);
+        const int t0 = xm[JUR_XMAP_LEV(1, lo)], t1 = xm[JUR_XMAP_LEV(1, lo + 1)];
+        const int p0 = xm[JUR_XMAP_LEV(0, lo)], p1 = xm[JUR_XMAP_LEV(0, lo + 1)];
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+          if (c >= nc) continue;
+          const double g = q == 0 ? e[0] * V[c][0] + e[1] * V[c][1] + e[2] * V[c][2]
+                                  : (q == 1 ? -f * (V[c][0] + V[c][1] + V[c][2]) : f * V[c][q >= 2 ? q - 2 : 0]);
+          double *kt = kt0 + (size_t)c * ktstride;
+          if (t0 >= 0) kt[(size_t)t0 * WAVE] += g * w0;
+          if (t1 >= 0) kt[(size_t)t1 * WAVE] += g * w1;
+          if (p0 >= 0) kt[(size_t)p0 * WAVE] += g * w2;
+          if (p1 >= 0) kt[(size_t)p1 * WAVE] += g * w3;
+        }
       }
     }
+    return o;
+#undef PCR
+  };
+  // step N-2 leads to the cut point; the clip's dependence on x_N-2 joins
+  // its adjoint afterwards
+  {
+    const double *o = step(N - 2);
+    double xh[3];
+    for (int i = 0; i < 3; i++) xh[i] = o[(size_t)(PC_XH + i) * WAVE];
+#pragma unroll
+    for (int c = 0; c < CH; c++)
+      for (int i = 0; i < 3; i++) X[c][i] += Xp[c][i] + gzp[c] * xh[i];
   }
-  W.flush(xm, kt0, ktstride, nc);
+  for (int k = N - 3; k >= 0; k--) step(k);
+  if (wb >= 0)
+    for (int j = 0; j < 3; j++) flush_level(j);
 #undef GZ
 #undef GD
 }
@@ -1462,20 +1783,35 @@ __global__ __launch_bounds__(WAVE) void k_jac_store(const double *kt, int nd, in
   }
 }
 
-constexpr int PATH_CH = 2;  // channels per lane of k_adj_path
+constexpr int PATH_CH = 4;  // channels per lane of k_adj_path
 
 template <bool CGA, typename SF, bool PATH>
 static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
-                       const RayInfo *info, const void *tail, int blk0, int nblk, const JacView &J, double *rad,
-                       double *tau, hipStream_t s) {
+                       const RayInfo *info, const void *tail, int blk0, int nblk, int maxnp, const JacView &J,
+                       double *rad, double *tau, hipStream_t s) {
   const int nb8 = padded_blocks(nblk);
+  if (PATH) {
+    // reverse-step coefficients of steps 0 .. np - 2
+    const int nchunk = (maxnp - 1 + PCO_STEPS - 1) / PCO_STEPS;
+    if (nchunk > 0)
+      hipLaunchKernelGGL(k_path_coef, dim3(nb8 * nchunk), dim3(WAVE), 0, s, M, B, L, info, blk0, nblk, nchunk, J);
+  }
   hipLaunchKernelGGL((k_adj_acc<SF, PATH>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0, nblk,
                      J, rad, tau);
   if (M.ng > 0) {
     const dim3 grid(nb8 * M.nd * M.ng), blk(WAVE);
     bool fast = false;
     if constexpr (!CGA && sizeof(SF) == sizeof(float)) {
-      if (E.part) {
+      if (E.part && M.ng <= 4) {
+        const dim3 grow(nb8 * M.nd);
+        if (M.ng <= 2)
+          hipLaunchKernelGGL((k_adj_chains_f32<2, PATH>), grow, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
+        else if (M.ng == 3)
+          hipLaunchKernelGGL((k_adj_chains_f32<3, PATH>), grow, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
+        else
+          hipLaunchKernelGGL((k_adj_chains_f32<4, PATH>), grow, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
+        fast = true;
+      } else if (E.part) {
         hipLaunchKernelGGL((k_adj_chain<false, float, true, PATH>), grid, blk, 0, s, M, B, L, E, info, blk0,
                            nblk, J);
         fast = true;
@@ -1492,13 +1828,13 @@ static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L,
 }
 
 void launch_adjoint(const ModelView &M, const BatchView &B, const LosView &L, const EpsView &E,
-                    const RayInfo *info, const void *tail, int blk0, int nblk, const JacView &J, bool fp32,
-                    double *rad, double *tau, hipStream_t s) {
+                    const RayInfo *info, const void *tail, int blk0, int nblk, int maxnp, const JacView &J,
+                    bool fp32, double *rad, double *tau, hipStream_t s) {
   if (M.nd == 0) return;
   const bool path = J.kz >= 0;
-#define JUR_ADJ(CGA, SF)                                                                  \
-  (path ? launch_adj<CGA, SF, true>(M, B, L, E, info, tail, blk0, nblk, J, rad, tau, s)  \
-        : launch_adj<CGA, SF, false>(M, B, L, E, info, tail, blk0, nblk, J, rad, tau, s))
+#define JUR_ADJ(CGA, SF)                                                                         \
+  (path ? launch_adj<CGA, SF, true>(M, B, L, E, info, tail, blk0, nblk, maxnp, J, rad, tau, s)  \
+        : launch_adj<CGA, SF, false>(M, B, L, E, info, tail, blk0, nblk, maxnp, J, rad, tau, s))
   if (M.formod == 0) {
     if (fp32) JUR_ADJ(true, float);
     else JUR_ADJ(true, double);

@@ -176,7 +176,7 @@ struct jur_ctx {
   jur::DevBuf<double> d_rad0, d_tau0;
   jur::DevBuf<int> d_err;
   // Jacobian work buffers
-  jur::DevBuf<double> d_kt, d_scr;
+  jur::DevBuf<double> d_kt, d_scr, d_pco;
   jur::DevBuf<size_t> d_soff;
   jur::DevBuf<int> d_xmap, d_nef;
   size_t los_budget = 0;  // bytes for LOS + emissivity buffers (0: from free HBM)

@@ -117,7 +117,13 @@ struct JacView {
   int *nef;             // [blk][nd][WAVE] points before the first opaque segment
   int kz;               // scratch index of d rad / d z_i (d rad / d ds_i at
                         // kz + 1): the refracted path is differentiated; -1: not
+  double *pco;          // kz >= 0: per march step the JUR_NPCO coefficients of
+                        // the reverse step, block b at soff[b] / (nd nscr) * JUR_NPCO,
+                        // [step][coefficient][lane]
 };
+
+// Coefficients per march step of the adjoint of the refracted path.
+constexpr int JUR_NPCO = 53;
 
 #define LOS_AT(base, L, f, ip) (base)[((size_t)(ip) * (L).nfield + (size_t)(f)) * WAVE]
 

@@ -37,7 +37,7 @@ void launch_hydrostatic(atm_t *, int, double, int, hipStream_t);
 void launch_lookup_f64(const ModelView &, const BatchView &, const LosView &, const EpsView &,
                        const RayInfo *, int, int, bool, hipStream_t);
 void launch_adjoint(const ModelView &, const BatchView &, const LosView &, const EpsView &,
-                    const RayInfo *, const void *, int, int, const JacView &, bool, double *, double *,
+                    const RayInfo *, const void *, int, int, int, const JacView &, bool, double *, double *,
                     hipStream_t);
 void launch_jac_store(const double *, int, int, int, int, int, const double *, const ChanInfo *, int,
                       const unsigned char *, double *, hipStream_t);
@@ -687,7 +687,8 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
     const bool losf = precision == JUR_FP32 && !jac;
     const size_t los_blk = (size_t)L.nfield * L.cap * WAVE * (sizeof(double) + (losf ? sizeof(float) : 0));
     const size_t pt_blk = (size_t)s.nd * s.ng * WAVE * (esz + (part ? sizeof(float4) : 0)) +
-                          (jac ? (size_t)s.nd * nscr * WAVE * sizeof(double) : 0);
+                          (jac ? (size_t)s.nd * nscr * WAVE * sizeof(double) : 0) +
+                          (dpath ? (size_t)JUR_NPCO * WAVE * sizeof(double) : 0);
     const size_t tile_blk = jac ? (size_t)s.nd * jac->ncol * WAVE * sizeof(double) : 0;
     size_t budget = c->los_budget;
     if (pm.base) {
@@ -701,7 +702,7 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
       fr += c->d_los.n * sizeof(double) + c->d_losf.n * sizeof(float) + c->d_eps.n + c->d_part.n * sizeof(float4) +
-            (c->d_kt.n + c->d_scr.n) * sizeof(double);
+            (c->d_kt.n + c->d_scr.n + c->d_pco.n) * sizeof(double);
       budget = std::min(fr / 2, (size_t)96 << 30);
     }
     // Rays are traced in chunks whose LOS takes at most half the budget; the
@@ -746,7 +747,7 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
         size_t tot = 0, stot = 0, mem = 0;
         while (s1 < nb) {
           // in scratch mode each slice is rounded up to 256 bytes
-          const size_t need = (size_t)npb[s1] * pt_blk + tile_blk + (pm.base ? 5 * 256 : 0);
+          const size_t need = (size_t)npb[s1] * pt_blk + tile_blk + (pm.base ? 6 * 256 : 0);
           if (s1 > s0 && mem + need > rest) break;
           eoff[s1 - s0] = tot;
           tot += (size_t)s.nd * s.ng * npb[s1] * WAVE;
@@ -773,16 +774,18 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[2], st));
         if (jac) {
-          double *d_scr, *d_kt;
+          double *d_scr, *d_kt, *d_pco = nullptr;
           int *d_nef;
           if ((rc = pm.get(c->d_scr, stot + 1, &d_scr))) return rc;
+          if (dpath && (rc = pm.get(c->d_pco, stot / ((size_t)s.nd * nscr) * JUR_NPCO + 1, &d_pco))) return rc;
           if ((rc = pm.get(c->d_kt, (size_t)ns * s.nd * jac->ncol * WAVE + 1, &d_kt))) return rc;
           if ((rc = pm.get(c->d_nef, (size_t)ns * s.nd * WAVE + 1, &d_nef))) return rc;
           HIPCHK(hipMemcpyAsync(d_soff, soff.data(), ns * sizeof(size_t), hipMemcpyHostToDevice, st));
           launch_zero(d_kt, (size_t)ns * s.nd * jac->ncol * WAVE, st);
           JacView J{jac->xmap, jac->nmap, jac->prof_map, jac->ncol, d_kt, d_scr, d_soff, nscr, d_nef,
-                    dpath ? nscr0 : -1};
-          launch_adjoint(M, B, Ls, E, d_info, d_tail, bs, ns, J, precision == JUR_FP32, d_rad, d_tau, st);
+                    dpath ? nscr0 : -1, d_pco};
+          const int maxnp = *std::max_element(npb.begin() + s0, npb.begin() + s1);
+          launch_adjoint(M, B, Ls, E, d_info, d_tail, bs, ns, maxnp, J, precision == JUR_FP32, d_rad, d_tau, st);
           HIPCHK(hipGetLastError());
           launch_jac_store(d_kt, s.nd, jac->ncol, bs, ns, B.nray, d_rad, c->d_chan.p, s.write_bbt, d_mask,
                            jac->k, st);
