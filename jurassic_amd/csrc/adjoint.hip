@@ -1520,8 +1520,10 @@ __global__ __launch_bounds__(WAVE) void k_path_coef(ModelView M, BatchView B, Lo
 
 // The reverse sweep over the steps, CH channels per lane (one ray).  Level
 // sums are kept for a window of three levels (the step's LB and the two
-// above) per channel and added to the K tile when the window moves.
-template <int CH>
+// above) per channel and added to the K tile when the window moves.  With
+// PF the inputs of the next step are loaded while a step is computed (for
+// small batches, where a wavefront's sweep is the kernel's duration).
+template <int CH, bool PF>
 __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, LosView L, EpsView E,
                                                    const RayInfo *info, const RayTail *tail, int blk0, int nblk,
                                                    JacView J) {
@@ -1617,11 +1619,26 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
       if (cp >= 0) kt[(size_t)cp * WAVE] += Pw[c][j];
     }
   };
-  auto step = [&](int k) {
+  // a step's inputs: its coefficients and the channels' point values
+  struct StepIn {
+    double c[JUR_NPCO];
+    double gz[CH], gd[CH];
+  };
+  auto load_in = [&](int k, StepIn &in) {
     const double *o = pc + (size_t)k * JUR_NPCO * WAVE;
-#define PCR(f) o[(size_t)(f) * WAVE]
+#pragma unroll
+    for (int f = 0; f < JUR_NPCO; f++) in.c[f] = o[(size_t)f * WAVE];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      in.gz[c] = c < nc ? GZ(c, k) : 0.0;
+      in.gd[c] = c < nc ? GD(c, k) : 0.0;
+    }
+  };
+  auto step = [&](int k, const StepIn &in) {
+#define PCR(f) in.c[f]
     const double hds = PCR(PC_HDS), rnv = PCR(PC_RNV), n = PCR(PC_N), sk = PCR(PC_SK);
     double e[3], e1[3], ngm[3], xh[3], a[3], b[3], mx[9];
+#pragma unroll
     for (int i = 0; i < 3; i++) {
       e[i] = PCR(PC_E + i);
       e1[i] = PCR(PC_E1 + i);
@@ -1630,13 +1647,13 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
       a[i] = PCR(PC_A + i);
       b[i] = PCR(PC_B + i);
     }
+#pragma unroll
     for (int i = 0; i < 9; i++) mx[i] = PCR(PC_MX + i);
     const int lb = (int)PCR(PC_LB);
     double V[CH][3];
 #pragma unroll
     for (int c = 0; c < CH; c++) {
-      const double gz = c < nc ? GZ(c, k) : 0.0;
-      const double gd = c < nc ? GD(c, k) : 0.0;
+      const double gz = in.gz[c], gd = in.gd[c];
       const double gs = (k == N - 2) ? 0.0 : 0.5 * (gd + gdn[c]);  // point N-2's raw ds was clipped
       gdn[c] = gd;
       double G[3];
@@ -1724,20 +1741,26 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
         }
       }
     }
-    return o;
 #undef PCR
   };
   // step N-2 leads to the cut point; the clip's dependence on x_N-2 joins
   // its adjoint afterwards
-  {
-    const double *o = step(N - 2);
-    double xh[3];
-    for (int i = 0; i < 3; i++) xh[i] = o[(size_t)(PC_XH + i) * WAVE];
+  StepIn cur, nxt;
+  load_in(N - 2, cur);
+  if (PF && N >= 3) load_in(N - 3, nxt);
+  step(N - 2, cur);
 #pragma unroll
-    for (int c = 0; c < CH; c++)
-      for (int i = 0; i < 3; i++) X[c][i] += Xp[c][i] + gzp[c] * xh[i];
+  for (int c = 0; c < CH; c++)
+    for (int i = 0; i < 3; i++) X[c][i] += Xp[c][i] + gzp[c] * cur.c[PC_XH + i];
+  for (int k = N - 3; k >= 0; k--) {
+    if (PF) {
+      cur = nxt;
+      if (k > 0) load_in(k - 1, nxt);
+    } else {
+      load_in(k, cur);
+    }
+    step(k, cur);
   }
-  for (int k = N - 3; k >= 0; k--) step(k);
   if (wb >= 0)
     for (int j = 0; j < 3; j++) flush_level(j);
 #undef GZ
@@ -1821,9 +1844,16 @@ static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L,
       hipLaunchKernelGGL((k_adj_chain<CGA, SF, false, PATH>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk, J);
   }
   if (PATH) {
+    // large batches: PATH_CH channels per lane share each step's
+    // coefficients; small ones (under two wavefronts per SIMD that way):
+    // one channel per lane, the next step's inputs loaded ahead
     const int ngrp = (M.nd + PATH_CH - 1) / PATH_CH;
-    hipLaunchKernelGGL((k_adj_path<PATH_CH>), dim3(nb8 * ngrp), dim3(WAVE), 0, s, M, B, L, E, info,
-                       (const RayTail *)tail, blk0, nblk, J);
+    if ((size_t)nb8 * ngrp >= 2048)
+      hipLaunchKernelGGL((k_adj_path<PATH_CH, false>), dim3(nb8 * ngrp), dim3(WAVE), 0, s, M, B, L, E, info,
+                         (const RayTail *)tail, blk0, nblk, J);
+    else
+      hipLaunchKernelGGL((k_adj_path<1, true>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info,
+                         (const RayTail *)tail, blk0, nblk, J);
   }
 }
 

@@ -336,3 +336,57 @@ def test_device_jacobian_matches_host(gpu, oracle):
     for p in range(2):
         rows = kh[p::2].reshape(nr * nd, n)
         assert np.allclose(rows, ks[p], rtol=1e-12, atol=1e-12 * np.abs(ks[p]).max())
+
+
+def _nadir_batch(ctl, nprof, nang, torch):
+    """nprof atmospheres x nang nadir views, ray order view major."""
+    atms = (abi.atm_t * nprof)()
+    for k in range(nprof):
+        atms[k] = scenes.make_profile_atm(["CO2", "H2O", "O3"], iprof=k)
+    geom = scenes.nadir_geometry(705.0, np.arange(nang) * 50.0 / 7.0)
+    nray = nprof * nang
+    prof = np.tile(np.arange(nprof, dtype=np.int32), nang)
+    g = np.zeros((nray, 7))
+    for j in range(nang):
+        for c, key in enumerate(("obsz", "obslon", "obslat", "vpz", "vplon", "vplat")):
+            g[j * nprof:(j + 1) * nprof, 1 + c] = geom[key][j]
+    dev = torch.device("cuda:0")
+    a_d = torch.from_numpy(np.frombuffer(bytes(atms), dtype=np.uint8).copy()).to(dev)
+    return atms, a_d, torch.from_numpy(prof).to(dev), torch.from_numpy(g).to(dev), nray
+
+
+def test_adjoint_path_batch_regimes_agree(gpu):
+    """The adjoint of the refracted path runs two ways: four channels per
+    lane sharing each step's coefficients for large batches, one channel per
+    lane with the next step's inputs loaded ahead for small ones.  The K rows
+    of the same rays agree between a batch of 688 atmospheres x 8 views with
+    96 channels (the first way) and one of its first 64 atmospheres (the
+    second); each lane's arithmetic is its own, so only the T and p columns'
+    order of additions may differ."""
+    torch = pytest.importorskip("torch")
+    ctl = scenes.nadir_jac_ctl(96)
+    assert ctl.refrac == 1
+    nang = 8
+    out = []
+    for nprof in (688, 64):
+        atms, a_d, prof_d, geom_d, nray = _nadir_batch(ctl, nprof, nang, torch)
+        xm, n = gpu.state_map(ctl, atms[0])
+        ctx = gpu.Context(ctl)
+        try:
+            rad = torch.zeros(nray, ctl.nd, dtype=torch.float64, device="cuda:0")
+            tau = torch.zeros_like(rad)
+            kd = torch.zeros(nray, ctl.nd, n, dtype=torch.float64, device="cuda:0")
+            pmap = torch.zeros(nprof, dtype=torch.int32, device="cuda:0")
+            ctx.jacobian_device(nprof, a_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
+                                rad.data_ptr(), tau.data_ptr(), xm[None, :], pmap.data_ptr(), n,
+                                kd.data_ptr(), precision=abi.JUR_FP32)
+            torch.cuda.synchronize()
+            rows = torch.cat([kd[j * nprof:j * nprof + 64] for j in range(nang)])
+            out.append((rad.view(nang, nprof, -1)[:, :64].cpu().numpy(), rows.cpu().numpy()))
+            del kd
+        finally:
+            ctx.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    k0, k1 = out[0][1], out[1][1]
+    assert np.isfinite(k0).all() and np.abs(k0).max() > 0
+    np.testing.assert_allclose(k1, k0, rtol=1e-12, atol=1e-12 * np.abs(k0).max())
