@@ -547,10 +547,10 @@ def wide_bench(args, torch, gpu, abi):
         log("wide %s context: %.1f s (tables packed in HBM, source function for %d channels)"
             % (name, t_ctx, nd))
 
-        def fwd():
-            ctx.formod_device(nprof, atm_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
+        def fwd(nr=nray):
+            ctx.formod_device(nprof, atm_d.data_ptr(), nr, prof_d.data_ptr(), geom_d.data_ptr(),
                               rad_d.data_ptr(), tau_d.data_ptr(), precision=abi.JUR_FP32)
-        fwd()
+        fwd(min(nray, 4096))  # warm-up on the first rays
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         fwd()
@@ -570,23 +570,33 @@ def wide_bench(args, torch, gpu, abi):
                 ctl.retq_zmin[ig], ctl.retq_zmax[ig] = -100, 100
             xm, n = gpu.state_map(ctl, atms[0])
             pmap_d = torch.zeros(nprof, dtype=torch.int32, device=dev)
-            k_d = torch.empty(nray * nd * n, dtype=torch.float64, device=dev)
+            # K of all rays exceeds HBM at full size (131072 x 2048 x 243 x
+            # 8 B = 522 GB): the rays go through in chunks of 4096, each
+            # chunk's K written to one buffer (the caller would stream it out)
+            chunk = min(nray, 4096)
+            k_d = torch.empty(chunk * nd * n, dtype=torch.float64, device=dev)
 
-            def adj():
-                ctx.jacobian_device(nprof, atm_d.data_ptr(), nray, prof_d.data_ptr(), geom_d.data_ptr(),
-                                    rad_d.data_ptr(), tau_d.data_ptr(), xm[None, :], pmap_d.data_ptr(), n,
+            def adj(r0, nr):
+                ctx.jacobian_device(nprof, atm_d.data_ptr(), nr, prof_d.data_ptr() + 4 * r0,
+                                    geom_d.data_ptr() + 56 * r0, rad_d.data_ptr() + 8 * nd * r0,
+                                    tau_d.data_ptr() + 8 * nd * r0, xm[None, :], pmap_d.data_ptr(), n,
                                     k_d.data_ptr(), precision=abi.JUR_FP32)
-            adj()
+            adj(0, chunk)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            adj()
+            kms = [0.0, 0.0, 0.0]
+            finite = True
+            for r0 in range(0, nray, chunk):
+                adj(r0, min(chunk, nray - r0))
+                kms = [a + b for a, b in zip(kms, ctx.last_timing())]
+                if r0 == 0:
+                    finite = bool(torch.isfinite(k_d).all().item())
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
-            kms = ctx.last_timing()
-            r["jacobian"] = dict(jacobians_per_s=nprof / dt, ms_per_pass=dt * 1e3, n=n, m=nd,
+            r["jacobian"] = dict(jacobians_per_s=nprof / dt, ms_per_pass=dt * 1e3, n=n, m=nd, chunk_rays=chunk,
                                  kernel_ms={"raytrace": kms[0], "lookup": kms[1],
                                             "reverse_sweep_and_store": kms[2]},
-                                 finite=bool(torch.isfinite(k_d).all().item()))
+                                 finite=finite)
             log("wide adjoint FP32 (T, H2O, O3; n = %d): %.1f ms per %d Jacobians (%.1f Jacobians/s)"
                 % (n, dt * 1e3, nprof, nprof / dt))
             del k_d

@@ -180,6 +180,8 @@ struct jur_ctx {
   jur::DevBuf<size_t> d_soff;
   jur::DevBuf<int> d_xmap, d_nef;
   size_t los_budget = 0;  // bytes for LOS + emissivity buffers (0: from free HBM)
+  int pass_kind = -1;     // precision, Jacobian and path flags of the last pass
+                          // that sized the work buffers from free HBM
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
   double last_mean_np = 0;

@@ -698,7 +698,21 @@ int forward_pass(jur_ctx *c, const BatchView &B, int precision, const unsigned c
       budget = pm.cap - pm.used > small ? pm.cap - pm.used - small : 0;
     } else if (budget == 0) {
       // half of the HBM free now (plus what these buffers already hold), at
-      // most 96 GiB
+      // most 96 GiB.  A pass of another kind divides the budget otherwise
+      // (LOS fields, emissivities, partials, scratch): its buffers are
+      // released first, so that capacities kept from the last kind do not
+      // hold memory the new division counts as free.
+      const int kind = precision * 4 + (jac ? 2 : 0) + (dpath ? 1 : 0);
+      if (kind != c->pass_kind) {
+        c->d_los.release();
+        c->d_losf.release();
+        c->d_eps.release();
+        c->d_part.release();
+        c->d_kt.release();
+        c->d_scr.release();
+        c->d_pco.release();
+        c->pass_kind = kind;
+      }
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
       fr += c->d_los.n * sizeof(double) + c->d_losf.n * sizeof(float) + c->d_eps.n + c->d_part.n * sizeof(float4) +
