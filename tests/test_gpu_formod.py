@@ -358,3 +358,56 @@ def test_formod_gpu_and_kernel_gpu_dropins(gpu, oracle):
     # BT differences over a 1 K step: entries agree to about 1e-10 BT / 1 K
     assert np.abs(k1 - k0).max() <= 2e-7
     assert np.abs(k0).max() > 0.01
+
+
+def test_fp32_bt_bound_limb_at_scale(gpu, oracle):
+    """The 0.01 K bound over a large batch of the limb workload: 512
+    atmospheres (seeded perturbed profiles) x 32 tangent heights 5..67 km x
+    12 channels, CO2/H2O/O3, continua and refraction on (196,608
+    radiances).  The FP64 device pass is the reference arithmetic (checked
+    against the oracle to 1e-10 on the first two atmospheres), the FP32 pass
+    must stay within 0.01 K of it everywhere."""
+    torch = pytest.importorskip("torch")
+    ctl = scenes.limb_3gas_ctl()
+    zt = scenes.limb_tangent_heights()
+    geom = scenes.limb_geometry(800.0, zt)
+    nprof, nr = 512, len(zt)
+    gases = [ctl.emitter[i].value.decode() for i in range(ctl.ng)]
+    atms = (abi.atm_t * nprof)()
+    for k in range(nprof):
+        atms[k] = scenes.make_profile_atm(gases, iprof=k)
+    nray = nprof * nr
+    prof = np.tile(np.arange(nprof, dtype=np.int32), nr)
+    g = np.zeros((nray, 7))
+    for j in range(nr):
+        for c, key in enumerate(("obsz", "obslon", "obslat", "vpz", "vplon", "vplat")):
+            g[j * nprof:(j + 1) * nprof, 1 + c] = geom[key][j]
+    dev = torch.device("cuda:0")
+    a_d = torch.from_numpy(np.frombuffer(bytes(atms), dtype=np.uint8).copy()).to(dev)
+    p_d, g_d = torch.from_numpy(prof).to(dev), torch.from_numpy(g).to(dev)
+    ctx = gpu.Context(ctl)
+    out = {}
+    try:
+        for prec in (abi.JUR_FP64, abi.JUR_FP32):
+            rad = torch.zeros(nray, ctl.nd, dtype=torch.float64, device=dev)
+            tau = torch.zeros_like(rad)
+            ctx.formod_device(nprof, a_d.data_ptr(), nray, p_d.data_ptr(), g_d.data_ptr(), rad.data_ptr(),
+                              tau.data_ptr(), precision=prec)
+            torch.cuda.synchronize()
+            out[prec] = rad.cpu().numpy().reshape(nr, nprof, ctl.nd)
+    finally:
+        ctx.close()
+    for k in range(2):
+        obs = scenes.make_obs(geom)
+        oracle.formod(ctl, scenes.make_profile_atm(gases, iprof=k), obs)
+        r0, _ = abi.obs_arrays(obs, ctl)
+        _cmp(out[abi.JUR_FP64][:, k, :], r0.T, RTOL64, "FP64 device pass vs oracle, atmosphere %d" % k)
+    nu = np.array([ctl.nu[i] for i in range(ctl.nd)])
+    c1, c2 = 1.19104259e-8, 1.43877506
+
+    def bt(r):
+        return c2 * nu / np.log1p(c1 * nu ** 3 / r)
+    err = np.abs(bt(out[abi.JUR_FP32]) - bt(out[abi.JUR_FP64]))
+    assert np.isfinite(err).all()
+    print("FP32 vs FP64 over %d radiances: max %.2e K, mean %.2e K" % (err.size, err.max(), err.mean()))
+    assert err.max() <= BT_TOL32
