@@ -314,8 +314,8 @@ def jacobian_bench(args, torch, gpu, abi):
         s_ = slice(b * 81, (b + 1) * 81)
         blocks[name] = float(np.linalg.norm(k32[:, s_] - k64[:, s_]) / np.linalg.norm(k64[:, s_]))
     log("FP32 vs FP64 adjoint on %d atmospheres: BT %.2e K, K rel L2 %s" % (nsub, bt_err, blocks))
-    # the adjoint (path held fixed) against kernel()'s one-sided differences
-    # (which also move the refracted path) on the FD atmospheres
+    # the adjoint (refracted path included) against kernel()'s one-sided
+    # differences on the FD atmospheres
     ncmp = min(nfd, nsub)
     kf = k_fd[:ncmp].cpu().numpy()  # [atm][angle][channel][column]
     fd_vs = {}
@@ -333,15 +333,17 @@ def jacobian_bench(args, torch, gpu, abi):
     # HBM bytes a Jacobian pass must move at least, per Jacobian: per (ray,
     # LOS point, channel) and emitter the lookup's stored partials (16 B) and
     # path emissivity (4 B), each written once and read once by the reverse
-    # chains, and the accumulation's per-point value H_i (8 B, written and
-    # read); per (ray, point) the LOS fields read by the lookups and the
-    # accumulation (10 FP64 fields, twice); the FP64 K written once
-    bytes_per_jac = (m * mean_np * (len(GASES) * 2 * (16 + 4) + 2 * 8) + NJAC_ANG * mean_np * 2 * 10 * 8
-                     + m * n * 8)
+    # chains; per (ray, point, channel) the accumulation's H_i (8 B) and the
+    # path's point values d rad/d z, d rad/d ds (16 B), each written and
+    # read once; per (ray, point) the LOS fields (16 FP64 fields with the
+    # march's positions and directions, read twice) and the reverse march's
+    # 53 FP64 coefficients (written and read once); the FP64 K written once
+    bytes_per_jac = (m * mean_np * (len(GASES) * 2 * (16 + 4) + 2 * 8 + 2 * 16)
+                     + NJAC_ANG * mean_np * (2 * 16 * 8 + 2 * 53 * 8) + m * n * 8)
     out = {
         "value": jac_rate, "unit": "Jacobians/s", "ms_per_step": dt * 1e3, "steps": steps,
         "workload": "nadir sounder: %d atmospheres x %d view angles x %d channels, n = %d, m = %d "
-                    "(CO2, H2O, O3; EGA; continua and refraction on; LOS held fixed)"
+                    "(CO2, H2O, O3; EGA; continua and refraction on; the refracted path differentiated)"
                     % (nprof, NJAC_ANG, nd, n, m),
         "precision": "FP32 lookups, FP64 geometry and adjoint accumulation",
         "mean_los_points": mean_np,
@@ -363,7 +365,8 @@ def jacobian_bench(args, torch, gpu, abi):
                      "unit": "GB/s", "frac": bytes_per_jac * jac_rate / HBM_PEAK,
                      "bytes_per_jacobian": bytes_per_jac,
                      "model": "unique bytes: partials and path emissivities written and read once, "
-                              "per-point accumulation values, LOS fields, K"},
+                              "per-point accumulation and path values, LOS fields, reverse-march "
+                              "coefficients, K"},
     }
     if not args.no_cpu:
         ns_cpu = args.jac_cpu_sample
