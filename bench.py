@@ -91,6 +91,24 @@ def build_workload(nprof):
     return ctl, atms, prof, g
 
 
+def host_cpu():
+    """The host's CPU model and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": avail, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(nsample):
     """FP64 oracle on this host's cores: OpenMP across atmospheres after one
     warm call (the kernel() pattern of the reference), on nsample profiles of
@@ -114,7 +132,7 @@ def cpu_baseline(nsample):
     nrad = nsample * len(zt) * ctl.nd
     return dict(value=nrad / dt, unit="radiances/s", cores=threads, kind="port",
                 sample="%d profiles x %d tangent heights x %d channels (%d radiances), FP64 oracle, %.1f s"
-                % (nsample, len(zt), ctl.nd, nrad, dt))
+                % (nsample, len(zt), ctl.nd, nrad, dt), host=host_cpu())
 
 
 def lookup_roofline(nprof, nray, nd, mean_np, kms):

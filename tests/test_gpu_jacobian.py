@@ -236,6 +236,28 @@ def test_adjoint_nadir_refraction_jacobian_workload(gpu, oracle):
     _check(ec, 1e-3, "nadir refraction FP32 vs central differences")
 
 
+def test_adjoint_refraction_coarse_steps(gpu, oracle):
+    """Steps of RAYDS = 10 km with no RAYDZ limit over levels 0.125 km apart
+    between 12 and 36 km (249 levels): half a step climbs 0.3-0.45 km there,
+    so a step's point and its four refractivity probes often span more than
+    three levels, where the path adjoint adds their terms point by point
+    instead of through its three-level window (and the window jumps).  T
+    and p in the state, refraction on, FP64."""
+    z = np.concatenate([np.arange(0.0, 12.0, 1.0), np.arange(12.0, 36.0, 0.125), np.arange(36.0, 81.0, 1.0)])
+    assert len(z) <= abi.NP
+    ctl, _, obs = scenes.limb_3gas(2, nrays=8)
+    atm = scenes.make_profile_atm(["CO2", "H2O", "O3"], iprof=2, z=z)
+    ctl.refrac = 1
+    ctl.raydz = 0.0
+    ctl.rett_zmin, ctl.rett_zmax = -100, 100
+    ctl.retp_zmin, ctl.retp_zmax = -100, 100
+    k1, _ = _adjoint(gpu, oracle, ctl, atm, obs)
+    assert np.isfinite(k1).all() and np.abs(k1).max() > 0
+    de = _directional(oracle, ctl, atm, obs, k1)
+    print("coarse refracted steps FP64: directional", de)
+    _check(de, 5e-4, "coarse refracted steps FP64 directional")
+
+
 def _scalar_scene():
     ctl, atm, obs = scenes.nadir_co2()
     ctl.refrac = 0
