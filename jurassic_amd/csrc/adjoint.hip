@@ -628,306 +628,6 @@ struct LevelWindow {
   }
 };
 
-// Forward sweep, surface seeds, reverse sweep of the accumulation: writes
-// rad, tau, H_i (scratch), the number of points before the first opaque
-// segment, and the K entries of continua, source function, window
-// extinction, clouds and surface.
-template <typename SF, bool PATH>
-__global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
-                                                  const RayInfo *info, int blk0, int nblk, JacView J,
-                                                  double *rad_out, double *tau_out) {
-  const int nd = M.nd, ng = M.ng;
-  int bl, id;
-  if (!xcd_block_map(blockIdx.x, nd, nblk, bl, id)) return;
-  const int lane = threadIdx.x;
-  const int ir = (blk0 + bl) * WAVE + lane;
-  if (ir >= B.nray) return;
-  const RayInfo R = info[ir];
-  if (R.status != JUR_OK) return;
-  const int ipf = B.ray_prof[ir];
-  const int im = J.prof_map[ipf];
-  const atm_t *atm = &B.atm[ipf];
-  const ChanInfo C = M.chan[id];
-  const double nu = C.nu;
-  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
-  const int npb = E.npb[bl];
-  const SF *epsb = (const SF *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
-#define EPS_AT(g, ip) epsb[((size_t)(g) * npb + (ip)) * WAVE]
-  double *sc = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE + lane;
-#define SCR(k, ip) sc[((size_t)(ip) * J.nscr + (k)) * WAVE]
-  double *kt = J.kt + ((size_t)bl * nd + id) * J.ncol * WAVE + lane;
-#define KT(col) kt[(size_t)(col) * WAVE]
-  const double *sr_col = M.sr + (size_t)id * TBLNS;
-
-  unsigned gm = 0;  // emitters with a table for this channel
-  for (int g = 0; g < ng; g++)
-    if (M.tab.np[g * nd + id] >= 30) gm |= 1u << g;
-  // product of the path transmittances of those emitters after point ip
-  auto qprod = [&](int ip) {
-    double q = 1;
-    for (int g = 0; g < ng; g++)
-      if (gm >> g & 1u) q *= trans_of(EPS_AT(g, ip));
-    return q;
-  };
-
-  double kcl = 0, wcl = 0;
-  const bool cloud = L.f_cl >= 0 && M.ncl > 0;
-  if (cloud) {
-    const double x0 = M.clnu[C.icl], x1 = M.clnu[C.icl + 1];
-    kcl = lin(x0, atm->clk[C.icl], x1, atm->clk[C.icl + 1], nu);
-    wcl = (nu - x0) / (x1 - x0);
-  }
-  double sfeps = 1.0, wsf = 0;
-  if (M.nsf > 0) {
-    const double x0 = M.sfnu[C.isf], x1 = M.sfnu[C.isf + 1];
-    sfeps = lin(x0, atm->sfeps[C.isf], x1, atm->sfeps[C.isf + 1], nu);
-    wsf = (nu - x0) / (x1 - x0);
-  }
-  const bool has_k = C.window < M.nw;
-  const CtmChan K = ctm_chan(M.ctm, nu);
-  // The Jacobian of the FP32 forward model (float continua and source
-  // function interpolation) takes the continua's transcendentals and the
-  // segment transmittance's exponential in float; the sweeps stay FP64.
-  constexpr bool F32 = sizeof(SF) == sizeof(float);
-  const CtmChanF KF = ctm_chan_f(K, nu);
-  // Segment terms of point (p, t, ds): beta and its gradient, the source
-  // function and its slope, and s = tg exp(-beta ds); rt = 1/t, rp = 1/p.
-  auto segment = [&](double p, double t, double ds, double q_h2o, double u_co2, double u_h2o, double kwin,
-                     double clx, double tg, BetaGrad &bg, double &src, double &dsrc, double &rt,
-                     double &rp, double &beta) {
-    if constexpr (F32) {
-      rt = 1 / t;
-      rp = 1 / p;
-      beta = beta_grad_f(M, K, KF, nu, p, t, rt, rp, 1 / ds, q_h2o, u_co2, u_h2o, kwin, clx, bg);
-      src = srcfunc_d(M.st, sr_col, t, dsrc);
-      return tg * (double)__expf((float)(-beta * ds));
-    } else {
-      rt = 1 / t;
-      rp = 1 / p;
-      beta = beta_grad(M, K, nu, p, t, ds, q_h2o, u_co2, u_h2o, kwin, clx, bg);
-      src = srcfunc_d(M.st, sr_col, t, dsrc);
-      return tg * exp(-beta * ds);
-    }
-  };
-  const int fq_h2o = M.ig_h2o >= 0 ? F_Q0 + M.ig_h2o : -1;
-  const int fu_h2o = M.ig_h2o >= 0 ? L.f_u + M.ig_h2o : -1;
-  const int fu_co2 = M.ig_co2 >= 0 ? L.f_u + M.ig_co2 : -1;
-
-  // Point inputs of the sweeps, loaded one point ahead so that a point's
-  // arithmetic does not wait for its own loads (absent fields read F_DS in
-  // their place and are not used).  The emitters' stored values are loaded
-  // with them and multiplied (in qprod's order) one iteration later.
-  struct PtIn {
-    double z, p, t, ds, qh, uc, uh, kw, cl;
-  };
-  const int f_qh = fq_h2o >= 0 ? fq_h2o : F_DS, f_uc = fu_co2 >= 0 ? fu_co2 : F_DS;
-  const int f_uh = fu_h2o >= 0 ? fu_h2o : F_DS, f_kw = has_k ? L.f_k + C.window : F_DS;
-  const int f_cl = cloud ? L.f_cl : F_DS;
-  auto ld_pt = [&](int ip, PtIn &x, bool withz) {
-    if (withz) x.z = LOSR(F_Z, ip);
-    x.p = LOSR(F_P, ip);
-    x.t = LOSR(F_T, ip);
-    x.ds = LOSR(F_DS, ip);
-    x.qh = LOSR(f_qh, ip);
-    x.uc = LOSR(f_uc, ip);
-    x.uh = LOSR(f_uh, ip);
-    x.kw = LOSR(f_kw, ip);
-    x.cl = LOSR(f_cl, ip);
-  };
-  SF en[NG];
-  auto ld_eps = [&](int ip) {
-#pragma unroll
-    for (int g = 0; g < NG; g++) en[g] = (g < ng && (gm >> g & 1u)) ? EPS_AT(g, ip) : SF(0);
-  };
-  auto prod_en = [&]() {
-    double q = 1;
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-      if (g < ng && (gm >> g & 1u)) q *= trans_of(en[g]);
-    return q;
-  };
-  PtIn nx{};
-
-  // ---------------- forward sweep: tau_i (and D_i) per point ----------------
-  double rad = 0, tau = 1, dr = 0;
-  int nef = R.np;
-  {
-    double qp = 1, qn = 1;
-    if (R.np > 0) {
-      ld_pt(0, nx, false);
-      ld_eps(0);
-      qn = prod_en();
-    }
-    for (int ip = 0; ip < R.np; ip++) {
-      const PtIn cu = nx;
-      const double q = qn;
-      const int ipn = ip + 1 < R.np ? ip + 1 : ip;
-      ld_pt(ipn, nx, false);
-      ld_eps(ipn);
-      if (!(q > 0)) {
-        nef = ip;
-        break;
-      }
-      const double tg = q / qp;
-      qp = q;
-      BetaGrad bg;
-      double src, dsrc, rt, rp, beta;
-      const double s = segment(cu.p, cu.t, cu.ds, fq_h2o >= 0 ? cu.qh : 0.0, fu_co2 >= 0 ? cu.uc : 0.0,
-                               fu_h2o >= 0 ? cu.uh : 0.0, has_k ? cu.kw : 0.0, cloud ? cu.cl * kcl : 0.0, tg,
-                               bg, src, dsrc, rt, rp, beta);
-      SCR(0, ip) = tau;
-      if (J.nscr > 1) SCR(1, ip) = dr;
-      rad += src * (1 - s) * tau;
-      dr = dr * s + src * (1 - s);
-      tau *= s;
-      qn = prod_en();
-    }
-  }
-
-  // surface emission, reflection and solar term, and their adjoint seeds
-  double tb = 0, db = 0, sft_bar = 0, sfeps_bar = 0;
-  if (M.sftype >= 1 && R.sft > 0) {
-    double dsrc_sf;
-    const double src_sf = srcfunc_d(M.st, sr_col, R.sft, dsrc_sf);
-    rad += sfeps * src_sf * tau;
-    tb += sfeps * src_sf;
-    sfeps_bar += src_sf * tau;
-    sft_bar = sfeps * tau * dsrc_sf;
-    if (M.sftype >= 2 && sfeps < 1) {
-      rad += dr * tau * (1 - sfeps);
-      tb += dr * (1 - sfeps);
-      db += tau * (1 - sfeps);
-      sfeps_bar -= dr * tau;
-      if (M.sftype >= 3 && R.pad > 0) {
-        const double ks = 6.764e-5 / (2. * JUR_PI) * C.planck_sun * R.rcos;
-        rad += ks * tau * (1 - sfeps) * tau;
-        tb += 2 * ks * tau * (1 - sfeps);
-        sfeps_bar -= ks * tau * tau;
-      }
-    }
-  }
-  rad_out[(size_t)ir * nd + id] = rad;
-  tau_out[(size_t)ir * nd + id] = tau;
-  J.nef[((size_t)bl * nd + id) * WAVE + lane] = nef;
-  if (im < 0 || im >= J.nmap) return;  // no state map: K row stays zero
-  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
-
-  // ---------------- reverse sweep of the accumulation ----------------
-  // H_i replaces tau_i in the scratch; continua, source function, window
-  // extinction and clouds contribute here, the emitter chains below.
-  double clz_bar = 0, cldz_bar = 0, clk_bar = 0;
-  {
-    const int qi[5] = {1, 0, has_k ? 2 + NG + C.window : -1, M.ig_co2 >= 0 ? 2 + M.ig_co2 : -1,
-                       M.ig_h2o >= 0 ? 2 + M.ig_h2o : -1};
-    LevelWindow<5, 0x1Fu, F32, PATH> W;
-    // the path adjoint's point values d rad / d z_i, d rad / d ds_i: zero
-    // beyond the first opaque segment, the emitter chains add theirs
-    if (PATH)
-      for (int ip = nef; ip < R.np; ip++) SCR(J.kz, ip) = SCR(J.kz + 1, ip) = 0.0;
-    int lev = -1;
-    double qi_cur = nef > 0 ? qprod(nef - 1) : 1;
-    // point ip's inputs and scratch, and the emitters' values at ip - 1,
-    // loaded one iteration ahead
-    double sc0n = 0, sc1n = 0;
-    auto ld_rev = [&](int ip) {
-      ld_pt(ip, nx, true);
-      sc0n = SCR(0, ip);
-      sc1n = J.nscr > 1 ? SCR(1, ip) : 0.0;
-      if (ip > 0) ld_eps(ip - 1);
-    };
-    if (nef > 0) ld_rev(nef - 1);
-    for (int ip = nef - 1; ip >= 0; ip--) {
-      const PtIn cu = nx;
-      const double tau_i = sc0n, dr_i = sc1n;
-      const double qm = ip > 0 ? prod_en() : 1;
-      if (ip > 0) ld_rev(ip - 1);
-      const double tg = qi_cur / qm;
-      qi_cur = qm;
-      const double z = cu.z, p = cu.p, t = cu.t, ds = cu.ds;
-      const double q_h2o = fq_h2o >= 0 ? cu.qh : 0.0;
-      const double u_co2 = fu_co2 >= 0 ? cu.uc : 0.0;
-      const double u_h2o = fu_h2o >= 0 ? cu.uh : 0.0;
-      const double aux = cloud ? cu.cl : 0.0;
-      BetaGrad bg;
-      double src, dsrc, rt, rp, beta;
-      const double s = segment(p, t, ds, q_h2o, u_co2, u_h2o, has_k ? cu.kw : 0.0, aux * kcl, tg, bg, src, dsrc,
-                               rt, rp, beta);
-      const double G = tau_i * (tb - src) + db * (dr_i - src);
-      const double src_bar = (1 - s) * (tau_i + db);
-      tb = src * (1 - s) + tb * s;
-      db = db * s;
-      const double H = G * s;
-      SCR(0, ip) = H;
-      const double beta_bar = -H * ds;
-
-      // u = 10 q p / (kB T) ds
-      const double ucoef = F32 ? 10 * p * rt * ds / JUR_KB : 10 * p / (JUR_KB * t) * ds;
-      const double uc_bar = beta_bar * bg.du_co2, uh_bar = beta_bar * bg.du_h2o;
-      const double uu = uc_bar * u_co2 + uh_bar * u_h2o;
-      double v[5];
-      v[0] = beta_bar * bg.dt + src_bar * dsrc - (F32 ? uu * rt : uu / t);  // T
-      v[1] = beta_bar * bg.dp + (F32 ? uu * rp : uu / p);                   // p
-      v[2] = beta_bar;                                                                  // k
-      v[3] = uc_bar * ucoef;                                                            // q_co2
-      v[4] = uh_bar * ucoef + beta_bar * bg.dq_h2o;                                     // q_h2o
-      const double gz = W.add(atm, z, lev, v, 1, p, xm, qi, kt);
-      if (PATH) {
-        // altitude: the level quantities' slopes and the cloud profile's;
-        // segment length: exp(-beta ds) (the continua's beta does not
-        // depend on ds, their u ~ ds cancels their 1/ds), the column
-        // densities' part comes from the emitter chains
-        double gzc = gz;
-        if (cloud && atm->cldz > 0) gzc -= beta_bar * kcl * aux * (z - atm->clz) / (atm->cldz * atm->cldz);
-        SCR(J.kz, ip) = gzc;
-        SCR(J.kz + 1, ip) = -H * beta;
-      }
-
-      if (cloud) {
-        clk_bar += beta_bar * aux;
-        if (atm->cldz > 0) {
-          const double cl_bar = beta_bar * kcl;
-          const double dz = (z - atm->clz) / atm->cldz;
-          clz_bar += cl_bar * aux * dz / atm->cldz;
-          cldz_bar += cl_bar * aux * dz * dz / atm->cldz;
-        }
-      }
-    }
-    W.flush(xm, qi, kt);
-  }
-
-  // surface terms
-  if (M.sftype >= 1 && R.sft > 0) {
-    if (M.nsf > 0 && atm->sft > 0) {
-      const int c = xm[JUR_XMAP_SFT];
-      if (c >= 0) KT(c) += sft_bar;
-    } else if (R.np > 0) {
-      // the surface temperature is the LOS temperature of the last point
-      double w;
-      const int lv = level_of(atm, LOSR(F_Z, R.np - 1), -1, w);
-      int c;
-      if ((c = xm[JUR_XMAP_LEV(1, lv)]) >= 0) KT(c) += (1 - w) * sft_bar;
-      if ((c = xm[JUR_XMAP_LEV(1, lv + 1)]) >= 0) KT(c) += w * sft_bar;
-      if (PATH && R.np - 1 < nef)
-        SCR(J.kz, R.np - 1) += sft_bar * (atm->t[lv + 1] - atm->t[lv]) / (atm->z[lv + 1] - atm->z[lv]);
-    }
-    if (M.nsf > 0) {
-      int c;
-      if ((c = xm[JUR_XMAP_SFEPS(C.isf)]) >= 0) KT(c) += (1 - wsf) * sfeps_bar;
-      if (C.isf + 1 < NSF && (c = xm[JUR_XMAP_SFEPS(C.isf + 1)]) >= 0) KT(c) += wsf * sfeps_bar;
-    }
-  }
-  if (cloud) {
-    int c;
-    if ((c = xm[JUR_XMAP_CLK(C.icl)]) >= 0) KT(c) += (1 - wcl) * clk_bar;
-    if (C.icl + 1 < NCL && (c = xm[JUR_XMAP_CLK(C.icl + 1)]) >= 0) KT(c) += wcl * clk_bar;
-    if ((c = xm[JUR_XMAP_CLZ]) >= 0) KT(c) += clz_bar;
-    if ((c = xm[JUR_XMAP_CLDZ]) >= 0) KT(c) += cldz_bar;
-  }
-#undef EPS_AT
-#undef SCR
-#undef KT
-}
-
 // Reverse sweep of one emitter chain per wavefront: L_g,i from H_i and the
 // recomputed table partials; T and p entries are shared with the other
 // emitters of the row (atomic adds), q_g entries belong to this chain.
@@ -1518,54 +1218,28 @@ __global__ __launch_bounds__(WAVE) void k_path_coef(ModelView M, BatchView B, Lo
   }
 }
 
-// The reverse sweep over the steps, CH channels per lane (one ray).  Level
-// sums are kept for a window of three levels (the step's LB and the two
-// above) per channel and added to the K tile when the window moves.  With
-// PF the inputs of the next step are loaded while a step is computed (for
-// small batches, where a wavefront's sweep is the kernel's duration).
-template <int CH, bool PF>
-__global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, LosView L, EpsView E,
-                                                   const RayInfo *info, const RayTail *tail, int blk0, int nblk,
-                                                   JacView J) {
-  const int nd = M.nd;
-  const int ngrp = (nd + CH - 1) / CH;
-  int bl, grp;
-  if (!xcd_block_map(blockIdx.x, ngrp, nblk, bl, grp)) return;
-  const int lane = threadIdx.x;
-  const int ir = (blk0 + bl) * WAVE + lane;
-  if (ir >= B.nray) return;
-  const RayInfo R = info[ir];
-  if (R.status != JUR_OK || R.np < 2) return;
-  const int ipf = B.ray_prof[ir];
-  const int im = J.prof_map[ipf];
-  if (im < 0 || im >= J.nmap) return;
-  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
-  const atm_t *atm = &B.atm[ipf];
-  const int nlev = atm->np;
-  const int id0 = grp * CH;
-  const int nc = nd - id0 < CH ? nd - id0 : CH;
-  const int npb = E.npb[bl];
-  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
-  const double *sc0 = J.scr + J.soff[bl] + (size_t)id0 * npb * J.nscr * WAVE + lane;
-  const size_t scstride = (size_t)npb * J.nscr * WAVE;
-#define GZ(c, ip) sc0[(c) * scstride + ((size_t)(ip) * J.nscr + J.kz) * WAVE]
-#define GD(c, ip) sc0[(c) * scstride + ((size_t)(ip) * J.nscr + J.kz + 1) * WAVE]
-  double *kt0 = J.kt + ((size_t)bl * nd + id0) * J.ncol * WAVE + lane;
-  const size_t ktstride = (size_t)J.ncol * WAVE;
-  const double *pc = J.pco + J.soff[bl] / ((size_t)nd * J.nscr) * JUR_NPCO + lane;
-  const RayTail T = tail[ir];
-  const int N = R.np;
-  auto xe = [&](int ip, int i) { return LOSR(L.f_xe + i, ip); };
+// The reverse march of CH channels of one ray (a lane): the adjoints (X, E)
+// of the state, the level window, and the clip of the last step.  Level sums
+// are kept for a window of three levels (the step's LB and the two above)
+// per channel and added to the K tile when the window moves.
+template <int CH>
+struct PathRev {
+  const int *xm;
+  int nlev, nc;
+  double *kt0;
+  size_t ktstride;
+  double X[CH][3], Ev[CH][3], Xp[CH][3], gzp[CH], gdn[CH], Tw[CH][3], Pw[CH][3];
+  int wb;
 
-  double X[CH][3], Ev[CH][3], Xp[CH][3], gzp[CH], gdn[CH];
-  // ---- the clipped last point: x_l = x_p + frac (x_cut - x_p), its raw
-  // step replacing point N-2's: ds_cut frac ----
-  {
+  // The clipped last point: x_l = x_p + frac (x_cut - x_p), its raw step
+  // replacing point N-2's: ds_cut frac.  gz: d rad / d z of point N-1; gdl,
+  // gdm: d rad / d ds of points N-1 and N-2.
+  __device__ __forceinline__ void init(const ModelView &M, const RayTail &T, const double *xl, const double *gz,
+                                       const double *gdl, const double *gdm) {
+    wb = -1;
     const double zp = T.zprev, zc = T.zcut;
     const double zb = zc < T.zmin ? T.zmin : T.zmax;
     const double frac = (zb - zp) / (zc - zp);
-    double xl[3];
-    for (int i = 0; i < 3; i++) xl[i] = xe(N - 1, i);
     const double rl = sqrt(xl[0] * xl[0] + xl[1] * xl[1] + xl[2] * xl[2]);
     const double rc = sqrt(T.xcut[0] * T.xcut[0] + T.xcut[1] * T.xcut[1] + T.xcut[2] * T.xcut[2]);
     // d ds / d (x, e) at the cut point
@@ -1584,14 +1258,11 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
     }
 #pragma unroll
     for (int c = 0; c < CH; c++) {
-      const double gz = c < nc ? GZ(c, N - 1) : 0.0;
-      const double gdl = c < nc ? GD(c, N - 1) : 0.0;
-      const double gdm = c < nc ? GD(c, N - 2) : 0.0;
-      gdn[c] = gdm;
-      const double gs = 0.5 * (gdm + gdl);  // d rad / d (clipped raw ds of point N-2)
+      gdn[c] = gdm[c];
+      const double gs = 0.5 * (gdm[c] + gdl[c]);  // d rad / d (clipped raw ds of point N-2)
       double gfrac = T.dscut * gs;
       for (int i = 0; i < 3; i++) {
-        const double xli = gz * xl[i] / rl;
+        const double xli = gz[c] * xl[i] / rl;
         Xp[c][i] = (1 - frac) * xli;
         X[c][i] = frac * xli;
         gfrac += xli * (T.xcut[i] - T.xprev[i]);
@@ -1605,9 +1276,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
     }
   }
 
-  double Tw[CH][3], Pw[CH][3];
-  int wb = -1;
-  auto flush_level = [&](int j) {
+  __device__ __forceinline__ void flush_level(int j) {
     const int l = wb + j;
     if (l >= nlev) return;
     const int ct = xm[JUR_XMAP_LEV(1, l)], cp = xm[JUR_XMAP_LEV(0, l)];
@@ -1618,24 +1287,12 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
       if (ct >= 0) kt[(size_t)ct * WAVE] += Tw[c][j];
       if (cp >= 0) kt[(size_t)cp * WAVE] += Pw[c][j];
     }
-  };
-  // a step's inputs: its coefficients and the channels' point values
-  struct StepIn {
-    double c[JUR_NPCO];
-    double gz[CH], gd[CH];
-  };
-  auto load_in = [&](int k, StepIn &in) {
-    const double *o = pc + (size_t)k * JUR_NPCO * WAVE;
-#pragma unroll
-    for (int f = 0; f < JUR_NPCO; f++) in.c[f] = o[(size_t)f * WAVE];
-#pragma unroll
-    for (int c = 0; c < CH; c++) {
-      in.gz[c] = c < nc ? GZ(c, k) : 0.0;
-      in.gd[c] = c < nc ? GD(c, k) : 0.0;
-    }
-  };
-  auto step = [&](int k, const StepIn &in) {
-#define PCR(f) in.c[f]
+  }
+
+  // The reverse of march step k (of N points) with its coefficients cf and
+  // the channels' point values gz = d rad / d z_k, gd = d rad / d ds_k.
+  __device__ __forceinline__ void step(int k, int N, const double *cf, const double *gzv, const double *gdv) {
+#define PCR(f) cf[f]
     const double hds = PCR(PC_HDS), rnv = PCR(PC_RNV), n = PCR(PC_N), sk = PCR(PC_SK);
     double e[3], e1[3], ngm[3], xh[3], a[3], b[3], mx[9];
 #pragma unroll
@@ -1653,7 +1310,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
     double V[CH][3];
 #pragma unroll
     for (int c = 0; c < CH; c++) {
-      const double gz = in.gz[c], gd = in.gd[c];
+      const double gz = gzv[c], gd = gdv[c];
       const double gs = (k == N - 2) ? 0.0 : 0.5 * (gd + gdn[c]);  // point N-2's raw ds was clipped
       gdn[c] = gd;
       double G[3];
@@ -1742,16 +1399,529 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
       }
     }
 #undef PCR
+  }
+
+  // after step N-2 (which leads to the cut point): the clip's dependence on
+  // x_N-2 joins its adjoint (cf: step N-2's coefficients)
+  __device__ __forceinline__ void join_clip(const double *cf) {
+#pragma unroll
+    for (int c = 0; c < CH; c++)
+      for (int i = 0; i < 3; i++) X[c][i] += Xp[c][i] + gzp[c] * cf[PC_XH + i];
+  }
+
+  __device__ __forceinline__ void finish() {
+    if (wb >= 0)
+      for (int j = 0; j < 3; j++) flush_level(j);
+  }
+};
+
+// A step's inputs: its coefficients and the channels' point values.
+template <int CH>
+struct PathStepIn {
+  double c[JUR_NPCO];
+  double gz[CH], gd[CH];
+};
+
+// Forward sweep, surface seeds, reverse sweep of the accumulation: writes
+// rad, tau, H_i (scratch), the number of points before the first opaque
+// segment, and the K entries of continua, source function, window
+// extinction, clouds and surface.
+// EMF > 0 (FP32 EGA with stored partials, ng <= EMF): the reverse sweep also
+// runs the emitter chains of the row, point by point, as k_adj_chains_f32
+// does; H_i is used where it is formed, and the chains' T, p and q adjoints
+// join the accumulation's in one level window.  PFU (with PATH): it also
+// runs the reverse march of the refracted path (PathRev, one channel) point
+// by point from the stored step coefficients, so that the row's whole
+// reverse sweep is one pass over the path: the form for small batches,
+// where each kernel lasts as long as one wavefront's sweep.
+template <typename SF, bool PATH, int EMF = 0, bool PFU = false>
+__global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosView L, EpsView E,
+                                                  const RayInfo *info, const RayTail *tail, int blk0, int nblk,
+                                                  JacView J, double *rad_out, double *tau_out) {
+  const int nd = M.nd, ng = M.ng;
+  int bl, id;
+  if (!xcd_block_map(blockIdx.x, nd, nblk, bl, id)) return;
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + bl) * WAVE + lane;
+  if (ir >= B.nray) return;
+  const RayInfo R = info[ir];
+  if (R.status != JUR_OK) return;
+  const int ipf = B.ray_prof[ir];
+  const int im = J.prof_map[ipf];
+  const atm_t *atm = &B.atm[ipf];
+  const ChanInfo C = M.chan[id];
+  const double nu = C.nu;
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const int npb = E.npb[bl];
+  const SF *epsb = (const SF *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
+#define EPS_AT(g, ip) epsb[((size_t)(g) * npb + (ip)) * WAVE]
+  double *sc = J.scr + J.soff[bl] + (size_t)id * npb * J.nscr * WAVE + lane;
+#define SCR(k, ip) sc[((size_t)(ip) * J.nscr + (k)) * WAVE]
+  double *kt = J.kt + ((size_t)bl * nd + id) * J.ncol * WAVE + lane;
+#define KT(col) kt[(size_t)(col) * WAVE]
+  const double *sr_col = M.sr + (size_t)id * TBLNS;
+
+  unsigned gm = 0;  // emitters with a table for this channel
+  for (int g = 0; g < ng; g++)
+    if (M.tab.np[g * nd + id] >= 30) gm |= 1u << g;
+  // product of the path transmittances of those emitters after point ip
+  auto qprod = [&](int ip) {
+    double q = 1;
+    for (int g = 0; g < ng; g++)
+      if (gm >> g & 1u) q *= trans_of(EPS_AT(g, ip));
+    return q;
   };
-  // step N-2 leads to the cut point; the clip's dependence on x_N-2 joins
-  // its adjoint afterwards
-  StepIn cur, nxt;
+
+  double kcl = 0, wcl = 0;
+  const bool cloud = L.f_cl >= 0 && M.ncl > 0;
+  if (cloud) {
+    const double x0 = M.clnu[C.icl], x1 = M.clnu[C.icl + 1];
+    kcl = lin(x0, atm->clk[C.icl], x1, atm->clk[C.icl + 1], nu);
+    wcl = (nu - x0) / (x1 - x0);
+  }
+  double sfeps = 1.0, wsf = 0;
+  if (M.nsf > 0) {
+    const double x0 = M.sfnu[C.isf], x1 = M.sfnu[C.isf + 1];
+    sfeps = lin(x0, atm->sfeps[C.isf], x1, atm->sfeps[C.isf + 1], nu);
+    wsf = (nu - x0) / (x1 - x0);
+  }
+  const bool has_k = C.window < M.nw;
+  const CtmChan K = ctm_chan(M.ctm, nu);
+  // The Jacobian of the FP32 forward model (float continua and source
+  // function interpolation) takes the continua's transcendentals and the
+  // segment transmittance's exponential in float; the sweeps stay FP64.
+  constexpr bool F32 = sizeof(SF) == sizeof(float);
+  const CtmChanF KF = ctm_chan_f(K, nu);
+  // Segment terms of point (p, t, ds): beta and its gradient, the source
+  // function and its slope, and s = tg exp(-beta ds); rt = 1/t, rp = 1/p.
+  auto segment = [&](double p, double t, double ds, double q_h2o, double u_co2, double u_h2o, double kwin,
+                     double clx, double tg, BetaGrad &bg, double &src, double &dsrc, double &rt,
+                     double &rp, double &beta) {
+    if constexpr (F32) {
+      rt = 1 / t;
+      rp = 1 / p;
+      beta = beta_grad_f(M, K, KF, nu, p, t, rt, rp, 1 / ds, q_h2o, u_co2, u_h2o, kwin, clx, bg);
+      src = srcfunc_d(M.st, sr_col, t, dsrc);
+      return tg * (double)__expf((float)(-beta * ds));
+    } else {
+      rt = 1 / t;
+      rp = 1 / p;
+      beta = beta_grad(M, K, nu, p, t, ds, q_h2o, u_co2, u_h2o, kwin, clx, bg);
+      src = srcfunc_d(M.st, sr_col, t, dsrc);
+      return tg * exp(-beta * ds);
+    }
+  };
+  const int fq_h2o = M.ig_h2o >= 0 ? F_Q0 + M.ig_h2o : -1;
+  const int fu_h2o = M.ig_h2o >= 0 ? L.f_u + M.ig_h2o : -1;
+  const int fu_co2 = M.ig_co2 >= 0 ? L.f_u + M.ig_co2 : -1;
+
+  // Point inputs of the sweeps, loaded one point ahead so that a point's
+  // arithmetic does not wait for its own loads (absent fields read F_DS in
+  // their place and are not used).  The emitters' stored values are loaded
+  // with them and multiplied (in qprod's order) one iteration later.
+  struct PtIn {
+    double z, p, t, ds, qh, uc, uh, kw, cl;
+  };
+  const int f_qh = fq_h2o >= 0 ? fq_h2o : F_DS, f_uc = fu_co2 >= 0 ? fu_co2 : F_DS;
+  const int f_uh = fu_h2o >= 0 ? fu_h2o : F_DS, f_kw = has_k ? L.f_k + C.window : F_DS;
+  const int f_cl = cloud ? L.f_cl : F_DS;
+  auto ld_pt = [&](int ip, PtIn &x, bool withz) {
+    if (withz) x.z = LOSR(F_Z, ip);
+    x.p = LOSR(F_P, ip);
+    x.t = LOSR(F_T, ip);
+    x.ds = LOSR(F_DS, ip);
+    x.qh = LOSR(f_qh, ip);
+    x.uc = LOSR(f_uc, ip);
+    x.uh = LOSR(f_uh, ip);
+    x.kw = LOSR(f_kw, ip);
+    x.cl = LOSR(f_cl, ip);
+  };
+  SF en[NG];
+  auto ld_eps = [&](int ip) {
+#pragma unroll
+    for (int g = 0; g < NG; g++) en[g] = (g < ng && (gm >> g & 1u)) ? EPS_AT(g, ip) : SF(0);
+  };
+  auto prod_en = [&]() {
+    double q = 1;
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+      if (g < ng && (gm >> g & 1u)) q *= trans_of(en[g]);
+    return q;
+  };
+  PtIn nx{};
+
+  // ---------------- forward sweep: tau_i (and D_i) per point ----------------
+  double rad = 0, tau = 1, dr = 0;
+  int nef = R.np;
+  {
+    double qp = 1, qn = 1;
+    if (R.np > 0) {
+      ld_pt(0, nx, false);
+      ld_eps(0);
+      qn = prod_en();
+    }
+    for (int ip = 0; ip < R.np; ip++) {
+      const PtIn cu = nx;
+      const double q = qn;
+      const int ipn = ip + 1 < R.np ? ip + 1 : ip;
+      ld_pt(ipn, nx, false);
+      ld_eps(ipn);
+      if (!(q > 0)) {
+        nef = ip;
+        break;
+      }
+      const double tg = q / qp;
+      qp = q;
+      BetaGrad bg;
+      double src, dsrc, rt, rp, beta;
+      const double s = segment(cu.p, cu.t, cu.ds, fq_h2o >= 0 ? cu.qh : 0.0, fu_co2 >= 0 ? cu.uc : 0.0,
+                               fu_h2o >= 0 ? cu.uh : 0.0, has_k ? cu.kw : 0.0, cloud ? cu.cl * kcl : 0.0, tg,
+                               bg, src, dsrc, rt, rp, beta);
+      SCR(0, ip) = tau;
+      if (J.nscr > 1) SCR(1, ip) = dr;
+      rad += src * (1 - s) * tau;
+      dr = dr * s + src * (1 - s);
+      tau *= s;
+      qn = prod_en();
+    }
+  }
+
+  // surface emission, reflection and solar term, and their adjoint seeds
+  double tb = 0, db = 0, sft_bar = 0, sfeps_bar = 0;
+  if (M.sftype >= 1 && R.sft > 0) {
+    double dsrc_sf;
+    const double src_sf = srcfunc_d(M.st, sr_col, R.sft, dsrc_sf);
+    rad += sfeps * src_sf * tau;
+    tb += sfeps * src_sf;
+    sfeps_bar += src_sf * tau;
+    sft_bar = sfeps * tau * dsrc_sf;
+    if (M.sftype >= 2 && sfeps < 1) {
+      rad += dr * tau * (1 - sfeps);
+      tb += dr * (1 - sfeps);
+      db += tau * (1 - sfeps);
+      sfeps_bar -= dr * tau;
+      if (M.sftype >= 3 && R.pad > 0) {
+        const double ks = 6.764e-5 / (2. * JUR_PI) * C.planck_sun * R.rcos;
+        rad += ks * tau * (1 - sfeps) * tau;
+        tb += 2 * ks * tau * (1 - sfeps);
+        sfeps_bar -= ks * tau * tau;
+      }
+    }
+  }
+  rad_out[(size_t)ir * nd + id] = rad;
+  tau_out[(size_t)ir * nd + id] = tau;
+  J.nef[((size_t)bl * nd + id) * WAVE + lane] = nef;
+  if (im < 0 || im >= J.nmap) return;  // no state map: K row stays zero
+  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
+
+  // ---------------- reverse sweep of the accumulation ----------------
+  // H_i replaces tau_i in the scratch; continua, source function, window
+  // extinction and clouds contribute here, the emitter chains below.
+  double clz_bar = 0, cldz_bar = 0, clk_bar = 0;
+  // the surface temperature taken from the last LOS point's T: its slope
+  // moves d rad / d z of that point (added here where the path adjoint runs
+  // in this sweep, after the sweep otherwise)
+  double gz_sft = 0;
+  if (PFU && M.sftype >= 1 && R.sft > 0 && !(M.nsf > 0 && atm->sft > 0) && R.np > 0 && R.np - 1 < nef) {
+    double w;
+    const int lv = level_of(atm, LOSR(F_Z, R.np - 1), -1, w);
+    gz_sft = sft_bar * (atm->t[lv + 1] - atm->t[lv]) / (atm->z[lv + 1] - atm->z[lv]);
+  }
+  {
+    // level quantities: T, p, the window's extinction, then CO2 and H2O, or
+    // with the chains every emitter's q
+    constexpr int NQW = EMF > 0 ? 3 + EMF : 5;
+    int qi[NQW];
+    qi[0] = 1;
+    qi[1] = 0;
+    qi[2] = has_k ? 2 + NG + C.window : -1;
+    if constexpr (EMF > 0) {
+#pragma unroll
+      for (int g = 0; g < EMF; g++) qi[3 + g] = g < ng ? 2 + g : -1;
+    } else {
+      qi[3] = M.ig_co2 >= 0 ? 2 + M.ig_co2 : -1;
+      qi[4] = M.ig_h2o >= 0 ? 2 + M.ig_h2o : -1;
+    }
+    LevelWindow<NQW, (1u << NQW) - 1u, F32, PATH> W;
+    // the path adjoint's point values d rad / d z_i, d rad / d ds_i: zero
+    // beyond the first opaque segment, the emitter chains add theirs
+    if (PATH && !PFU)
+      for (int ip = nef; ip < R.np; ip++) SCR(J.kz, ip) = SCR(J.kz + 1, ip) = 0.0;
+    int lev = -1;
+    double qi_cur = nef > 0 ? qprod(nef - 1) : 1;
+    // the chains' state: Lambda carry, the path transmittance after the
+    // point (as the float the chain divides by), the point's stored partials
+    // (dep, dp, dT, du) and column density, loaded one point ahead
+    constexpr int EC = EMF > 0 ? EMF : 1;
+    bool on[EC];
+    double carry[EC], ucur[EC], unx[EC];
+    float trc[EC];
+    float4 dcur[EC], dnx[EC];
+    const float4 *pab = EMF > 0 ? E.part + E.off[bl] + (size_t)id * ng * npb * WAVE + lane : nullptr;
+#pragma unroll
+    for (int g = 0; g < EC; g++) {
+      on[g] = EMF > 0 && g < ng && (gm >> g & 1u);
+      carry[g] = ucur[g] = unx[g] = 0;
+      trc[g] = (on[g] && nef > 0) ? (float)trans_of(EPS_AT(g, nef - 1)) : 1.f;
+      dcur[g] = dnx[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    double Hn = 0;
+    // the path's reverse march (PFU): its state, the held values of the last
+    // point, and the next step's coefficients loaded one point ahead
+    const int N = R.np;
+    constexpr int PC1 = PFU ? JUR_NPCO : 1;
+    PathRev<1> P;
+    double pcn[PC1];
+    bool pstart = false;
+    double gz_last = 0, gd_last = 0;
+    const double *pcf = nullptr;
+    if constexpr (PFU) {
+      P.xm = xm;
+      P.nlev = atm->np;
+      P.nc = 1;
+      P.kt0 = kt;
+      P.ktstride = (size_t)J.ncol * WAVE;
+      pcf = J.pco + J.soff[bl] / ((size_t)nd * J.nscr) * JUR_NPCO + lane;
+      const int k0 = nef - 1 < N - 2 ? nef - 1 : N - 2;
+      if (k0 >= 0)
+#pragma unroll
+        for (int f = 0; f < JUR_NPCO; f++) pcn[f] = pcf[((size_t)k0 * JUR_NPCO + f) * WAVE];
+    }
+    // point ip's inputs and scratch, and the emitters' values at ip - 1,
+    // loaded one iteration ahead
+    double sc0n = 0, sc1n = 0;
+    auto ld_rev = [&](int ip) {
+      ld_pt(ip, nx, true);
+      sc0n = SCR(0, ip);
+      sc1n = J.nscr > 1 ? SCR(1, ip) : 0.0;
+      if (ip > 0) ld_eps(ip - 1);
+      if constexpr (EMF > 0) {
+#pragma unroll
+        for (int g = 0; g < EMF; g++)
+          if (on[g]) {
+            dnx[g] = pab[((size_t)g * npb + ip) * WAVE];
+            unx[g] = LOSR(L.f_u + g, ip);
+          }
+      }
+    };
+    if (nef > 0) ld_rev(nef - 1);
+    for (int ip = nef - 1; ip >= 0; ip--) {
+      const PtIn cu = nx;
+      const double tau_i = sc0n, dr_i = sc1n;
+      const double qm = ip > 0 ? prod_en() : 1;
+      float em1[EC];  // the emitters' path emissivities before point ip
+#pragma unroll
+      for (int g = 0; g < EC; g++) {
+        em1[g] = EMF > 0 ? (float)en[g] : 0.f;
+        dcur[g] = dnx[g];
+        ucur[g] = unx[g];
+      }
+      if (ip > 0) ld_rev(ip - 1);
+      const double tg = qi_cur / qm;
+      qi_cur = qm;
+      const double z = cu.z, p = cu.p, t = cu.t, ds = cu.ds;
+      const double q_h2o = fq_h2o >= 0 ? cu.qh : 0.0;
+      const double u_co2 = fu_co2 >= 0 ? cu.uc : 0.0;
+      const double u_h2o = fu_h2o >= 0 ? cu.uh : 0.0;
+      const double aux = cloud ? cu.cl : 0.0;
+      BetaGrad bg;
+      double src, dsrc, rt, rp, beta;
+      const double s = segment(p, t, ds, q_h2o, u_co2, u_h2o, has_k ? cu.kw : 0.0, aux * kcl, tg, bg, src, dsrc,
+                               rt, rp, beta);
+      const double G = tau_i * (tb - src) + db * (dr_i - src);
+      const double src_bar = (1 - s) * (tau_i + db);
+      tb = src * (1 - s) + tb * s;
+      db = db * s;
+      const double H = G * s;
+      if (EMF == 0) SCR(0, ip) = H;
+      const double beta_bar = -H * ds;
+
+      // u = 10 q p / (kB T) ds
+      const double ucoef = F32 ? 10 * p * rt * ds / JUR_KB : 10 * p / (JUR_KB * t) * ds;
+      const double uc_bar = beta_bar * bg.du_co2, uh_bar = beta_bar * bg.du_h2o;
+      const double uu = uc_bar * u_co2 + uh_bar * u_h2o;
+      double v[NQW];
+      v[0] = beta_bar * bg.dt + src_bar * dsrc - (F32 ? uu * rt : uu / t);  // T
+      v[1] = beta_bar * bg.dp + (F32 ? uu * rp : uu / p);                   // p
+      v[2] = beta_bar;                                                                  // k
+      const double vco2 = uc_bar * ucoef, vh2o = uh_bar * ucoef + beta_bar * bg.dq_h2o;
+      double gu = 0;  // the chains' sum of u_bar u_seg (d rad / d ds_i through u)
+      if constexpr (EMF > 0) {
+        // the chains (as k_adj_chains_f32: float reciprocals of T, p and the
+        // transmittance, whose 1e-7 is far below the FP32 lookups')
+        const double rtf = (double)frcp((float)t), rpf = (double)frcp((float)p);
+        const double ucf = 10 * p * rtf * ds * (1 / JUR_KB);
+        double vt = 0, vp = 0;
+#pragma unroll
+        for (int g = 0; g < EMF; g++) {
+          double vq = (g == M.ig_co2 ? vco2 : 0.0) + (g == M.ig_h2o ? vh2o : 0.0);
+          if (on[g]) {
+            const double lam = (Hn - H) * (double)frcp(trc[g]) + carry[g];
+            const double u_bar = lam * dcur[g].w;
+            carry[g] = lam * dcur[g].x;
+            vt += lam * dcur[g].z - u_bar * ucur[g] * rtf;
+            vp += lam * dcur[g].y + u_bar * ucur[g] * rpf;
+            vq += u_bar * ucf;
+            gu += u_bar * ucur[g];
+            trc[g] = ip > 0 ? (float)(1.0 - (double)em1[g]) : 1.f;
+          }
+          v[3 + g] = vq;
+        }
+        v[0] += vt;
+        v[1] += vp;
+        Hn = H;
+      } else {
+        v[3] = vco2;  // q_co2
+        v[4] = vh2o;  // q_h2o
+      }
+      const double gz = W.add(atm, z, lev, v, 1, p, xm, qi, kt);
+      if (PATH) {
+        // altitude: the level quantities' slopes and the cloud profile's;
+        // segment length: exp(-beta ds) (the continua's beta does not
+        // depend on ds, their u ~ ds cancels their 1/ds), the column
+        // densities' part comes from the emitter chains
+        double gzc = gz;
+        if (cloud && atm->cldz > 0) gzc -= beta_bar * kcl * aux * (z - atm->clz) / (atm->cldz * atm->cldz);
+        const double gdc = EMF > 0 ? -H * beta + gu / ds : -H * beta;
+        if constexpr (PFU) {
+          if (ip == N - 1) {
+            gz_last = gzc + gz_sft;
+            gd_last = gdc;
+          } else {
+            if (!pstart) {
+              // the first step reached: the clip of the last step, with
+              // point N-2's d rad / d ds where this is that point
+              double xl[3];
+              for (int i = 0; i < 3; i++) xl[i] = LOSR(L.f_xe + i, N - 1);
+              const double gdm = ip == N - 2 ? gdc : 0.0;
+              P.init(M, tail[ir], xl, &gz_last, &gd_last, &gdm);
+              pstart = true;
+            }
+            P.step(ip, N, pcn, &gzc, &gdc);
+            if (ip == N - 2) P.join_clip(pcn);
+            // the next step's coefficients, in flight during the next
+            // point's accumulation
+            if (ip > 0)
+#pragma unroll
+              for (int f = 0; f < JUR_NPCO; f++) pcn[f] = pcf[((size_t)(ip - 1) * JUR_NPCO + f) * WAVE];
+          }
+        } else {
+          SCR(J.kz, ip) = gzc;
+          SCR(J.kz + 1, ip) = gdc;
+        }
+      }
+
+      if (cloud) {
+        clk_bar += beta_bar * aux;
+        if (atm->cldz > 0) {
+          const double cl_bar = beta_bar * kcl;
+          const double dz = (z - atm->clz) / atm->cldz;
+          clz_bar += cl_bar * aux * dz / atm->cldz;
+          cldz_bar += cl_bar * aux * dz * dz / atm->cldz;
+        }
+      }
+    }
+    if (PFU && pstart) P.finish();
+    W.flush(xm, qi, kt);
+  }
+
+  // surface terms
+  if (M.sftype >= 1 && R.sft > 0) {
+    if (M.nsf > 0 && atm->sft > 0) {
+      const int c = xm[JUR_XMAP_SFT];
+      if (c >= 0) KT(c) += sft_bar;
+    } else if (R.np > 0) {
+      // the surface temperature is the LOS temperature of the last point
+      double w;
+      const int lv = level_of(atm, LOSR(F_Z, R.np - 1), -1, w);
+      int c;
+      if ((c = xm[JUR_XMAP_LEV(1, lv)]) >= 0) KT(c) += (1 - w) * sft_bar;
+      if ((c = xm[JUR_XMAP_LEV(1, lv + 1)]) >= 0) KT(c) += w * sft_bar;
+      if (PATH && !PFU && R.np - 1 < nef)
+        SCR(J.kz, R.np - 1) += sft_bar * (atm->t[lv + 1] - atm->t[lv]) / (atm->z[lv + 1] - atm->z[lv]);
+    }
+    if (M.nsf > 0) {
+      int c;
+      if ((c = xm[JUR_XMAP_SFEPS(C.isf)]) >= 0) KT(c) += (1 - wsf) * sfeps_bar;
+      if (C.isf + 1 < NSF && (c = xm[JUR_XMAP_SFEPS(C.isf + 1)]) >= 0) KT(c) += wsf * sfeps_bar;
+    }
+  }
+  if (cloud) {
+    int c;
+    if ((c = xm[JUR_XMAP_CLK(C.icl)]) >= 0) KT(c) += (1 - wcl) * clk_bar;
+    if (C.icl + 1 < NCL && (c = xm[JUR_XMAP_CLK(C.icl + 1)]) >= 0) KT(c) += wcl * clk_bar;
+    if ((c = xm[JUR_XMAP_CLZ]) >= 0) KT(c) += clz_bar;
+    if ((c = xm[JUR_XMAP_CLDZ]) >= 0) KT(c) += cldz_bar;
+  }
+#undef EPS_AT
+#undef SCR
+#undef KT
+}
+
+// The reverse sweep over the steps, CH channels per lane (one ray).  With
+// PF the inputs of the next step are loaded while a step is computed (for
+// small batches, where a wavefront's sweep is the kernel's duration).
+template <int CH, bool PF>
+__global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, LosView L, EpsView E,
+                                                   const RayInfo *info, const RayTail *tail, int blk0, int nblk,
+                                                   JacView J) {
+  const int nd = M.nd;
+  const int ngrp = (nd + CH - 1) / CH;
+  int bl, grp;
+  if (!xcd_block_map(blockIdx.x, ngrp, nblk, bl, grp)) return;
+  const int lane = threadIdx.x;
+  const int ir = (blk0 + bl) * WAVE + lane;
+  if (ir >= B.nray) return;
+  const RayInfo R = info[ir];
+  if (R.status != JUR_OK || R.np < 2) return;
+  const int ipf = B.ray_prof[ir];
+  const int im = J.prof_map[ipf];
+  if (im < 0 || im >= J.nmap) return;
+  const int *xm = J.xmap + (size_t)im * JUR_XMAP_LEN;
+  const atm_t *atm = &B.atm[ipf];
+  const int id0 = grp * CH;
+  const int nc = nd - id0 < CH ? nd - id0 : CH;
+  const int npb = E.npb[bl];
+  const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
+  const double *sc0 = J.scr + J.soff[bl] + (size_t)id0 * npb * J.nscr * WAVE + lane;
+  const size_t scstride = (size_t)npb * J.nscr * WAVE;
+#define GZ(c, ip) sc0[(c) * scstride + ((size_t)(ip) * J.nscr + J.kz) * WAVE]
+#define GD(c, ip) sc0[(c) * scstride + ((size_t)(ip) * J.nscr + J.kz + 1) * WAVE]
+  const double *pc = J.pco + J.soff[bl] / ((size_t)nd * J.nscr) * JUR_NPCO + lane;
+  const RayTail T = tail[ir];
+  const int N = R.np;
+  PathRev<CH> P;
+  P.xm = xm;
+  P.nlev = atm->np;
+  P.nc = nc;
+  P.kt0 = J.kt + ((size_t)bl * nd + id0) * J.ncol * WAVE + lane;
+  P.ktstride = (size_t)J.ncol * WAVE;
+  {
+    double xl[3], gz[CH], gdl[CH], gdm[CH];
+    for (int i = 0; i < 3; i++) xl[i] = LOSR(L.f_xe + i, N - 1);
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      gz[c] = c < nc ? GZ(c, N - 1) : 0.0;
+      gdl[c] = c < nc ? GD(c, N - 1) : 0.0;
+      gdm[c] = c < nc ? GD(c, N - 2) : 0.0;
+    }
+    P.init(M, T, xl, gz, gdl, gdm);
+  }
+  auto load_in = [&](int k, PathStepIn<CH> &in) {
+    const double *o = pc + (size_t)k * JUR_NPCO * WAVE;
+#pragma unroll
+    for (int f = 0; f < JUR_NPCO; f++) in.c[f] = o[(size_t)f * WAVE];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      in.gz[c] = c < nc ? GZ(c, k) : 0.0;
+      in.gd[c] = c < nc ? GD(c, k) : 0.0;
+    }
+  };
+  PathStepIn<CH> cur, nxt;
   load_in(N - 2, cur);
   if (PF && N >= 3) load_in(N - 3, nxt);
-  step(N - 2, cur);
-#pragma unroll
-  for (int c = 0; c < CH; c++)
-    for (int i = 0; i < 3; i++) X[c][i] += Xp[c][i] + gzp[c] * cur.c[PC_XH + i];
+  P.step(N - 2, N, cur.c, cur.gz, cur.gd);
+  P.join_clip(cur.c);
   for (int k = N - 3; k >= 0; k--) {
     if (PF) {
       cur = nxt;
@@ -1759,10 +1929,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
     } else {
       load_in(k, cur);
     }
-    step(k, cur);
+    P.step(k, N, cur.c, cur.gz, cur.gd);
   }
-  if (wb >= 0)
-    for (int j = 0; j < 3; j++) flush_level(j);
+  P.finish();
 #undef GZ
 #undef GD
 }
@@ -1819,8 +1988,30 @@ static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L,
     if (nchunk > 0)
       hipLaunchKernelGGL(k_path_coef, dim3(nb8 * nchunk), dim3(WAVE), 0, s, M, B, L, info, blk0, nblk, nchunk, J);
   }
-  hipLaunchKernelGGL((k_adj_acc<SF, PATH>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info, blk0, nblk,
-                     J, rad, tau);
+  // small batches (at most two wavefronts per SIMD), FP32 EGA with stored
+  // partials and at most four emitters: the emitter chains and the path
+  // adjoint run in the accumulation's reverse sweep, one pass instead of
+  // three that each last as long as the longest ray's sweep
+  bool fused = false;
+  if constexpr (!CGA && sizeof(SF) == sizeof(float)) {
+    if (E.part && M.ng >= 1 && M.ng <= 4 && (size_t)nb8 * M.nd < 2048) {
+      const dim3 grow(nb8 * M.nd), blk(WAVE);
+      const RayTail *tl = (const RayTail *)tail;
+      if (M.ng <= 2)
+        hipLaunchKernelGGL((k_adj_acc<SF, PATH, 2, PATH>), grow, blk, 0, s, M, B, L, E, info, tl, blk0, nblk, J,
+                           rad, tau);
+      else if (M.ng == 3)
+        hipLaunchKernelGGL((k_adj_acc<SF, PATH, 3, PATH>), grow, blk, 0, s, M, B, L, E, info, tl, blk0, nblk, J,
+                           rad, tau);
+      else
+        hipLaunchKernelGGL((k_adj_acc<SF, PATH, 4, PATH>), grow, blk, 0, s, M, B, L, E, info, tl, blk0, nblk, J,
+                           rad, tau);
+      fused = true;
+    }
+  }
+  if (fused) return;
+  hipLaunchKernelGGL((k_adj_acc<SF, PATH>), dim3(nb8 * M.nd), dim3(WAVE), 0, s, M, B, L, E, info,
+                     (const RayTail *)tail, blk0, nblk, J, rad, tau);
   if (M.ng > 0) {
     const dim3 grid(nb8 * M.nd * M.ng), blk(WAVE);
     bool fast = false;
