@@ -684,14 +684,30 @@ def serving_bench(args, gpu, abi):
             reqs.append(gpu.Server.request(int(kinds[i]), atms[which[i]], geo[key], rett=(-100.0, 100.0),
                                            retq=retq))
     srv = gpu.Server(ctl, ret, precision=abi.JUR_FP32, lanes=args.serve_lanes,
-                     forward_lane=1 if args.serve_forward_lane else 0)
-    # warm: tables, kernels, allocations
-    for kind in (abi.JUR_REQ_FORWARD, abi.JUR_REQ_JACOBIAN, abi.JUR_REQ_RETRIEVAL):
-        i = srv.submit(gpu.Server.request(kind, apr if kind == abi.JUR_REQ_RETRIEVAL else atms[0],
-                                          meas["limb", 8] if kind == abi.JUR_REQ_RETRIEVAL else geo["limb", 8],
-                                          rett=(-100.0, 100.0)))
-        srv.wait(i, 300.0)
-        srv.release(i)
+                     forward_lane=1 if args.serve_forward_lane else 0,
+                     path_split=1 if args.serve_path_split else 0)
+    # warm: tables, kernels, allocations (every lane: both geometries), then
+    # the first serve_warm seconds of the schedule, untimed, so that the work
+    # buffers have grown to their steady-state sizes
+    for g in ("limb", "nadir"):
+        for kind in (abi.JUR_REQ_FORWARD, abi.JUR_REQ_JACOBIAN, abi.JUR_REQ_RETRIEVAL):
+            i = srv.submit(gpu.Server.request(kind, apr if kind == abi.JUR_REQ_RETRIEVAL else atms[0],
+                                              meas[g, 8] if kind == abi.JUR_REQ_RETRIEVAL else geo[g, 8],
+                                              rett=(-100.0, 100.0)))
+            srv.wait(i, 300.0)
+            srv.release(i)
+    nwarm = int(np.searchsorted(t_arr, args.serve_warm))
+    if nwarm:
+        t0 = time.perf_counter()
+        wid = []
+        for i in range(nwarm):
+            dt = t_arr[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            wid.append(srv.submit(reqs[i]))
+        for i in wid:
+            srv.wait(i, 600.0)
+            srv.release(i)
     r0 = srv.stats()[0]
     log("serving: %d requests over %.1f s (%.0f/s), %d forward, %d Jacobian, %d retrieval"
         % (nreq, t_arr[-1], args.serve_rate, (kinds == 0).sum(), (kinds == 1).sum(), (kinds == 2).sum()))
@@ -750,6 +766,7 @@ def serving_bench(args, gpu, abi):
         "offered_req_per_s": nreq / t_arr[-1],
         "submit_lag_max_s": lag[0],
         "device_rounds": rounds - r0, "max_in_flight": max_active, "lanes": args.serve_lanes, "forward_lane": bool(args.serve_forward_lane),
+        "path_split": bool(args.serve_path_split), "warm_s": args.serve_warm,
         "latency_ms": {"p50": float(np.percentile(lat[ok], 50) * 1e3), "p99": float(np.percentile(lat[ok], 99) * 1e3),
                        "max": float(lat[ok].max() * 1e3)},
         "wall_s": wall,
@@ -762,6 +779,13 @@ def serving_bench(args, gpu, abi):
             out["latency_ms_" + name] = {"p50": float(np.percentile(lat[sel], 50) * 1e3),
                                          "p99": float(np.percentile(lat[sel], 99) * 1e3),
                                          "count": int(sel.sum())}
+    for g in ("limb", "nadir"):
+        for k, name in ((0, "forward"), (2, "retrieval")):
+            sel = ok & (kinds == k) & (geos == g)
+            if sel.any():
+                out["latency_ms_%s_%s" % (name, g)] = {"p50": float(np.percentile(lat[sel], 50) * 1e3),
+                                                       "p99": float(np.percentile(lat[sel], 99) * 1e3),
+                                                       "count": int(sel.sum())}
     sel = ok & (kinds == 2)
     if sel.any():
         out["retrieval_mean_iterations"] = float(niter[sel].mean())
@@ -790,9 +814,13 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-warm", type=float, default=1.0,
+                    help="seconds of the request schedule run untimed before the timed run")
     ap.add_argument("--serve-lanes", type=int, default=1)
     ap.add_argument("--serve-forward-lane", action="store_true",
                     help="one more server lane for forward requests")
+    ap.add_argument("--serve-path-split", type=int, default=1,
+                    help="1: one server lane for the requests with limb rays, one for the others")
     ap.add_argument("--jac-only", action="store_true", help="skip the forward-model section")
     ap.add_argument("--jac-nprof", type=int, default=NJAC_PROF)
     ap.add_argument("--jac-steps", type=int, default=2)

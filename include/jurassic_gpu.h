@@ -365,6 +365,13 @@ typedef struct {
                              requests (forward pass, no Jacobian-mode
                              sweep); the other lanes then take Jacobians
                              and retrievals */
+  int path_split;         /* 1: one more lane that takes the requests with
+                             no limb ray (nadir and slant views that end on
+                             the ground: a few hundred LOS points); the
+                             other lanes take those with limb rays (up to
+                             some 2000 points), so that the short requests'
+                             rounds do not last as long as the longest
+                             limb ray's sweeps.  Not with forward_lane. */
 } jur_server_opts_t;
 
 typedef struct {

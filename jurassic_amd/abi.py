@@ -69,7 +69,8 @@ JUR_ERR_TIMEOUT = 9
 
 class server_opts_t(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("max_bytes", C.c_size_t),
-                ("max_batch", C.c_int), ("lanes", C.c_int), ("forward_lane", C.c_int)]
+                ("max_batch", C.c_int), ("lanes", C.c_int), ("forward_lane", C.c_int),
+                ("path_split", C.c_int)]
 
 
 class request_t(C.Structure):

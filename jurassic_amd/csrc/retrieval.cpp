@@ -115,6 +115,7 @@ struct Job {
   int ktrc = 0;
   std::vector<double> sa_in;  // packed inputs of S_a (8 n)
   int ray0 = 0;
+  bool limb = false;       // a ray of the job crosses a tangent point
   size_t bytes = 0;        // device footprint of the job's own buffers
   size_t admit_bytes = 0;  // what the server reserved for it
   // server bookkeeping
@@ -1108,8 +1109,42 @@ struct jur_lane {
   hipStream_t st = nullptr;
   std::thread th;
   unsigned kinds = 7u;  // request kinds the lane takes (bit JUR_REQ_*)
-  bool takes(const jur::Job *J) const { return kinds >> J->kind & 1u; }
+  unsigned paths = 3u;  // bit 0: requests without limb rays, bit 1: with
+  bool takes(const jur::Job *J) const { return (kinds >> J->kind & 1u) && (paths >> (J->limb ? 1 : 0) & 1u); }
 };
+
+// Does ray ir look through a tangent point?  The straight line from the
+// observer through the view point (geo2cart, src/jurassic.c:3500-3516)
+// comes closest to the Earth's centre ahead of the observer and above the
+// ground: a limb view, whose LOS crosses the atmosphere twice.  Nadir and
+// slant views reach the ground first.
+static bool ray_is_limb(const obs_t *o, int ir) {
+  auto cart = [](double z, double lon, double lat, double *x) {
+    const double r = z + JUR_RE, lo = lon * M_PI / 180, la = lat * M_PI / 180;
+    x[0] = r * cos(la) * cos(lo);
+    x[1] = r * cos(la) * sin(lo);
+    x[2] = r * sin(la);
+  };
+  double xo[3], xv[3], d[3];
+  cart(o->obsz[ir], o->obslon[ir], o->obslat[ir], xo);
+  cart(o->vpz[ir], o->vplon[ir], o->vplat[ir], xv);
+  double n = 0;
+  for (int i = 0; i < 3; i++) {
+    d[i] = xv[i] - xo[i];
+    n += d[i] * d[i];
+  }
+  n = sqrt(n);
+  if (!(n > 0)) return false;
+  double t = 0;
+  for (int i = 0; i < 3; i++) t -= xo[i] * d[i] / n;
+  if (!(t > 0)) return false;
+  double rm = 0;
+  for (int i = 0; i < 3; i++) {
+    const double c = xo[i] + t * d[i] / n;
+    rm += c * c;
+  }
+  return sqrt(rm) > JUR_RE;
+}
 
 struct jur_server {
   std::vector<std::unique_ptr<jur_lane>> lanes;
@@ -1242,7 +1277,11 @@ int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret, const jur_server_o
   // forward_lane: lane 0 takes the forward requests, so that forward
   // models do not wait for Jacobian rounds; the others take the rest
   const bool split = opt && opt->forward_lane;
-  const int nlanes = ((opt && opt->lanes > 0) ? opt->lanes : 1) + (split ? 1 : 0);
+  // path_split: lanes 0 .. lanes - 1 take the requests with limb rays, the
+  // last one the others
+  const bool psplit = opt && opt->path_split;
+  const int nlanes = ((opt && opt->lanes > 0) ? opt->lanes : 1) + (split || psplit ? 1 : 0);
+  if (split && psplit) return set_err(JUR_ERR_ARG, "forward_lane and path_split exclude each other");
   if (method != JUR_JAC_FD && method != JUR_JAC_ADJOINT) return set_err(JUR_ERR_ARG, "invalid method");
   if (precision != JUR_FP64 && precision != JUR_FP32) return set_err(JUR_ERR_ARG, "invalid precision");
   if (nlanes > 16) return set_err(JUR_ERR_ARG, "at most 16 lanes");
@@ -1273,6 +1312,7 @@ int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret, const jur_server_o
       L->kinds = l == 0 ? 1u << JUR_REQ_FORWARD : (1u << JUR_REQ_JACOBIAN | 1u << JUR_REQ_RETRIEVAL);
       L->E.fwd_pass = l == 0;
     }
+    if (psplit) L->paths = l == nlanes - 1 ? 1u : 2u;
     s->lanes.push_back(std::move(L));
   }
   jur_server *raw = s.get();
@@ -1366,6 +1406,7 @@ int jur_server_submit(jur_server_t *s, const jur_request_t *req, uint64_t *id) {
     J->niter = &J->own_niter;
   }
   J->t_submit = t_submit;
+  for (int ir = 0; ir < J->obs_meas->nr && !J->limb; ir++) J->limb = ray_is_limb(J->obs_meas, ir);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->fatal) return set_err(s->fatal, "server stopped: %s", s->fatal_msg);

@@ -370,10 +370,10 @@ class Server:
     retrievals submitted at any time, batched continuously on the GPU."""
 
     def __init__(self, ctl, ret, method=abi.JUR_JAC_ADJOINT, precision=abi.JUR_FP32, max_bytes=0,
-                 max_batch=0, lanes=0, forward_lane=0):
+                 max_batch=0, lanes=0, forward_lane=0, path_split=0):
         self.ptr = C.c_void_p()
         self.ret = ret
-        opt = abi.server_opts_t(method, precision, max_bytes, max_batch, lanes, forward_lane)
+        opt = abi.server_opts_t(method, precision, max_bytes, max_batch, lanes, forward_lane, path_split)
         _check(lib().jur_server_create(C.byref(ctl), C.byref(ret), C.byref(opt), C.byref(self.ptr)))
 
     @staticmethod

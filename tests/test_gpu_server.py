@@ -56,8 +56,9 @@ def _mix(oracle):
     return ctl, reqs
 
 
-def _run(gpu, ctl, ret, reqs, precision, lanes=0, forward_lane=0):
-    srv = gpu.Server(ctl, ret, precision=precision, lanes=lanes, forward_lane=forward_lane)
+def _run(gpu, ctl, ret, reqs, precision, lanes=0, forward_lane=0, path_split=0):
+    srv = gpu.Server(ctl, ret, precision=precision, lanes=lanes, forward_lane=forward_lane,
+                     path_split=path_split)
     try:
         keep = []
         ids = []
@@ -133,13 +134,14 @@ def test_server_matches_direct_calls(gpu, oracle):
 def test_server_forward_lane(gpu, oracle):
     """With a forward lane, forward requests take the forward pass (the
     kernels of formod: equal to the last bit), and Jacobians and retrievals
-    are those of the one-lane server; so are they with two general lanes."""
+    are those of the one-lane server; so are they with two general lanes,
+    and with the limb requests and the others on lanes of their own."""
     ctl, reqs = _mix(oracle)
     ret = _ret(ctl.nd)
     prec = abi.JUR_FP64
     base, _ = _run(gpu, ctl, ret, reqs, prec)
-    for lanes, fl in ((1, 1), (2, 0)):
-        got, (rounds, _) = _run(gpu, ctl, ret, reqs, prec, lanes=lanes, forward_lane=fl)
+    for lanes, fl, ps in ((1, 1, 0), (2, 0, 0), (1, 0, 1)):
+        got, (rounds, _) = _run(gpu, ctl, ret, reqs, prec, lanes=lanes, forward_lane=fl, path_split=ps)
         assert rounds >= 1
         for r, (res, (obs, atm, k, hist)), (res0, (obs0, atm0, k0, hist0)) in zip(reqs, got, base):
             if r["kind"] == abi.JUR_REQ_FORWARD and fl:
