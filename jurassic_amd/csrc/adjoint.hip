@@ -1081,7 +1081,9 @@ __device__ __forceinline__ void march_step(const ModelView &M, const atm_t *atm,
 // PC_LO its level pair (8 bits per point).
 constexpr int PC_HDS = 0, PC_E = 1, PC_E1 = 4, PC_RNV = 7, PC_N = 8, PC_NGM = 9, PC_MX = 12, PC_SK = 21,
               PC_XH = 22, PC_A = 25, PC_B = 28, PC_LB = 31, PC_TAU = 32, PC_PI = 41, PC_W = 32, PC_LO = 52;
-static_assert(PC_LO + 1 == JUR_NPCO && NP <= 256, "reverse-step coefficient layout");
+constexpr int PC_USED = PC_LO + 1;  // coefficients in use (JUR_NPCO: the stride)
+static_assert(PC_USED <= JUR_NPCO && JUR_NPCO * WAVE * 8 % 1024 == 0 && NP <= 256,
+              "reverse-step coefficient layout");
 constexpr int PCO_STEPS = 16;  // steps per lane of k_path_coef
 
 // The point's and the probes' refractivity adjoints as linear forms of V:
@@ -1094,6 +1096,10 @@ __device__ __forceinline__ void point_forms(const double *e, double f, double c[
   }
 }
 
+// LM (lane-major, for the small-batch sweep that stages each lane's step in
+// LDS): step k of lane l at ((k 64 + l) JUR_NPCO + f); otherwise
+// coefficient-major, (k JUR_NPCO + f) 64 + l.
+template <bool LM>
 __global__ __launch_bounds__(WAVE) void k_path_coef(ModelView M, BatchView B, LosView L, const RayInfo *info,
                                                     int blk0, int nblk, int nchunk, JacView J) {
   int bl, chk;
@@ -1108,7 +1114,7 @@ __global__ __launch_bounds__(WAVE) void k_path_coef(ModelView M, BatchView B, Lo
   if (k0 >= k1) return;
   const atm_t *atm = &B.atm[B.ray_prof[ir]];
   const double *los = L.los + (size_t)bl * L.cap * L.nfield * WAVE + lane;
-  double *pc = J.pco + J.soff[bl] / ((size_t)M.nd * J.nscr) * JUR_NPCO + lane;
+  double *pc = J.pco + J.soff[bl] / ((size_t)M.nd * J.nscr) * JUR_NPCO + (LM ? (size_t)lane * JUR_NPCO : lane);
   int hint = -1;
   for (int k = k0; k < k1; k++) {
     double x[3], e[3];
@@ -1119,7 +1125,7 @@ __global__ __launch_bounds__(WAVE) void k_path_coef(ModelView M, BatchView B, Lo
     MarchStep S;
     march_step(M, atm, x, e, hint, S);
     double *o = pc + (size_t)k * JUR_NPCO * WAVE;
-#define PCW(f, v) o[(size_t)(f) * WAVE] = (v)
+#define PCW(f, v) o[LM ? (size_t)(f) : (size_t)(f) * WAVE] = (v)
     PCW(PC_HDS, 0.5 * S.ds);
     PCW(PC_RNV, 1 / S.nv);
     PCW(PC_N, S.n);
@@ -1291,8 +1297,12 @@ struct PathRev {
 
   // The reverse of march step k (of N points) with its coefficients cf and
   // the channels' point values gz = d rad / d z_k, gd = d rad / d ds_k.
+  // LAY 0: coefficient f at cf[f]; LAY 1 (a lane's coefficients staged in
+  // LDS by 16 B direct-to-LDS loads, pair i at i KiB + lane 16 B): at
+  // cf[(f / 2) 128 + f % 2] with cf at the lane's first pair.
+  template <int LAY = 0>
   __device__ __forceinline__ void step(int k, int N, const double *cf, const double *gzv, const double *gdv) {
-#define PCR(f) cf[f]
+#define PCR(f) (LAY == 0 ? cf[f] : cf[((f) >> 1) * 128 + ((f) & 1)])
     const double hds = PCR(PC_HDS), rnv = PCR(PC_RNV), n = PCR(PC_N), sk = PCR(PC_SK);
     double e[3], e1[3], ngm[3], xh[3], a[3], b[3], mx[9];
 #pragma unroll
@@ -1403,10 +1413,14 @@ struct PathRev {
 
   // after step N-2 (which leads to the cut point): the clip's dependence on
   // x_N-2 joins its adjoint (cf: step N-2's coefficients)
+  template <int LAY = 0>
   __device__ __forceinline__ void join_clip(const double *cf) {
 #pragma unroll
     for (int c = 0; c < CH; c++)
-      for (int i = 0; i < 3; i++) X[c][i] += Xp[c][i] + gzp[c] * cf[PC_XH + i];
+      for (int i = 0; i < 3; i++) {
+        const int f = PC_XH + i;
+        X[c][i] += Xp[c][i] + gzp[c] * (LAY == 0 ? cf[f] : cf[(f >> 1) * 128 + (f & 1)]);
+      }
   }
 
   __device__ __forceinline__ void finish() {
@@ -1418,7 +1432,7 @@ struct PathRev {
 // A step's inputs: its coefficients and the channels' point values.
 template <int CH>
 struct PathStepIn {
-  double c[JUR_NPCO];
+  double c[PC_USED];
   double gz[CH], gd[CH];
 };
 
@@ -1669,23 +1683,42 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
     // the path's reverse march (PFU): its state, the held values of the last
     // point, and the next step's coefficients loaded one point ahead
     const int N = R.np;
-    constexpr int PC1 = PFU ? JUR_NPCO : 1;
     PathRev<1> P;
-    double pcn[PC1];
     bool pstart = false;
     double gz_last = 0, gd_last = 0;
-    const double *pcf = nullptr;
+    // the next step's coefficients are staged in LDS (one wavefront per
+    // workgroup: 27 KiB), copied by the wavefront's direct-to-LDS loads
+    // while the next point's accumulation runs, instead of in registers
+    const char *pcg = nullptr;
+    double *pcs = nullptr;
+    // (lane-major coefficients: each lane copies its own step's 432 B; the
+    // lanes of a wavefront are at different steps)
+    auto pc_fetch = [&](int k) {
+      if constexpr (PFU) {
+        // the reads of the last step's coefficients have completed before
+        // the copy overwrites them; the memory clobber keeps the compiler
+        // from moving those reads below the copy, whose LDS write it does
+        // not see
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        const char *g = pcg + ((size_t)k * WAVE + lane) * JUR_NPCO * sizeof(double);
+#pragma unroll
+        for (int i = 0; i < JUR_NPCO / 2; i++)
+          __builtin_amdgcn_global_load_lds((const void *)(g + i * 16),
+                                           (__attribute__((address_space(3))) void *)((char *)pcs + i * 1024),
+                                           16, 0, 0);
+      }
+    };
     if constexpr (PFU) {
+      __shared__ double pc_lds[JUR_NPCO * WAVE];
+      pcs = pc_lds;
       P.xm = xm;
       P.nlev = atm->np;
       P.nc = 1;
       P.kt0 = kt;
       P.ktstride = (size_t)J.ncol * WAVE;
-      pcf = J.pco + J.soff[bl] / ((size_t)nd * J.nscr) * JUR_NPCO + lane;
+      pcg = (const char *)(J.pco + J.soff[bl] / ((size_t)nd * J.nscr) * JUR_NPCO);
       const int k0 = nef - 1 < N - 2 ? nef - 1 : N - 2;
-      if (k0 >= 0)
-#pragma unroll
-        for (int f = 0; f < JUR_NPCO; f++) pcn[f] = pcf[((size_t)k0 * JUR_NPCO + f) * WAVE];
+      if (k0 >= 0) pc_fetch(k0);
     }
     // point ip's inputs and scratch, and the emitters' values at ip - 1,
     // loaded one iteration ahead
@@ -1797,13 +1830,16 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
               P.init(M, tail[ir], xl, &gz_last, &gd_last, &gdm);
               pstart = true;
             }
-            P.step(ip, N, pcn, &gzc, &gdc);
-            if (ip == N - 2) P.join_clip(pcn);
+            // the copy of this step's coefficients has landed in LDS: the
+            // direct-to-LDS loads count as vector memory loads, which the
+            // compiler does not track for LDS reads (the memory clobber
+            // keeps the reads below the wait)
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            P.template step<1>(ip, N, pcs + 2 * lane, &gzc, &gdc);
+            if (ip == N - 2) P.template join_clip<1>(pcs + 2 * lane);
             // the next step's coefficients, in flight during the next
             // point's accumulation
-            if (ip > 0)
-#pragma unroll
-              for (int f = 0; f < JUR_NPCO; f++) pcn[f] = pcf[((size_t)(ip - 1) * JUR_NPCO + f) * WAVE];
+            if (ip > 0) pc_fetch(ip - 1);
           }
         } else {
           SCR(J.kz, ip) = gzc;
@@ -1910,7 +1946,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_path(ModelView M, BatchView B, Los
   auto load_in = [&](int k, PathStepIn<CH> &in) {
     const double *o = pc + (size_t)k * JUR_NPCO * WAVE;
 #pragma unroll
-    for (int f = 0; f < JUR_NPCO; f++) in.c[f] = o[(size_t)f * WAVE];
+    for (int f = 0; f < PC_USED; f++) in.c[f] = o[(size_t)f * WAVE];
 #pragma unroll
     for (int c = 0; c < CH; c++) {
       in.gz[c] = c < nc ? GZ(c, k) : 0.0;
@@ -1982,19 +2018,29 @@ static void launch_adj(const ModelView &M, const BatchView &B, const LosView &L,
                        const RayInfo *info, const void *tail, int blk0, int nblk, int maxnp, const JacView &J,
                        double *rad, double *tau, hipStream_t s) {
   const int nb8 = padded_blocks(nblk);
-  if (PATH) {
-    // reverse-step coefficients of steps 0 .. np - 2
-    const int nchunk = (maxnp - 1 + PCO_STEPS - 1) / PCO_STEPS;
-    if (nchunk > 0)
-      hipLaunchKernelGGL(k_path_coef, dim3(nb8 * nchunk), dim3(WAVE), 0, s, M, B, L, info, blk0, nblk, nchunk, J);
-  }
   // small batches (at most two wavefronts per SIMD), FP32 EGA with stored
   // partials and at most four emitters: the emitter chains and the path
   // adjoint run in the accumulation's reverse sweep, one pass instead of
   // three that each last as long as the longest ray's sweep
+  bool one = false;
+  if constexpr (!CGA && sizeof(SF) == sizeof(float))
+    one = E.part && M.ng >= 1 && M.ng <= 4 && (size_t)nb8 * M.nd < 2048;
+  if (PATH) {
+    // reverse-step coefficients of steps 0 .. np - 2 (lane-major for the
+    // one-sweep form)
+    const int nchunk = (maxnp - 1 + PCO_STEPS - 1) / PCO_STEPS;
+    if (nchunk > 0) {
+      if (one)
+        hipLaunchKernelGGL(k_path_coef<true>, dim3(nb8 * nchunk), dim3(WAVE), 0, s, M, B, L, info, blk0, nblk,
+                           nchunk, J);
+      else
+        hipLaunchKernelGGL(k_path_coef<false>, dim3(nb8 * nchunk), dim3(WAVE), 0, s, M, B, L, info, blk0, nblk,
+                           nchunk, J);
+    }
+  }
   bool fused = false;
   if constexpr (!CGA && sizeof(SF) == sizeof(float)) {
-    if (E.part && M.ng >= 1 && M.ng <= 4 && (size_t)nb8 * M.nd < 2048) {
+    if (one) {
       const dim3 grow(nb8 * M.nd), blk(WAVE);
       const RayTail *tl = (const RayTail *)tail;
       if (M.ng <= 2)

@@ -122,8 +122,10 @@ struct JacView {
                         // [step][coefficient][lane]
 };
 
-// Coefficients per march step of the adjoint of the refracted path.
-constexpr int JUR_NPCO = 53;
+// Coefficients per march step of the adjoint of the refracted path: 53
+// used and one spare, so that a step's block (JUR_NPCO x 64 lanes x 8 B)
+// is a whole number of 1 KiB wavefront loads.
+constexpr int JUR_NPCO = 54;
 
 #define LOS_AT(base, L, f, ip) (base)[((size_t)(ip) * (L).nfield + (size_t)(f)) * WAVE]
 
