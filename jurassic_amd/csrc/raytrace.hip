@@ -86,30 +86,44 @@ __device__ __forceinline__ double sza(double sec, double lon, double lat) {
 // a LOS field and, for the FP32 forward pass, its float copy (of the
 // fields the FP32 radiance kernel reads: the column densities of CO2 and
 // H2O and the H2O mixing ratio among the per-gas fields)
-#define LOSW3(f, ip, v, copy)                                  \
-  do {                                                         \
-    const double v_ = (v);                                     \
-    LOSW(f, ip) = v_;                                          \
-    if (basef && (copy)) LOS_AT(basef, L, f, ip) = (float)v_;  \
+// (wr: the lane writes; with several lanes per ray only the first)
+#define LOSW3(f, ip, v, copy)                                    \
+  do {                                                           \
+    const double v_ = (v);                                       \
+    if (wr) {                                                    \
+      LOSW(f, ip) = v_;                                          \
+      if (basef && (copy)) LOS_AT(basef, L, f, ip) = (float)v_;  \
+    }                                                            \
   } while (0)
 #define LOSW2(f, ip, v) LOSW3(f, ip, v, true)
 
+// RL lanes per ray.  RL = 4 (small batches, where a wavefront's march is the
+// kernel's duration): the four lanes of a ray march it together and each
+// evaluates one of the four refraction probes of a step, which the lanes
+// then exchange; every lane does the same operations on the same values, so
+// the LOS is the same to the last bit, and the first lane writes it.  A
+// wavefront then covers a quarter of a 64-ray block.
+template <int RL>
 __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, LosView L,
                                                    RayInfo *info, RayTail *tail, int blk0) {
   JUR_FP_EXACT
   const int lane = threadIdx.x;
-  const int blk = blk0 + blockIdx.x;
-  const int ir = blk * WAVE + lane;
+  const int lb = blockIdx.x / RL;  // ray block of the chunk
+  const int rib = RL == 1 ? lane : (blockIdx.x % RL) * (WAVE / RL) + lane / RL;  // ray in the block
+  const int sub = RL == 1 ? 0 : lane % RL;
+  const bool wr = sub == 0;
+  const int blk = blk0 + lb;
+  const int ir = blk * WAVE + rib;
   if (ir >= B.nray) return;
-  double *base = L.los + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane;
-  float *basef = L.losf ? L.losf + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane : nullptr;
+  double *base = L.los + (size_t)lb * L.cap * L.nfield * WAVE + rib;
+  float *basef = L.losf ? L.losf + (size_t)lb * L.cap * L.nfield * WAVE + rib : nullptr;
 
   const double *g = B.ray_geom + (size_t)7 * ir;
   const int iprof = B.ray_prof[ir];
   if (iprof < 0 || iprof >= B.nprof || B.atm[iprof].np < 2 || B.atm[iprof].np > NP) {
     RayInfo E{};
     E.status = JUR_ERR_ARG;
-    info[ir] = E;
+    if (wr) info[ir] = E;
     return;
   }
   const atm_t *atm = &B.atm[iprof];
@@ -168,11 +182,11 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
 
   if (obsz < zmin) {
     R.status = JUR_ERR_OBSERVER;
-    info[ir] = R;
+    if (wr) info[ir] = R;
     return;
   }
   if (vpz > zmax) {
-    info[ir] = R;
+    if (wr) info[ir] = R;
     return;
   }
 
@@ -274,7 +288,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       // capacity exceeded: the reference limit NLOS -> error, a smaller
       // working capacity -> the host retries with NLOS
       R.status = (L.cap >= NLOS) ? JUR_ERR_LOS : 100;
-      info[ir] = R;
+      if (wr) info[ir] = R;
       return;
     }
     // the trapezoid segment length of the point and its column densities
@@ -283,7 +297,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     // clip of the last step changes the last two points, which k_ray_finish
     // rewrites)
     const double dtr = np == 0 ? ds * 0.5 : 0.5 * (ds1 + ds);
-    LOSW(F_Z, np) = z;
+    LOSW3(F_Z, np, z, false);
     LOSW2(F_P, np, p);
     LOSW2(F_T, np, t);
     const double ucol = 10 * p / (JUR_KB * t) * dtr;  // column density per unit vmr
@@ -303,8 +317,8 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     }
     if (L.f_xe >= 0)
       for (int i = 0; i < 3; i++) {
-        LOSW(L.f_xe + i, np) = x[i];
-        LOSW(L.f_xe + 3 + i, np) = ex0[i];
+        LOSW3(L.f_xe + i, np, x[i], false);
+        LOSW3(L.f_xe + 3 + i, np, ex0[i], false);
       }
     // MinTrack::add, with the raw step lengths at the minimum and after it
     if (!nan_seen) {
@@ -312,24 +326,30 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
       if (lower) zlow = z;
       if (lower || isnan(z)) {
         imin = np;
-        tl->ds_imin = ds;
-        if (np > 0)
-          for (int i = 0; i < 3; i++) tl->mt.xbefore[i] = xprev[i];
+        if (wr) {
+          tl->ds_imin = ds;
+          if (np > 0)
+            for (int i = 0; i < 3; i++) tl->mt.xbefore[i] = xprev[i];
+        }
         after_pending = true;
         nan_seen = isnan(z);
       } else if (after_pending) {
-        for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
-        tl->ds_imin1 = ds;
+        if (wr) {
+          for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
+          tl->ds_imin1 = ds;
+        }
         after_pending = false;
       }
     } else if (after_pending) {
-      for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
-      tl->ds_imin1 = ds;
+      if (wr) {
+        for (int i = 0; i < 3; i++) tl->mt.xafter[i] = x[i];
+        tl->ds_imin1 = ds;
+      }
       after_pending = false;
     }
     ds2 = ds1;
     ds1 = ds;
-    if (np == 0)
+    if (np == 0 && wr)
       for (int i = 0; i < 3; i++) tl->xfirst[i] = x[i];
     for (int i = 0; i < 3; i++) xprev[i] = x[i];
     zprev = z;
@@ -340,7 +360,23 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
 
     for (int i = 0; i < 3; i++) ex1[i] = ex0[i] * n;
 
-    if (M.refrac && z <= zrefrac) {
+    if (RL == 4 && M.refrac && z <= zrefrac) {
+      // lane sub's probe: the midpoint (0) or the midpoint displaced by h
+      // along axis sub - 1, after the displacements of the axes before it
+      // were made and undone as in the serial order below
+      for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
+      for (int i = 0; i + 1 < sub; i++) {
+        xh[i] += h;
+        xh[i] -= h;
+      }
+      if (sub > 0) xh[sub - 1] += h;
+      double pq, tq;
+      int zg2 = zg;
+      intpol_pt(atm, anp, zdir, zg2, lc, cart2z(xh), &pq, &tq);
+      const double rq = 7.753e-05 * pq / tq;
+      n = __shfl(rq, 0, RL);
+      for (int i = 0; i < 3; i++) ngr[i] = (__shfl(rq, i + 1, RL) - n) * (1 / h);
+    } else if (M.refrac && z <= zrefrac) {
       // the refractivity at the midpoint and at the three probes displaced
       // by h along each axis, in the reference's order (each displacement
       // is undone by subtracting h before the next axis moves)
@@ -383,6 +419,7 @@ __global__ __launch_bounds__(WAVE) void k_raytrace(ModelView M, BatchView B, Los
     for (int i = 0; i < 3; i++) ex0[i] = ex1[i];
   }
   R.np = np;
+  if (!wr) return;
   if (np > 0) {
     for (int i = 0; i < 3; i++) {
       tl->xprev[i] = xprev[i];
@@ -416,6 +453,7 @@ __global__ __launch_bounds__(WAVE) void k_ray_finish(ModelView M, BatchView B, L
   const int lane = threadIdx.x;
   const int ir = (blk0 + blockIdx.x) * WAVE + lane;
   if (ir >= B.nray) return;
+  constexpr bool wr = true;
   RayInfo R = info[ir];
   if (R.status != JUR_OK || R.np == 0) return;
   double *base = L.los + (size_t)blockIdx.x * L.cap * L.nfield * WAVE + lane;
@@ -595,7 +633,11 @@ size_t ray_tail_bytes() { return sizeof(RayTail); }
 void launch_raytrace(const ModelView &M, const BatchView &B, const LosView &L, RayInfo *info, void *tail,
                      int blk0, int nblk, hipStream_t s) {
   const size_t lds = (size_t)(6 + 2 * M.ng + 2 * M.nw) * WAVE * sizeof(double);
-  hipLaunchKernelGGL(k_raytrace, dim3(nblk), dim3(WAVE), lds, s, M, B, L, info, (RayTail *)tail, blk0);
+  // small batches: four lanes per ray, a quarter block per wavefront
+  if (nblk <= 512)
+    hipLaunchKernelGGL(k_raytrace<4>, dim3(4 * nblk), dim3(WAVE), lds, s, M, B, L, info, (RayTail *)tail, blk0);
+  else
+    hipLaunchKernelGGL(k_raytrace<1>, dim3(nblk), dim3(WAVE), lds, s, M, B, L, info, (RayTail *)tail, blk0);
   hipLaunchKernelGGL(k_ray_finish, dim3(nblk), dim3(WAVE), 0, s, M, B, L, info, (const RayTail *)tail, blk0);
 }
 

@@ -411,3 +411,45 @@ def test_fp32_bt_bound_limb_at_scale(gpu, oracle):
     assert np.isfinite(err).all()
     print("FP32 vs FP64 over %d radiances: max %.2e K, mean %.2e K" % (err.size, err.max(), err.mean()))
     assert err.max() <= BT_TOL32
+
+
+def test_raytrace_lanes_per_ray_agree(gpu):
+    """The ray tracer marches a ray on one lane for large batches and on four
+    (one refraction probe each, exchanged by shuffles) for small ones, with
+    the same operations on the same values: the FP64 radiances of the same
+    rays are equal to the last bit between a batch of 1100 limb atmospheres
+    x 32 tangent heights (550 ray blocks, one lane per ray) and one of its
+    first 64 atmospheres (four lanes per ray)."""
+    torch = pytest.importorskip("torch")
+    ctl = scenes.limb_3gas_ctl()
+    assert ctl.refrac == 1
+    zt = scenes.limb_tangent_heights()
+    geom = scenes.limb_geometry(800.0, zt)
+    nr = len(zt)
+    gases = [ctl.emitter[i].value.decode() for i in range(ctl.ng)]
+    dev = torch.device("cuda:0")
+    out = []
+    for nprof in (1100, 64):
+        atms = (abi.atm_t * nprof)()
+        for k in range(nprof):
+            atms[k] = scenes.make_profile_atm(gases, iprof=k)
+        nray = nprof * nr
+        prof = np.tile(np.arange(nprof, dtype=np.int32), nr)
+        g = np.zeros((nray, 7))
+        for j in range(nr):
+            for c, key in enumerate(("obsz", "obslon", "obslat", "vpz", "vplon", "vplat")):
+                g[j * nprof:(j + 1) * nprof, 1 + c] = geom[key][j]
+        a_d = torch.from_numpy(np.frombuffer(bytes(atms), dtype=np.uint8).copy()).to(dev)
+        p_d, g_d = torch.from_numpy(prof).to(dev), torch.from_numpy(g).to(dev)
+        ctx = gpu.Context(ctl)
+        try:
+            rad = torch.zeros(nray, ctl.nd, dtype=torch.float64, device=dev)
+            tau = torch.zeros_like(rad)
+            ctx.formod_device(nprof, a_d.data_ptr(), nray, p_d.data_ptr(), g_d.data_ptr(), rad.data_ptr(),
+                              tau.data_ptr(), precision=abi.JUR_FP64)
+            torch.cuda.synchronize()
+            out.append(rad.cpu().numpy().reshape(nr, nprof, ctl.nd)[:, :64])
+        finally:
+            ctx.close()
+    assert np.isfinite(out[0]).all() and (out[0] > 0).all()
+    assert np.array_equal(out[0], out[1])
