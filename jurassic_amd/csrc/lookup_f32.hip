@@ -26,19 +26,32 @@
 namespace jur {
 
 // LOS field f of point ip: wave-uniform block base, 32-bit lane offset
-#define LOSR(f, ip) ld32(losb, ((unsigned)(ip) * nfield + (unsigned)(f)) * WAVE + lane)
+#define LOSR(f, ip) ld32(losb, ((unsigned)(ip) * nfield + (unsigned)(f)) * WAVE + ln)
 
-template <bool CGA, bool PART>
+// RL lanes per chain.  RL = 4 (small batches, where a wavefront's chain is
+// the kernel's duration; EGA): the four lanes of a (ray, channel, emitter)
+// chain each take one corner of the cell's step and exchange the corners'
+// results (and bracket indices) by shuffles; every lane then does the same
+// operations on the same values as one lane would, so the emissivities are
+// the same to the last bit, and the first lane writes them.  A wavefront
+// covers a quarter of a 64-ray block.
+template <bool CGA, bool PART, int RL = 1>
 __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, LosView L,
                                                      EpsView E, const RayInfo *info, int blk0,
                                                      int nblk) {
+  static_assert(RL == 1 || !CGA, "the four-lane chains are the EGA ones");
   const int nd = M.nd, ng = M.ng;
   int bl, rem;
-  if (!xcd_block_map(blockIdx.x, nd * ng, nblk, bl, rem)) return;
+  if (!xcd_block_map(blockIdx.x, nd * ng * RL, nblk, bl, rem)) return;
+  const int qtr = rem % RL;
+  rem /= RL;
   const int id = rem / ng;
   const int ig = rem - id * ng;
   const int lane = threadIdx.x;
-  const int ir = (blk0 + bl) * WAVE + lane;
+  const int sub = lane % RL;  // the lane's corner (RL = 4)
+  const int ln = RL == 1 ? lane : qtr * (WAVE / RL) + lane / RL;  // the ray's lane in its block
+  const bool wr = sub == 0;
+  const int ir = (blk0 + bl) * WAVE + ln;
   if (ir >= B.nray) return;
   const TabView &T = M.tab;
   const int tix = ig * nd + id;
@@ -189,7 +202,14 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
         const float us = (float)useg;
         float ec[4], dep[4] = {0.f, 0.f, 0.f, 0.f}, deu[4] = {0.f, 0.f, 0.f, 0.f};
         if (CGA || !mono) {
-          if (!mono) {
+          if (RL > 1 && !mono) {
+            const int c = sub;
+            const float4 *kcc = c == 0 ? kc[0] : (c == 1 ? kc[1] : (c == 2 ? kc[2] : kc[3]));
+            const int ncc = c == 0 ? nc[0] : (c == 1 ? nc[1] : (c == 2 ? nc[2] : nc[3]));
+            const float e = col_eps_bis(kcc, ncc, col_u_bis(kcc, ncc, ep) + useg);
+#pragma unroll
+            for (int k = 0; k < 4; k++) ec[k] = __shfl(e, k, RL);
+          } else if (!mono) {
 #pragma unroll 1
             for (int c = 0; c < 4; c++)
               ec[c] = CGA ? col_eps_bis(kc[c], nc[c], cu)
@@ -205,6 +225,40 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
           pn = LOSR(F_P, ipn);
           tn = LOSR(F_T, ipn);
           un = LOSR(L.f_u + ig, ipn);
+        } else if constexpr (RL > 1) {
+          // the lane's corner: its bracket records, the step, and the
+          // corners' results gathered from the quad
+          const int c = sub;
+          const float4 *kcc = c == 0 ? kc[0] : (c == 1 ? kc[1] : (c == 2 ? kc[2] : kc[3]));
+          const int ncc = c == 0 ? nc[0] : (c == 1 ? nc[1] : (c == 2 ? nc[2] : nc[3]));
+          const unsigned koc = c == 0 ? ko[0] : (c == 1 ? ko[1] : (c == 2 ? ko[2] : ko[3]));
+          int g = c == 0 ? gi[0] : (c == 1 ? gi[1] : (c == 2 ? gi[2] : gi[3]));
+          g = g < 0 ? 0 : (g > ncc - 2 ? ncc - 2 : g);
+          const float4 r = ld32(recb, koc + g);
+          const float4 rn = ld32(recb, koc + (g + 1 < ncc - 2 ? g + 1 : ncc - 2));
+          __builtin_amdgcn_sched_barrier(0);
+          pn = LOSR(F_P, ipn);
+          tn = LOSR(F_T, ipn);
+          un = LOSR(L.f_u + ig, ipn);
+          __builtin_amdgcn_sched_barrier(0);
+          float e, de = 0.f, du = 0.f;
+          bool slow;
+          if constexpr (PART) {
+            ega_step_fast2_d(r, rn, ncc, ep, us, g, e, de, du, slow);
+            if (slow) e = ega_corner_f32_d(kcc, ncc, ep, useg, g, de, du);
+          } else {
+            ega_step_fast2(r, rn, ncc, ep, us, useg, g, e, slow);
+            if (slow) e = ega_step_slow(kcc, ncc, ep, useg, g, r);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            ec[k] = __shfl(e, k, RL);
+            gi[k] = __shfl(g, k, RL);
+            if constexpr (PART) {
+              dep[k] = __shfl(de, k, RL);
+              deu[k] = __shfl(du, k, RL);
+            }
+          }
         } else {
           // the bracket records at the previous segment's index and the next
           float4 r[4], rn[4];
@@ -283,11 +337,14 @@ __global__ __launch_bounds__(WAVE) void k_lookup_f32(ModelView M, BatchView B, L
       tn = LOSR(F_T, ipn);
       un = LOSR(L.f_u + ig, ipn);
     }
-    st32(outb, (unsigned)ip * WAVE + lane, ep);
-    if (PART) st32(partb, (unsigned)ip * WAVE + lane, dpart);
+    if (wr) {
+      st32(outb, (unsigned)ip * WAVE + ln, ep);
+      if (PART) st32(partb, (unsigned)ip * WAVE + ln, dpart);
+    }
     if (ep >= 1.f) {
       // tau_path below float resolution: every later segment is opaque too
-      for (ip++; ip < R.np; ip++) st32(outb, (unsigned)ip * WAVE + lane, 1.f);
+      if (wr)
+        for (ip++; ip < R.np; ip++) st32(outb, (unsigned)ip * WAVE + ln, 1.f);
       break;
     }
     p = pn;
@@ -300,6 +357,16 @@ void launch_lookup_f32(const ModelView &M, const BatchView &B, const LosView &L,
                        const RayInfo *info, int blk0, int nblk, hipStream_t s) {
   if (M.nd * M.ng == 0) return;
   const dim3 grid(padded_blocks(nblk) * M.nd * M.ng), blk(WAVE);
+  // small batches, EGA: four lanes per chain while that stays within the
+  // five wavefronts per SIMD its registers allow
+  if (M.formod != 0 && (size_t)padded_blocks(nblk) * M.nd * M.ng * 4 <= 5120) {
+    const dim3 g4(padded_blocks(nblk) * M.nd * M.ng * 4);
+    if (E.part)
+      hipLaunchKernelGGL((k_lookup_f32<false, true, 4>), g4, blk, 0, s, M, B, L, E, info, blk0, nblk);
+    else
+      hipLaunchKernelGGL((k_lookup_f32<false, false, 4>), g4, blk, 0, s, M, B, L, E, info, blk0, nblk);
+    return;
+  }
   if (M.formod == 0)
     hipLaunchKernelGGL((k_lookup_f32<true, false>), grid, blk, 0, s, M, B, L, E, info, blk0, nblk);
   else if (E.part)

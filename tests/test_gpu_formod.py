@@ -413,43 +413,85 @@ def test_fp32_bt_bound_limb_at_scale(gpu, oracle):
     assert err.max() <= BT_TOL32
 
 
-def test_raytrace_lanes_per_ray_agree(gpu):
-    """The ray tracer marches a ray on one lane for large batches and on four
-    (one refraction probe each, exchanged by shuffles) for small ones, with
-    the same operations on the same values: the FP64 radiances of the same
-    rays are equal to the last bit between a batch of 1100 limb atmospheres
-    x 32 tangent heights (550 ray blocks, one lane per ray) and one of its
-    first 64 atmospheres (four lanes per ray)."""
-    torch = pytest.importorskip("torch")
-    ctl = scenes.limb_3gas_ctl()
-    assert ctl.refrac == 1
+def _limb_batch(torch, ctl, nprof):
     zt = scenes.limb_tangent_heights()
     geom = scenes.limb_geometry(800.0, zt)
     nr = len(zt)
     gases = [ctl.emitter[i].value.decode() for i in range(ctl.ng)]
+    atms = (abi.atm_t * nprof)()
+    for k in range(nprof):
+        atms[k] = scenes.make_profile_atm(gases, iprof=k)
+    nray = nprof * nr
+    prof = np.tile(np.arange(nprof, dtype=np.int32), nr)
+    g = np.zeros((nray, 7))
+    for j in range(nr):
+        for c, key in enumerate(("obsz", "obslon", "obslat", "vpz", "vplon", "vplat")):
+            g[j * nprof:(j + 1) * nprof, 1 + c] = geom[key][j]
     dev = torch.device("cuda:0")
-    out = []
+    a_d = torch.from_numpy(np.frombuffer(bytes(atms), dtype=np.uint8).copy()).to(dev)
+    return atms, a_d, torch.from_numpy(prof).to(dev), torch.from_numpy(g).to(dev), nray, nr
+
+
+def test_raytrace_lanes_per_ray_agree(gpu):
+    """The ray tracer marches a ray on one lane for large batches and on four
+    (one refraction probe each, exchanged by shuffles) for small ones, and
+    the FP32 lookups run a chain on one lane or on four (one cell corner
+    each); both with the same operations on the same values.  The FP64 and
+    FP32 radiances of the same rays are equal to the last bit between a
+    batch of 1100 limb atmospheres x 32 tangent heights (550 ray blocks, one
+    lane per ray and chain) and one of its first 64 atmospheres (four)."""
+    torch = pytest.importorskip("torch")
+    ctl = scenes.limb_3gas_ctl()
+    assert ctl.refrac == 1
+    out = {abi.JUR_FP64: [], abi.JUR_FP32: []}
     for nprof in (1100, 64):
-        atms = (abi.atm_t * nprof)()
-        for k in range(nprof):
-            atms[k] = scenes.make_profile_atm(gases, iprof=k)
-        nray = nprof * nr
-        prof = np.tile(np.arange(nprof, dtype=np.int32), nr)
-        g = np.zeros((nray, 7))
-        for j in range(nr):
-            for c, key in enumerate(("obsz", "obslon", "obslat", "vpz", "vplon", "vplat")):
-                g[j * nprof:(j + 1) * nprof, 1 + c] = geom[key][j]
-        a_d = torch.from_numpy(np.frombuffer(bytes(atms), dtype=np.uint8).copy()).to(dev)
-        p_d, g_d = torch.from_numpy(prof).to(dev), torch.from_numpy(g).to(dev)
+        atms, a_d, p_d, g_d, nray, nr = _limb_batch(torch, ctl, nprof)
         ctx = gpu.Context(ctl)
         try:
-            rad = torch.zeros(nray, ctl.nd, dtype=torch.float64, device=dev)
-            tau = torch.zeros_like(rad)
-            ctx.formod_device(nprof, a_d.data_ptr(), nray, p_d.data_ptr(), g_d.data_ptr(), rad.data_ptr(),
-                              tau.data_ptr(), precision=abi.JUR_FP64)
-            torch.cuda.synchronize()
-            out.append(rad.cpu().numpy().reshape(nr, nprof, ctl.nd)[:, :64])
+            for prec in out:
+                rad = torch.zeros(nray, ctl.nd, dtype=torch.float64, device="cuda:0")
+                tau = torch.zeros_like(rad)
+                ctx.formod_device(nprof, a_d.data_ptr(), nray, p_d.data_ptr(), g_d.data_ptr(), rad.data_ptr(),
+                                  tau.data_ptr(), precision=prec)
+                torch.cuda.synchronize()
+                out[prec].append(rad.cpu().numpy().reshape(nr, nprof, ctl.nd)[:, :64])
         finally:
             ctx.close()
-    assert np.isfinite(out[0]).all() and (out[0] > 0).all()
-    assert np.array_equal(out[0], out[1])
+    for prec, (big, small) in out.items():
+        assert np.isfinite(big).all() and (big > 0).all()
+        assert np.array_equal(big, small), prec
+
+
+def test_lookup_lanes_per_chain_jacobian_agree(gpu):
+    """The FP32 lookups with stored partials on four lanes per chain (small
+    batches) give the Jacobian of the one-lane chains: K rows of 16 limb
+    atmospheres x 32 tangent heights (T and H2O columns) agree between that
+    batch alone (8 ray blocks: four lanes) and inside one of 80 atmospheres
+    (40 blocks, more than the four-lane grid's 5120 wavefronts: one lane);
+    both run the one-sweep adjoint."""
+    torch = pytest.importorskip("torch")
+    ctl = scenes.limb_3gas_ctl()
+    ctl.rett_zmin, ctl.rett_zmax = -100.0, 100.0
+    ctl.retq_zmin[1], ctl.retq_zmax[1] = -100.0, 100.0
+    out = []
+    for nprof in (80, 16):
+        atms, a_d, p_d, g_d, nray, nr = _limb_batch(torch, ctl, nprof)
+        xm, n = gpu.state_map(ctl, atms[0])
+        ctx = gpu.Context(ctl)
+        try:
+            rad = torch.zeros(nray, ctl.nd, dtype=torch.float64, device="cuda:0")
+            tau = torch.zeros_like(rad)
+            kd = torch.zeros(nray, ctl.nd, n, dtype=torch.float64, device="cuda:0")
+            pmap = torch.zeros(nprof, dtype=torch.int32, device="cuda:0")
+            ctx.jacobian_device(nprof, a_d.data_ptr(), nray, p_d.data_ptr(), g_d.data_ptr(), rad.data_ptr(),
+                                tau.data_ptr(), xm[None, :], pmap.data_ptr(), n, kd.data_ptr(),
+                                precision=abi.JUR_FP32)
+            torch.cuda.synchronize()
+            k = kd.view(nr, nprof, ctl.nd, n)[:, :16].cpu().numpy()
+            out.append((rad.view(nr, nprof, -1)[:, :16].cpu().numpy(), k))
+        finally:
+            ctx.close()
+    (r0, k0), (r1, k1) = out
+    assert np.array_equal(r0, r1)
+    assert np.isfinite(k0).all() and np.abs(k0).max() > 0
+    np.testing.assert_allclose(k1, k0, rtol=1e-13, atol=1e-13 * np.abs(k0).max())
