@@ -30,6 +30,39 @@ int set_err(int code, const char *fmt, ...);
     }                                                                                \
   } while (0)
 
+// Pinned host memory that grows on demand (contents are not kept when it
+// grows): the source or target of asynchronous copies, which the runtime
+// then runs on the copy engines without staging them through a pageable
+// bounce buffer.
+template <typename T>
+struct HostBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  ~HostBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int reserve(size_t m) {
+    if (m <= n && p) return JUR_OK;
+    release();
+    if (m == 0) m = 1;
+    if (hipHostMalloc((void **)&p, m * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return set_err(JUR_ERR_MEM, "pinned host allocation of %zu bytes failed", m * sizeof(T));
+    }
+    n = m;
+    return JUR_OK;
+  }
+  T *data() { return p; }
+  T &operator[](size_t i) { return p[i]; }
+};
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;

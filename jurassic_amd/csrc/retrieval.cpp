@@ -258,7 +258,10 @@ struct Engine {
   DevBuf<int> p_prof, p_xmap, p_pmap;
   DevBuf<double> p_geom, p_rad, p_tau, p_k;
   DevBuf<unsigned char> p_mask;
-  std::vector<double> h_buf;
+  // pinned transfer buffers: the atmospheres of a pass, and the values that
+  // cross PCIe each round (radiances, K rows, LM vectors)
+  HostBuf<atm_t> h_atm;
+  HostBuf<double> h_buf;
   hipStream_t st = 0;
   // seconds spent: [0] admission, [1] Jacobian passes, [2] forward passes,
   // [3] dense algebra and bookkeeping
@@ -409,7 +412,7 @@ int Engine::setup(std::vector<Job *> &done) {
     off_sig[k] = tot;
     tot += pending[k]->m;
   }
-  h_buf.resize(tot);
+  if ((rc = h_buf.reserve(tot))) return rc;
   for (size_t q = 0; q < miss.size(); q++) {
     const Job &J = *pending[miss_k[q]];
     memcpy(&h_buf[off_sa[q]], J.sa_in.data(), J.sa_in.size() * sizeof(double));
@@ -525,7 +528,8 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
     maxn = std::max(maxn, jobs[r]->n);
   }
   const int np = (int)ids.size();
-  std::vector<atm_t> atms(np);
+  if ((rc = h_atm.reserve(np))) return rc;
+  atm_t *atms = h_atm.data();
   std::vector<int> prof(nray), xm, pmap(np);
   int nmap = 0;
   const bool jac_pass = with_k || (method == JUR_JAC_ADJOINT && !fwd_pass);
@@ -552,7 +556,7 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
   }
   // stream-ordered uploads from buffers that live to the end of this call;
   // the pass synchronizes the stream before it returns
-  if ((rc = p_atm.upload_async(atms.data(), np, st))) return rc;
+  if ((rc = p_atm.upload_async(atms, np, st))) return rc;
   if ((rc = p_prof.upload_async(prof.data(), nray, st))) return rc;
   if ((rc = p_geom.upload_async(geom.data(), geom.size(), st))) return rc;
   if ((rc = p_mask.upload_async(mask.data(), mask.size(), st))) return rc;
@@ -600,7 +604,7 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
       koff[q] = ktot;
       ktot += (size_t)J.m * maxn;
     }
-  h_buf.resize((size_t)nray * nd * 2 + ktot);
+  if ((rc = h_buf.reserve((size_t)nray * nd * 2 + ktot))) return rc;
   HIPCHK(hipMemcpyAsync(h_buf.data(), p_rad.p, (size_t)nray * nd * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(h_buf.data() + (size_t)nray * nd, p_tau.p, (size_t)nray * nd * sizeof(double),
                         hipMemcpyDeviceToHost, st));
@@ -668,7 +672,7 @@ int Engine::chisq(const std::vector<int> &ids) {
     off[k] = tot;
     tot += jobs[live[k]]->n + jobs[live[k]]->m;
   }
-  h_buf.resize(tot);
+  if ((rc = h_buf.reserve(tot))) return rc;
   for (size_t k = 0; k < live.size(); k++) {
     Job &J = *jobs[live[k]];
     for (int i = 0; i < J.n; i++) J.dx[i] = J.x_i[i] - J.x_a[i];
@@ -716,7 +720,7 @@ int Engine::stage_get(const std::vector<int> &ids, int which) {
   if ((rc = list(ids))) return rc;
   launch_lm_stage(d_desc.p, d_list.p, (int)ids.size(), d_off.p, d_stage.p, which, st);
   HIPCHK(hipGetLastError());
-  h_buf.resize(tot);
+  if ((rc = h_buf.reserve(tot))) return rc;
   HIPCHK(hipMemcpyAsync(h_buf.data(), d_stage.p, tot * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (size_t k = 0; k < ids.size(); k++) {
