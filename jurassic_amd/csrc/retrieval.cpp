@@ -520,7 +520,27 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
     }
     return JUR_OK;
   }
-  // one device pass over the rays of all jobs in ids
+  // one device pass over the rays of all jobs in ids, the Jacobian requests
+  // that want K first: their K rows are then one range, copied back at once
+  {
+    auto jk = [&](size_t q) { return jobs[ids[q]]->kind == JUR_REQ_JACOBIAN && wantk[q] != 0; };
+    bool lead = true, other = false;
+    for (size_t q = 0; q < ids.size() && lead; q++) {
+      if (jk(q) && other) lead = false;
+      other |= !jk(q);
+    }
+    if (!lead) {
+      std::vector<int> i2;
+      std::vector<char> w2;
+      for (int pass = 1; pass >= 0; pass--)
+        for (size_t q = 0; q < ids.size(); q++)
+          if (jk(q) == (pass == 1)) {
+            i2.push_back(ids[q]);
+            w2.push_back(wantk[q]);
+          }
+      return eval(i2, w2);
+    }
+  }
   int nray = 0, maxn = 1;
   for (int r : ids) {
     jobs[r]->ray0 = nray;
@@ -609,13 +629,9 @@ int Engine::eval(const std::vector<int> &ids, const std::vector<char> &wantk) {
   HIPCHK(hipMemcpyAsync(h_buf.data() + (size_t)nray * nd, p_tau.p, (size_t)nray * nd * sizeof(double),
                         hipMemcpyDeviceToHost, st));
   double *krows = h_buf.data() + (size_t)nray * nd * 2;
-  if (with_k)
-    for (size_t q = 0; q < ids.size(); q++) {
-      const Job &J = *jobs[ids[q]];
-      if (J.kind != JUR_REQ_JACOBIAN || !wantk[q]) continue;
-      HIPCHK(hipMemcpyAsync(krows + koff[q], p_k.p + (size_t)J.ray0 * nd * maxn,
-                            (size_t)J.m * maxn * sizeof(double), hipMemcpyDeviceToHost, st));
-    }
+  // the Jacobian requests lead the pass (ray0 = koff / (nd maxn)): their rows
+  // are the first ktot values of p_k
+  if (ktot) HIPCHK(hipMemcpyAsync(krows, p_k.p, ktot * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (int r : ids) {
     Job &J = *jobs[r];
