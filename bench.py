@@ -816,7 +816,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-warm", type=float, default=1.0,
                     help="seconds of the request schedule run untimed before the timed run")
-    ap.add_argument("--serve-lanes", type=int, default=1)
+    ap.add_argument("--serve-lanes", type=int, default=2,
+                    help="server lanes for the requests with limb rays (with --serve-path-split 1; "
+                         "one more lane takes the others)")
     ap.add_argument("--serve-forward-lane", action="store_true",
                     help="one more server lane for forward requests")
     ap.add_argument("--serve-path-split", type=int, default=1,
