@@ -709,6 +709,7 @@ def serving_bench(args, gpu, abi):
             srv.wait(i, 600.0)
             srv.release(i)
     r0 = srv.stats()[0]
+    prof0 = srv.profile()
     log("serving: %d requests over %.1f s (%.0f/s), %d forward, %d Jacobian, %d retrieval"
         % (nreq, t_arr[-1], args.serve_rate, (kinds == 0).sum(), (kinds == 1).sum(), (kinds == 2).sum()))
     ids = [None] * nreq
@@ -772,6 +773,11 @@ def serving_bench(args, gpu, abi):
         "wall_s": wall,
         "worker_seconds": {"admission": prof[0], "jacobian_passes": prof[1], "forward_passes": prof[2],
                            "dense_and_bookkeeping": prof[3]},
+        "timed_allocations": {"calls": int(prof[5] - prof0[5]), "seconds": prof[4] - prof0[4]},
+        "latency_ms_p99_by_second": [
+            float(np.percentile(lat[ok & (t_arr >= s0) & (t_arr < s0 + 1)], 99) * 1e3)
+            for s0 in range(int(np.ceil(t_arr[-1])))
+            if (ok & (t_arr >= s0) & (t_arr < s0 + 1)).any()],
     }
     for k, name in ((0, "forward"), (1, "jacobian"), (2, "retrieval")):
         sel = ok & (kinds == k)
@@ -822,7 +828,7 @@ def main():
     ap.add_argument("--serve-forward-lane", action="store_true",
                     help="one more server lane for forward requests")
     ap.add_argument("--serve-path-split", type=int, default=1,
-                    help="1: one server lane for the requests with limb rays, one for the others")
+                    help="1: --serve-lanes lanes for the requests with limb rays, one more for the others")
     ap.add_argument("--jac-only", action="store_true", help="skip the forward-model section")
     ap.add_argument("--jac-nprof", type=int, default=NJAC_PROF)
     ap.add_argument("--jac-steps", type=int, default=2)

@@ -416,7 +416,10 @@ int jur_server_fetch(jur_server_t *srv, uint64_t id, obs_t *obs, atm_t *atm,
 int jur_server_stats(const jur_server_t *srv, long *rounds, long *max_active);
 /* Worker time in seconds, summed over lanes: [0] admission, [1] device
    passes (with the adjoint, every pass), [2] forward-only passes (the
-   finite-difference method), [3] dense algebra and bookkeeping (n <= 4). */
+   finite-difference method), [3] dense algebra and bookkeeping; then,
+   counted over the whole process since it started, [4] seconds spent in the
+   library's device and pinned-host allocations and [5] how many there were
+   (n <= 6). */
 int jur_server_profile(const jur_server_t *srv, double *sec, int n);
 
 /* ERRMSG-style wrapper of jur_gpu_kernel(..., JUR_JAC_FD, JUR_FP64). */

@@ -7,6 +7,9 @@
 
 #include <stddef.h>
 
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -30,6 +33,31 @@ int set_err(int code, const char *fmt, ...);
     }                                                                                \
   } while (0)
 
+// Allocation calls of the library, process-wide: how many, and the time
+// spent in them.  hipFree and hipHostFree wait for the whole device, so a
+// buffer that grows during a serving run stalls every stream.
+struct AllocStats {
+  std::atomic<long> calls{0};
+  std::atomic<long long> ns{0};
+};
+inline AllocStats g_alloc;
+struct AllocTimer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  ~AllocTimer() {
+    g_alloc.calls++;
+    g_alloc.ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
+                      std::chrono::steady_clock::now() - t0).count();
+  }
+};
+// Size of a buffer of elements of elt bytes that has to grow from n to at
+// least m elements: half as much again as before, at most 256 MiB more, so
+// that sizes that creep upwards reallocate rarely and a large pass, which
+// sizes its buffers to a share of the free HBM, still gets what it asked for.
+inline size_t grow_to(size_t n, size_t m, size_t elt) {
+  const size_t g = std::min(n / 2, ((size_t)256 << 20) / elt);
+  return m < n + g ? n + g : m;
+}
+
 // Pinned host memory that grows on demand (contents are not kept when it
 // grows): the source or target of asynchronous copies, which the runtime
 // then runs on the copy engines without staging them through a pageable
@@ -49,6 +77,8 @@ struct HostBuf {
   }
   int reserve(size_t m) {
     if (m <= n && p) return JUR_OK;
+    AllocTimer t;
+    m = grow_to(n, m, sizeof(T));
     release();
     if (m == 0) m = 1;
     if (hipHostMalloc((void **)&p, m * sizeof(T), hipHostMallocDefault) != hipSuccess) {
@@ -75,6 +105,8 @@ struct DevBuf {
   }
   int reserve(size_t m) {
     if (m <= n && p) return JUR_OK;
+    AllocTimer t;
+    m = grow_to(n, m, sizeof(T));
     release();
     if (m == 0) m = 1;
     if (hipMalloc((void **)&p, m * sizeof(T)) != hipSuccess) {

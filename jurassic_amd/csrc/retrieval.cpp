@@ -66,6 +66,7 @@ struct DevPool {
       return p;
     }
     double *p = nullptr;
+    AllocTimer t;
     if (hipMalloc((void **)&p, ((size_t)1 << c) * sizeof(double)) != hipSuccess) {
       (void)hipGetLastError();
       return nullptr;
@@ -1502,6 +1503,8 @@ int jur_server_profile(const jur_server_t *s, double *sec, int n) {
     sec[i] = 0;
     for (auto &L : s->lanes) sec[i] += L->E.prof[i];
   }
+  if (n > 4) sec[4] = 1e-9 * (double)jur::g_alloc.ns.load();
+  if (n > 5) sec[5] = (double)jur::g_alloc.calls.load();
   return JUR_OK;
 }
 

@@ -426,9 +426,10 @@ class Server:
         return a.value, b.value
 
     def profile(self):
-        """Worker seconds: admission, Jacobian passes, forward passes, dense algebra."""
-        t = (C.c_double * 4)()
-        _check(lib().jur_server_profile(self.ptr, t, 4))
+        """Worker seconds: admission, Jacobian passes, forward passes, dense algebra;
+        then the process's allocation seconds and allocation count."""
+        t = (C.c_double * 6)()
+        _check(lib().jur_server_profile(self.ptr, t, 6))
         return list(t)
 
     def close(self):
