@@ -773,7 +773,8 @@ def serving_bench(args, gpu, abi):
         "wall_s": wall,
         "worker_seconds": {"admission": prof[0], "jacobian_passes": prof[1], "forward_passes": prof[2],
                            "dense_and_bookkeeping": prof[3]},
-        "timed_allocations": {"calls": int(prof[5] - prof0[5]), "seconds": prof[4] - prof0[4]},
+        "timed_allocations": {"calls": int(prof[5] - prof0[5]), "seconds": prof[4] - prof0[4],
+                              "regrow_seconds": prof[6] - prof0[6]},
         "latency_ms_p99_by_second": [
             float(np.percentile(lat[ok & (t_arr >= s0) & (t_arr < s0 + 1)], 99) * 1e3)
             for s0 in range(int(np.ceil(t_arr[-1])))

@@ -418,8 +418,9 @@ int jur_server_stats(const jur_server_t *srv, long *rounds, long *max_active);
    passes (with the adjoint, every pass), [2] forward-only passes (the
    finite-difference method), [3] dense algebra and bookkeeping; then,
    counted over the whole process since it started, [4] seconds spent in the
-   library's device and pinned-host allocations and [5] how many there were
-   (n <= 6). */
+   library's device and pinned-host allocations, [5] how many there were and
+   [6] the seconds of those that first freed a smaller buffer (a free waits
+   for the whole device) (n <= 7). */
 int jur_server_profile(const jur_server_t *srv, double *sec, int n);
 
 /* ERRMSG-style wrapper of jur_gpu_kernel(..., JUR_JAC_FD, JUR_FP64). */

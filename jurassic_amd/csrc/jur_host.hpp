@@ -33,20 +33,24 @@ int set_err(int code, const char *fmt, ...);
     }                                                                                \
   } while (0)
 
-// Allocation calls of the library, process-wide: how many, and the time
-// spent in them.  hipFree and hipHostFree wait for the whole device, so a
-// buffer that grows during a serving run stalls every stream.
+// Allocation calls of the library, process-wide: how many, the time spent
+// in them, and the part of it spent in calls that freed a smaller buffer
+// first (a free waits for the whole device).
 struct AllocStats {
   std::atomic<long> calls{0};
   std::atomic<long long> ns{0};
+  std::atomic<long long> regrow_ns{0};  // of calls that freed a smaller buffer first
 };
 inline AllocStats g_alloc;
 struct AllocTimer {
+  bool regrow = false;  // set when the call freed a smaller buffer
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   ~AllocTimer() {
+    const long long d = std::chrono::duration_cast<std::chrono::nanoseconds>(
+                            std::chrono::steady_clock::now() - t0).count();
     g_alloc.calls++;
-    g_alloc.ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
-                      std::chrono::steady_clock::now() - t0).count();
+    g_alloc.ns += d;
+    if (regrow) g_alloc.regrow_ns += d;
   }
 };
 // Size of a buffer of elements of elt bytes that has to grow from n to at
@@ -58,6 +62,13 @@ inline size_t grow_to(size_t n, size_t m, size_t elt) {
   return m < n + g ? n + g : m;
 }
 
+// A buffer of at most this size that grows keeps its old allocation until it
+// is released instead of freeing it at once: hipFree and hipHostFree wait
+// for the whole device, which would stall every stream of a serving process
+// each time one of its work buffers grows.  With the growth rule above the
+// kept allocations add up to at most twice the final size.
+constexpr size_t KEEP_ON_GROW_BYTES = (size_t)64 << 20;
+
 // Pinned host memory that grows on demand (contents are not kept when it
 // grows): the source or target of asynchronous copies, which the runtime
 // then runs on the copy engines without staging them through a pageable
@@ -66,12 +77,15 @@ template <typename T>
 struct HostBuf {
   T *p = nullptr;
   size_t n = 0;
+  std::vector<T *> kept;  // smaller allocations replaced by growth
   HostBuf() = default;
   HostBuf(const HostBuf &) = delete;
   HostBuf &operator=(const HostBuf &) = delete;
   ~HostBuf() { release(); }
   void release() {
     if (p) (void)hipHostFree(p);
+    for (T *q : kept) (void)hipHostFree(q);
+    kept.clear();
     p = nullptr;
     n = 0;
   }
@@ -79,7 +93,14 @@ struct HostBuf {
     if (m <= n && p) return JUR_OK;
     AllocTimer t;
     m = grow_to(n, m, sizeof(T));
-    release();
+    if (p && n * sizeof(T) <= KEEP_ON_GROW_BYTES) {
+      kept.push_back(p);
+      p = nullptr;
+      n = 0;
+    } else {
+      t.regrow = p != nullptr;
+      release();
+    }
     if (m == 0) m = 1;
     if (hipHostMalloc((void **)&p, m * sizeof(T), hipHostMallocDefault) != hipSuccess) {
       (void)hipGetLastError();
@@ -97,9 +118,15 @@ template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  std::vector<T *> kept;  // smaller allocations replaced by growth
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
+    for (T *q : kept) (void)hipFree(q);
+    kept.clear();
     p = nullptr;
     n = 0;
   }
@@ -107,7 +134,14 @@ struct DevBuf {
     if (m <= n && p) return JUR_OK;
     AllocTimer t;
     m = grow_to(n, m, sizeof(T));
-    release();
+    if (p && n * sizeof(T) <= KEEP_ON_GROW_BYTES) {
+      kept.push_back(p);
+      p = nullptr;
+      n = 0;
+    } else {
+      t.regrow = p != nullptr;
+      release();
+    }
     if (m == 0) m = 1;
     if (hipMalloc((void **)&p, m * sizeof(T)) != hipSuccess) {
       (void)hipGetLastError();

@@ -1505,6 +1505,7 @@ int jur_server_profile(const jur_server_t *s, double *sec, int n) {
   }
   if (n > 4) sec[4] = 1e-9 * (double)jur::g_alloc.ns.load();
   if (n > 5) sec[5] = (double)jur::g_alloc.calls.load();
+  if (n > 6) sec[6] = 1e-9 * (double)jur::g_alloc.regrow_ns.load();
   return JUR_OK;
 }
 

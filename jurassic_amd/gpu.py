@@ -427,9 +427,10 @@ class Server:
 
     def profile(self):
         """Worker seconds: admission, Jacobian passes, forward passes, dense algebra;
-        then the process's allocation seconds and allocation count."""
-        t = (C.c_double * 6)()
-        _check(lib().jur_server_profile(self.ptr, t, 6))
+        then the process's allocation seconds, allocation count and the seconds of
+        the allocations that replaced a smaller buffer."""
+        t = (C.c_double * 7)()
+        _check(lib().jur_server_profile(self.ptr, t, 7))
         return list(t)
 
     def close(self):
