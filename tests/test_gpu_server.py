@@ -135,12 +135,13 @@ def test_server_forward_lane(gpu, oracle):
     """With a forward lane, forward requests take the forward pass (the
     kernels of formod: equal to the last bit), and Jacobians and retrievals
     are those of the one-lane server; so are they with two general lanes,
-    and with the limb requests and the others on lanes of their own."""
+    and with the limb requests on one or two lanes and the others on a lane
+    of their own."""
     ctl, reqs = _mix(oracle)
     ret = _ret(ctl.nd)
     prec = abi.JUR_FP64
     base, _ = _run(gpu, ctl, ret, reqs, prec)
-    for lanes, fl, ps in ((1, 1, 0), (2, 0, 0), (1, 0, 1)):
+    for lanes, fl, ps in ((1, 1, 0), (2, 0, 0), (1, 0, 1), (2, 0, 1)):
         got, (rounds, _) = _run(gpu, ctl, ret, reqs, prec, lanes=lanes, forward_lane=fl, path_split=ps)
         assert rounds >= 1
         for r, (res, (obs, atm, k, hist)), (res0, (obs0, atm0, k0, hist0)) in zip(reqs, got, base):
