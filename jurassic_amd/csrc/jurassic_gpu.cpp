@@ -1016,10 +1016,12 @@ static int formod_packages(const ctl_t *ctl, atm_t *atm, obs_t *obs, int n, int 
     rc = forward_pass(c, B, precision, S.mask.p, S.rad.p, S.tau.p, S.tp.p, &fp, 0);
   }
   if (rc) return rc;
-  std::vector<double> rad((size_t)nray * nd), tau((size_t)nray * nd), tp((size_t)nray * 3);
-  HIPCHK(hipMemcpy(rad.data(), S.rad.p, rad.size() * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(tau.data(), S.tau.p, tau.size() * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(tp.data(), S.tp.p, tp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // one spare element each: a batch without rays copies nothing, from and to
+  // valid addresses
+  std::vector<double> rad((size_t)nray * nd + 1), tau((size_t)nray * nd + 1), tp((size_t)nray * 3 + 1);
+  HIPCHK(hipMemcpy(rad.data(), S.rad.p, (size_t)nray * nd * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(tau.data(), S.tau.p, (size_t)nray * nd * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(tp.data(), S.tp.p, (size_t)nray * 3 * sizeof(double), hipMemcpyDeviceToHost));
   if (ctl->hydz >= 0) {
     // hydrostatic() rewrote the pressure profiles in place
     std::vector<atm_t> tmp(n);
@@ -1204,7 +1206,7 @@ static int kernel_fd(const ctl_t *ctl, atm_t *atm, obs_t *obs, double *k, size_t
   std::vector<double> rad((size_t)n * nr * nd + 1), tau((size_t)n * nr * nd + 1);
   if (n > 0 && (rc = formod_shared_geometry(ctl, atms.data(), (int)n, obs, precision, rad.data(), tau.data())))
     return rc;
-  memset(k, 0, m * n * sizeof(double));
+  if (m * n) memset(k, 0, m * n * sizeof(double));
   for (size_t j = 0; j < n; j++) {
     size_t i = 0;
     for (int ir = 0; ir < nr; ir++)

@@ -1367,7 +1367,8 @@ int jur_server_submit(jur_server_t *s, const jur_request_t *req, uint64_t *id) {
   if (req->kind != JUR_REQ_FORWARD && req->kind != JUR_REQ_JACOBIAN && req->kind != JUR_REQ_RETRIEVAL)
     return set_err(JUR_ERR_ARG, "invalid request kind");
   const double t_submit = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-  if (req->obs->nr < 1 || req->obs->nr > NR) return set_err(JUR_ERR_ARG, "Too many rays!");
+  if (req->obs->nr < 1) return set_err(JUR_ERR_ARG, "request without rays");
+  if (req->obs->nr > NR) return set_err(JUR_ERR_ARG, "Too many rays!");
   std::unique_ptr<Job> J(new Job());
   J->kind = req->kind;
   // The job's ctl only defines its state vector (atm2x and friends read the

@@ -360,6 +360,66 @@ def test_formod_gpu_and_kernel_gpu_dropins(gpu, oracle):
     assert np.abs(k0).max() > 0.01
 
 
+def test_empty_inputs(gpu, oracle):
+    """No rays: formod's ray loop (src/jurassic.c:3047) and kernel()'s rows
+    are empty, and the entry points return without error. A package without
+    rays in a batch leaves the others' results as they are alone, and a
+    device batch without rays (null pointers) and a server request without
+    rays are accepted and rejected as documented."""
+    ctl, atm, obs = scenes.nadir_co2()
+    empty = abi.copy_struct(obs)
+    empty.nr = 0
+    for prec in (abi.JUR_FP64, abi.JUR_FP32):
+        o = abi.copy_struct(empty)
+        gpu.formod(ctl, abi.copy_struct(atm), o, prec)
+        assert o.nr == 0
+    atms = (abi.atm_t * 3)()
+    obss = (abi.obs_t * 3)()
+    for k, src in enumerate((empty, obs, empty)):
+        atms[k] = abi.copy_struct(atm)
+        obss[k] = abi.copy_struct(src)
+    gpu.formod_batch(ctl, atms, obss)
+    alone = abi.copy_struct(obs)
+    gpu.formod(ctl, abi.copy_struct(atm), alone)
+    np.testing.assert_array_equal(abi.obs_arrays(obss[1], ctl)[0], abi.obs_arrays(alone, ctl)[0])
+    assert obss[0].nr == 0 and obss[2].nr == 0
+
+    ctl.rett_zmin, ctl.rett_zmax = -100, 100
+    m, n = oracle.sizes(ctl, atm, empty)
+    assert m == 0 and n > 0
+    for method, prec in ((abi.JUR_JAC_FD, abi.JUR_FP64), (abi.JUR_JAC_ADJOINT, abi.JUR_FP32)):
+        k = gpu.kernel(ctl, abi.copy_struct(atm), abi.copy_struct(empty), method, prec, m=m, n=n)
+        assert k.shape == (0, n)
+
+    ctx = gpu.Context(ctl)
+    try:
+        for prec in (abi.JUR_FP64, abi.JUR_FP32):
+            ctx.formod_device(0, None, 0, None, None, None, None, precision=prec)
+        xm, n1 = gpu.state_map(ctl, atm)
+        ctx.jacobian_device(0, None, 0, None, None, None, None, xm[None, :], None, n1, None,
+                            precision=abi.JUR_FP32)
+    finally:
+        ctx.close()
+
+    srv = gpu.Server(ctl, _ret_for_empty(ctl.nd))
+    try:
+        with pytest.raises(gpu.JurassicError, match="without rays"):
+            srv.submit(gpu.Server.request(abi.JUR_REQ_FORWARD, atm, empty))
+    finally:
+        srv.close()
+
+
+def _ret_for_empty(nd):
+    ret = abi.ret_t()
+    ret.kernel_recomp = 3
+    ret.conv_itmax = 10
+    ret.conv_dmin = 0.1
+    for id in range(nd):
+        ret.err_noise[id] = 0.1
+    ret.err_temp = 5.0
+    return ret
+
+
 def test_fp32_bt_bound_limb_at_scale(gpu, oracle):
     """The 0.01 K bound over a large batch of the limb workload: 512
     atmospheres (seeded perturbed profiles) x 32 tangent heights 5..67 km x
