@@ -400,7 +400,8 @@ typedef struct {
 int jur_server_create(const ctl_t *ctl, const jur_ret_t *ret,
                       const jur_server_opts_t *opt, jur_server_t **srv);
 void jur_server_destroy(jur_server_t *srv);
-/* Copies the request's inputs and queues it; *id names it. */
+/* Copies the request's inputs and queues it; *id names it.  A request needs
+   1 <= obs->nr <= NR rays (JUR_ERR_ARG otherwise). */
 int jur_server_submit(jur_server_t *srv, const jur_request_t *req,
                       uint64_t *id);
 /* Waits until request id is finished (timeout_s < 0: no limit). */
