@@ -409,6 +409,30 @@ def test_empty_inputs(gpu, oracle):
         srv.close()
 
 
+def test_max_rays(gpu, oracle):
+    """One observation at its capacity, obs->nr = NR (128 nadir and 128 limb
+    rays of uneven LOS lengths), against the oracle; NR + 1 rays are
+    rejected as the reference's raytrace/formod ERRMSG does ("Too many
+    rays!")."""
+    ctl, atm, _ = scenes.nadir_co2()
+    half = abi.NR // 2
+    g1 = scenes.nadir_geometry(705.0, np.linspace(0.0, 60.0, half))
+    g2 = scenes.limb_geometry(800.0, np.linspace(4.0, 70.0, abi.NR - half))
+    obs = scenes.make_obs({k: np.concatenate([g1[k], g2[k]]) for k in g1})
+    assert obs.nr == abi.NR
+    (_, o_cpu), (_, o_gpu) = _run_both(gpu, oracle, ctl, atm, obs)
+    _assert_obs(ctl, o_gpu, o_cpu)
+    (_, o_cpu), (_, o_gpu) = _run_both(gpu, oracle, ctl, atm, obs, abi.JUR_FP32)
+    r1, _ = abi.obs_arrays(o_gpu, ctl)
+    r0, _ = abi.obs_arrays(o_cpu, ctl)
+    nu = np.asarray(ctl.nu[:ctl.nd])[:, None]
+    assert np.abs(_bt(r1, nu) - _bt(r0, nu)).max() <= BT_TOL32
+    bad = abi.copy_struct(obs)
+    bad.nr = abi.NR + 1
+    with pytest.raises(gpu.JurassicError, match="Too many rays"):
+        gpu.formod(ctl, abi.copy_struct(atm), bad)
+
+
 def _ret_for_empty(nd):
     ret = abi.ret_t()
     ret.kernel_recomp = 3
