@@ -66,8 +66,10 @@ inline size_t grow_to(size_t n, size_t m, size_t elt) {
 // is released instead of freeing it at once: hipFree and hipHostFree wait
 // for the whole device, which would stall every stream of a serving process
 // each time one of its work buffers grows.  With the growth rule above the
-// kept allocations add up to at most twice the final size.
-constexpr size_t KEEP_ON_GROW_BYTES = (size_t)64 << 20;
+// kept allocations add up to less than three times the final size (twice
+// for a buffer that stays under 512 MiB).  Larger buffers, which only
+// large passes use, are freed as before.
+constexpr size_t KEEP_ON_GROW_BYTES = (size_t)1 << 30;
 
 // Pinned host memory that grows on demand (contents are not kept when it
 // grows): the source or target of asynchronous copies, which the runtime
