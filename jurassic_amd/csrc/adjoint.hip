@@ -356,10 +356,10 @@ __device__ __forceinline__ double beta_grad(const ModelView &M, const CtmChan &K
     g.dt += u_co2 * p * dctw * f;
   }
   if (M.ig_h2o >= 0 && K.h2o) {
-    const double S = K.h2o_slf * pow(K.h2o_ratio, (296 - t) / (296 - 260));
+    const double S = K.h2o_slf * ctm_h2o_pow(K, (296 - t) / (296 - 260));
     const double dS = K.h2o_ratio > 0 ? S * log(K.h2o_ratio) * (-1.0 / 36) : 0.0;
     const double bq = .7193876 * nu;
-    const double th = tanh(bq / t);
+    const double th = tanh_pos(bq / t);
     const double a1u = nu * th;                            // a1 per unit column
     const double da1u = nu * (1 - th * th) * (-bq / (t * t));
     const double a2 = 296 / t, da2 = -296 / (t * t);

@@ -602,100 +602,51 @@ struct GasConst {
 };
 
 // ------------------------------------------------------------------
-// Continua (src/jurassic.c:886-2918), FP64, reference expression trees.
+// Continua (formod_continua, src/jurassic.c:3075-3117, with ctmco2
+// :886-1744, ctmh2o :1746-2794, ctmn2 :2796-2859, ctmo2 :2861-2918), FP64.
 // ------------------------------------------------------------------
-__device__ __forceinline__ double ctmco2(const CtmView &c, double nu, double p, double t, double u) {
-  JUR_FP_EXACT
-  const double xw = nu / 2 + 1;
-  if (xw >= 1 && xw < 2001) {
-    const int iw = (int)xw;
-    const double dw = xw - iw;
-    const double ew = 1 - dw;
-    const double cw296 = ew * c.co2_296[iw - 1] + dw * c.co2_296[iw];
-    const double cw260 = ew * c.co2_260[iw - 1] + dw * c.co2_260[iw];
-    const double cw230 = ew * c.co2_230[iw - 1] + dw * c.co2_230[iw];
-    const double dt230 = t - 230;
-    const double dt260 = t - 260;
-    const double dt296 = t - 296;
-    const double ctw = dt260 * 5.050505e-4 * dt296 * cw230 - dt230 * 9.259259e-4 * dt296 * cw260 +
-                       dt230 * 4.208754e-4 * dt260 * cw296;
-    return u / JUR_NA / 1000 * p / JUR_P0 * ctw;
-  }
-  return 0;
-}
-
-__device__ __forceinline__ double ctmh2o(const CtmView &c, double nu, double p, double t, double q,
-                                         double u) {
-  JUR_FP_EXACT
-  double sfac;
-  const double xw = nu / 10 + 1;
-  if (xw >= 1 && xw < 2001) {
-    const int iw = (int)xw;
-    const double dw = xw - iw;
-    const double ew = 1 - dw;
-    const double cw296 = ew * c.h2o_296[iw - 1] + dw * c.h2o_296[iw];
-    const double cw260 = ew * c.h2o_260[iw - 1] + dw * c.h2o_260[iw];
-    const double cwfrn = ew * c.h2o_frn[iw - 1] + dw * c.h2o_frn[iw];
-    if (nu <= 820 || nu >= 960) {
-      sfac = 1;
-    } else {
-      const double xx = (nu - 820) / 10;
-      const int ix = (int)xx;
-      const double dx = xx - ix;
-      sfac = (1 - dx) * c.h2o_xfcrev[ix] + dx * c.h2o_xfcrev[ix + 1];
-    }
-    const double ctwslf = sfac * cw296 * pow(cw260 / cw296, (296 - t) / (296 - 260));
-    const double vf2 = (nu - 370) * (nu - 370);
-    const double vf6 = vf2 * vf2 * vf2;
-    const double fscal = 36100 / (vf2 + vf6 * 1e-8 + 36100) * -.25 + 1;
-    const double ctwfrn = cwfrn * fscal;
-    const double a1 = nu * u * tanh(.7193876 / t * nu);
-    const double a2 = 296 / t;
-    const double a3 = p / JUR_P0 * (q * ctwslf + (1 - q) * ctwfrn) * 1e-20;
-    return a1 * a2 * a3;
-  }
-  return 0;
-}
-
-__device__ __forceinline__ double ctmn2(const CtmView &c, double nu, double p, double t) {
-  JUR_FP_EXACT
-  const double q_n2 = 0.79, t0 = 273.0, tr = 296.0;
-  if (nu < c.n2_nu[0] || nu > c.n2_nu[97]) return 0;
-  const int idx = locate_reg(c.n2_nu, 98, nu);
-  const double b = lin(c.n2_nu[idx], c.n2_b[idx], c.n2_nu[idx + 1], c.n2_b[idx + 1], nu);
-  const double beta = lin(c.n2_nu[idx], c.n2_beta[idx], c.n2_nu[idx + 1], c.n2_beta[idx + 1], nu);
-  const double r = p / JUR_P0 * t0 / t;
-  return 0.1 * (r * r) * exp(beta * (1 / tr - 1 / t)) * q_n2 * b *
-         (q_n2 + (1 - q_n2) * (1.294 - 0.4545 * t / tr));
-}
-
-__device__ __forceinline__ double ctmo2(const CtmView &c, double nu, double p, double t) {
-  JUR_FP_EXACT
-  const double q_o2 = 0.21, t0 = 273, tr = 296;
-  if (nu < c.o2_nu[0] || nu > c.o2_nu[89]) return 0;
-  const int idx = locate_reg(c.o2_nu, 90, nu);
-  const double b = lin(c.o2_nu[idx], c.o2_b[idx], c.o2_nu[idx + 1], c.o2_b[idx + 1], nu);
-  const double beta = lin(c.o2_nu[idx], c.o2_beta[idx], c.o2_nu[idx + 1], c.o2_beta[idx + 1], nu);
-  const double r = p / JUR_P0 * t0 / t;
-  return 0.1 * (r * r) * exp(beta * (1 / tr - 1 / t)) * q_o2 * b;
-}
-
 // The continua split into a per-channel part (everything that depends only
-// on the wavenumber) and a per-segment part, with the same operations as
-// ctmco2/ctmh2o/ctmn2/ctmo2 above: a radiance kernel evaluates the channel
-// part once per lane instead of once per segment.
+// on the wavenumber) and a per-segment part, with the reference's
+// operations: a radiance kernel evaluates the channel part once per lane
+// instead of once per segment.  Two evaluations differ from the reference's
+// expressions by a few ulp (radiances within 5e-16 relative), which halves
+// the cost of the H2O term:
+//  - the self-continuum power pow(cw260/cw296, (296 - T)/36) is
+//    exp((296 - T)/36 * ln(cw260/cw296)), the logarithm taken once per
+//    channel; for a ratio of 0, infinity or NaN this is what pow gives too,
+//    except at exponent 0 (T = 296 K), where pow gives 1 and so does the
+//    code;
+//  - tanh(x), x = 0.7193876 nu / T > 0, is em / (em + 2) with em =
+//    expm1(2x), which loses no digits for small x (1 beyond x = 20, where
+//    tanh(x) rounds to 1).
 struct CtmChan {
   bool co2, h2o, n2, o2;
   double co2_296, co2_260, co2_230;
   double h2o_slf, h2o_ratio, h2o_frn;  // sfac*cw296, cw260/cw296, cwfrn*fscal
+  double h2o_lr;                       // ln(h2o_ratio)
   double n2_b, n2_beta, o2_b, o2_beta;
 };
+
+// h2o_ratio^e (the H2O self-continuum temperature power).
+__device__ __forceinline__ double ctm_h2o_pow(const CtmChan &k, double e) {
+  JUR_FP_EXACT
+  return e == 0 ? 1.0 : exp(e * k.h2o_lr);
+}
+
+// tanh(x) for x >= 0.
+__device__ __forceinline__ double tanh_pos(double x) {
+  JUR_FP_EXACT
+  if (x > 20) return 1.0;
+  const double em = expm1(2 * x);
+  return em / (em + 2);
+}
 
 __device__ __forceinline__ CtmChan ctm_chan(const CtmView &c, double nu) {
   JUR_FP_EXACT
   CtmChan k;
   k.co2 = k.h2o = k.n2 = k.o2 = false;
   k.co2_296 = k.co2_260 = k.co2_230 = k.h2o_slf = k.h2o_ratio = k.h2o_frn = 0;
+  k.h2o_lr = 0;
   k.n2_b = k.n2_beta = k.o2_b = k.o2_beta = 0;
   {
     const double xw = nu / 2 + 1;
@@ -733,6 +684,7 @@ __device__ __forceinline__ CtmChan ctm_chan(const CtmView &c, double nu) {
       k.h2o = true;
       k.h2o_slf = sfac * cw296;
       k.h2o_ratio = cw260 / cw296;
+      k.h2o_lr = log(k.h2o_ratio);
       k.h2o_frn = cwfrn * fscal;
     }
   }
@@ -766,8 +718,8 @@ __device__ __forceinline__ double ctmh2o_seg(const CtmChan &k, double nu, double
                                              double u) {
   JUR_FP_EXACT
   if (!k.h2o) return 0;
-  const double ctwslf = k.h2o_slf * pow(k.h2o_ratio, (296 - t) / (296 - 260));
-  const double a1 = nu * u * tanh(.7193876 / t * nu);
+  const double ctwslf = k.h2o_slf * ctm_h2o_pow(k, (296 - t) / (296 - 260));
+  const double a1 = nu * u * tanh_pos(.7193876 / t * nu);
   const double a2 = 296 / t;
   const double a3 = p / JUR_P0 * (q * ctwslf + (1 - q) * k.h2o_frn) * 1e-20;
   return a1 * a2 * a3;
