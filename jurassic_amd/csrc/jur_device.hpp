@@ -703,6 +703,10 @@ __device__ __forceinline__ CtmChan ctm_chan(const CtmView &c, double nu) {
   return k;
 }
 
+// The per-segment parts take 1/T (rt) where the reference divides by T, and
+// divide by constants as products with their reciprocals, folded at compile
+// time: an FP64 division costs about fifteen instructions, and a product by
+// a reciprocal differs from the quotient by at most an ulp or so.
 __device__ __forceinline__ double ctmco2_seg(const CtmChan &k, double p, double t, double u) {
   JUR_FP_EXACT
   if (!k.co2) return 0;
@@ -711,35 +715,35 @@ __device__ __forceinline__ double ctmco2_seg(const CtmChan &k, double p, double 
   const double dt296 = t - 296;
   const double ctw = dt260 * 5.050505e-4 * dt296 * k.co2_230 - dt230 * 9.259259e-4 * dt296 * k.co2_260 +
                      dt230 * 4.208754e-4 * dt260 * k.co2_296;
-  return u / JUR_NA / 1000 * p / JUR_P0 * ctw;
+  return u * (1 / JUR_NA / 1000 / JUR_P0) * p * ctw;
 }
 
-__device__ __forceinline__ double ctmh2o_seg(const CtmChan &k, double nu, double p, double t, double q,
-                                             double u) {
+__device__ __forceinline__ double ctmh2o_seg(const CtmChan &k, double nu, double p, double t, double rt,
+                                             double q, double u) {
   JUR_FP_EXACT
   if (!k.h2o) return 0;
-  const double ctwslf = k.h2o_slf * ctm_h2o_pow(k, (296 - t) / (296 - 260));
-  const double a1 = nu * u * tanh_pos(.7193876 / t * nu);
-  const double a2 = 296 / t;
-  const double a3 = p / JUR_P0 * (q * ctwslf + (1 - q) * k.h2o_frn) * 1e-20;
+  const double ctwslf = k.h2o_slf * ctm_h2o_pow(k, (296 - t) * (1.0 / (296 - 260)));
+  const double a1 = nu * u * tanh_pos(.7193876 * nu * rt);
+  const double a2 = 296 * rt;
+  const double a3 = p * (1 / JUR_P0) * (q * ctwslf + (1 - q) * k.h2o_frn) * 1e-20;
   return a1 * a2 * a3;
 }
 
-__device__ __forceinline__ double ctmn2_seg(const CtmChan &k, double p, double t) {
+__device__ __forceinline__ double ctmn2_seg(const CtmChan &k, double p, double t, double rt) {
   JUR_FP_EXACT
   if (!k.n2) return 0;
   const double q_n2 = 0.79, t0 = 273.0, tr = 296.0;
-  const double r = p / JUR_P0 * t0 / t;
-  return 0.1 * (r * r) * exp(k.n2_beta * (1 / tr - 1 / t)) * q_n2 * k.n2_b *
-         (q_n2 + (1 - q_n2) * (1.294 - 0.4545 * t / tr));
+  const double r = p * (t0 / JUR_P0) * rt;
+  return 0.1 * (r * r) * exp(k.n2_beta * (1 / tr - rt)) * q_n2 * k.n2_b *
+         (q_n2 + (1 - q_n2) * (1.294 - t * (0.4545 / tr)));
 }
 
-__device__ __forceinline__ double ctmo2_seg(const CtmChan &k, double p, double t) {
+__device__ __forceinline__ double ctmo2_seg(const CtmChan &k, double p, double rt) {
   JUR_FP_EXACT
   if (!k.o2) return 0;
   const double q_o2 = 0.21, t0 = 273, tr = 296;
-  const double r = p / JUR_P0 * t0 / t;
-  return 0.1 * (r * r) * exp(k.o2_beta * (1 / tr - 1 / t)) * q_o2 * k.o2_b;
+  const double r = p * (t0 / JUR_P0) * rt;
+  return 0.1 * (r * r) * exp(k.o2_beta * (1 / tr - rt)) * q_o2 * k.o2_b;
 }
 
 // formod_srcfunc (src/jurassic.c:3483-3496) for one channel column of the
@@ -748,6 +752,16 @@ __device__ __forceinline__ double srcfunc(const double *st, const double *sr_col
   JUR_FP_EXACT
   const int it = locate_reg(st, TBLNS, t);
   return lin(st[it], sr_col[it], st[it + 1], sr_col[it + 1], t);
+}
+
+// srcfunc on the uniform temperature grid with its inverse spacing inv_dst
+// in hand (the per-segment form: products instead of the two divisions of
+// locate_reg and LIN).
+__device__ __forceinline__ double srcfunc_u(const double *st, double inv_dst, const double *sr_col, double t) {
+  JUR_FP_EXACT
+  int it = (int)((t - st[0]) * inv_dst);
+  it = it < 0 ? 0 : (it > TBLNS - 2 ? TBLNS - 2 : it);
+  return sr_col[it] + (sr_col[it + 1] - sr_col[it]) * inv_dst * (t - st[it]);
 }
 
 // PLANCK (src/jurassic.h:183-184).

@@ -7,7 +7,12 @@
 // field is read with one coalesced access per wavefront; the work per segment
 // is a few FP64 operations, so the kernel streams.
 //
-// Built with -ffp-contract=off; operation order follows the reference.
+// Built with -ffp-contract=off; operation order follows the reference,
+// except that the per-segment continua and source function multiply by
+// reciprocals (1/T, 1/ds, the grid's inverse spacing, constants folded at
+// compile time) where the reference divides, and the H2O continuum's power
+// and tanh are evaluated through exp/expm1 (jur_device.hpp, CtmChan):
+// radiances within a few ulp of the reference expressions.
 #include "jur_device.hpp"
 
 namespace jur {
@@ -34,6 +39,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f64(ModelView M, BatchView B,
   const int npb = E.npb[bl];
   const double *eps_g = (const double *)E.eps + E.off[bl] + (size_t)id * ng * npb * WAVE + lane;
   const double *sr_col = M.sr + (size_t)id * TBLNS;
+  const double inv_dst = 1 / (M.st[1] - M.st[0]);
 
   double kcl = 0;
   if (L.f_cl >= 0 && M.ncl > 0)
@@ -58,15 +64,16 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f64(ModelView M, BatchView B,
     }
 
     // formod_continua (src/jurassic.c:3075-3117)
+    const double rt = 1 / t, rds = 1 / ds;
     double beta = has_k ? LOSR(L.f_k + C.window, ip) : 0.0;
     if (L.f_cl >= 0) beta += LOSR(L.f_cl, ip) * kcl;
-    if (M.ig_co2 >= 0) beta += ctmco2_seg(K, p, t, LOSR(L.f_u + M.ig_co2, ip)) / ds;
+    if (M.ig_co2 >= 0) beta += ctmco2_seg(K, p, t, LOSR(L.f_u + M.ig_co2, ip)) * rds;
     if (M.ig_h2o >= 0)
-      beta += ctmh2o_seg(K, nu, p, t, LOSR(F_Q0 + M.ig_h2o, ip), LOSR(L.f_u + M.ig_h2o, ip)) / ds;
-    if (M.ctm_n2) beta += ctmn2_seg(K, p, t);
-    if (M.ctm_o2) beta += ctmo2_seg(K, p, t);
+      beta += ctmh2o_seg(K, nu, p, t, rt, LOSR(F_Q0 + M.ig_h2o, ip), LOSR(L.f_u + M.ig_h2o, ip)) * rds;
+    if (M.ctm_n2) beta += ctmn2_seg(K, p, t, rt);
+    if (M.ctm_o2) beta += ctmo2_seg(K, p, rt);
 
-    const double src = srcfunc(M.st, sr_col, t);
+    const double src = srcfunc_u(M.st, inv_dst, sr_col, t);
 
     if (tau_gas > 0) {
       const double eps = 1 - tau_gas * exp(-beta * ds);
