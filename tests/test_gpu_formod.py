@@ -2,7 +2,9 @@
 
 FP64 path: radiance and transmittance within 1e-10 relative of the oracle
 (the oracle restates the reference formod; the GPU kernels keep its
-operation order, so only last-bit differences of the math library remain).
+operation order except for a few products by reciprocals and the H2O
+continuum's power and tanh through exp/expm1, so only differences of a few
+ulp remain).
 FP32 path: brightness temperature within 0.01 K of the FP64 oracle.
 All calls go through the C-ABI of libjurassic_gpu.so.
 """
