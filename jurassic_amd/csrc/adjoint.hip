@@ -442,18 +442,19 @@ __device__ __forceinline__ double beta_grad_f(const ModelView &M, const CtmChan 
     const double da1u = nu * (1 - th * th) * (-bq * rt * rt);
     const double a2 = 296 * rt, da2 = -296 * rt * rt;
     const double mix = q_h2o * S + (1 - q_h2o) * K.h2o_frn;
-    const double a3 = p / JUR_P0 * mix * 1e-20;
-    const double da3 = p / JUR_P0 * q_h2o * dS * 1e-20;
+    const double pp0 = p * (1 / JUR_P0);
+    const double a3 = pp0 * mix * 1e-20;
+    const double da3 = pp0 * q_h2o * dS * 1e-20;
     beta += a1u * u_h2o * a2 * a3 * rds;
     g.du_h2o = a1u * a2 * a3 * rds;
-    g.dq_h2o = a1u * u_h2o * a2 * (p / JUR_P0 * (S - K.h2o_frn) * 1e-20) * rds;
-    g.dp += a1u * u_h2o * a2 * (mix / JUR_P0 * 1e-20) * rds;
+    g.dq_h2o = a1u * u_h2o * a2 * (pp0 * (S - K.h2o_frn) * 1e-20) * rds;
+    g.dp += a1u * u_h2o * a2 * (mix * (1 / JUR_P0) * 1e-20) * rds;
     g.dt += u_h2o * (da1u * a2 * a3 + a1u * da2 * a3 + a1u * a2 * da3) * rds;
   }
   if (M.ctm_n2 && K.n2) {
-    const double qn = 0.79, r = p / JUR_P0 * 273.0 * rt;
+    const double qn = 0.79, r = p * (273.0 / JUR_P0) * rt;
     const double ex = (double)__expf((float)(K.n2_beta * (1 / 296.0 - rt)));
-    const double F = qn + (1 - qn) * (1.294 - 0.4545 * t / 296.0);
+    const double F = qn + (1 - qn) * (1.294 - t * (0.4545 / 296.0));
     const double base = 0.1 * r * r * ex * qn * K.n2_b;
     const double c = base * F;
     beta += c;
@@ -461,7 +462,7 @@ __device__ __forceinline__ double beta_grad_f(const ModelView &M, const CtmChan 
     g.dt += c * (-2 * rt + K.n2_beta * rt * rt) + base * (1 - qn) * (-0.4545 / 296.0);
   }
   if (M.ctm_o2 && K.o2) {
-    const double r = p / JUR_P0 * 273.0 * rt;
+    const double r = p * (273.0 / JUR_P0) * rt;
     const double c = 0.1 * r * r * (double)__expf((float)(K.o2_beta * (1 / 296.0 - rt))) * 0.21 * K.o2_b;
     beta += c;
     g.dp += 2 * c * rp;
@@ -474,6 +475,16 @@ __device__ __forceinline__ double beta_grad_f(const ModelView &M, const CtmChan 
 __device__ __forceinline__ double srcfunc_d(const double *st, const double *sr_col, double t, double &d) {
   const int it = locate_reg(st, TBLNS, t);
   d = (sr_col[it + 1] - sr_col[it]) / (st[it + 1] - st[it]);
+  return sr_col[it] + d * (t - st[it]);
+}
+
+// srcfunc_d on the uniform grid with its inverse spacing inv_dst in hand
+// (products instead of the two divisions; the FP32 Jacobian's sweeps).
+__device__ __forceinline__ double srcfunc_du(const double *st, double inv_dst, const double *sr_col, double t,
+                                             double &d) {
+  int it = (int)((t - st[0]) * inv_dst);
+  it = it < 0 ? 0 : (it > TBLNS - 2 ? TBLNS - 2 : it);
+  d = (sr_col[it + 1] - sr_col[it]) * inv_dst;
   return sr_col[it] + d * (t - st[it]);
 }
 
@@ -1474,6 +1485,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
   double *kt = J.kt + ((size_t)bl * nd + id) * J.ncol * WAVE + lane;
 #define KT(col) kt[(size_t)(col) * WAVE]
   const double *sr_col = M.sr + (size_t)id * TBLNS;
+  const double inv_dst = 1 / (M.st[1] - M.st[0]);
 
   unsigned gm = 0;  // emitters with a table for this channel
   for (int g = 0; g < ng; g++)
@@ -1512,10 +1524,10 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
                      double clx, double tg, BetaGrad &bg, double &src, double &dsrc, double &rt,
                      double &rp, double &beta) {
     if constexpr (F32) {
-      rt = 1 / t;
-      rp = 1 / p;
-      beta = beta_grad_f(M, K, KF, nu, p, t, rt, rp, 1 / ds, q_h2o, u_co2, u_h2o, kwin, clx, bg);
-      src = srcfunc_d(M.st, sr_col, t, dsrc);
+      rt = rcp_d(t);
+      rp = rcp_d(p);
+      beta = beta_grad_f(M, K, KF, nu, p, t, rt, rp, rcp_d(ds), q_h2o, u_co2, u_h2o, kwin, clx, bg);
+      src = srcfunc_du(M.st, inv_dst, sr_col, t, dsrc);
       return tg * (double)__expf((float)(-beta * ds));
     } else {
       rt = 1 / t;
@@ -1770,7 +1782,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
       const double beta_bar = -H * ds;
 
       // u = 10 q p / (kB T) ds
-      const double ucoef = F32 ? 10 * p * rt * ds / JUR_KB : 10 * p / (JUR_KB * t) * ds;
+      const double ucoef = F32 ? 10 * p * rt * ds * (1 / JUR_KB) : 10 * p / (JUR_KB * t) * ds;
       const double uc_bar = beta_bar * bg.du_co2, uh_bar = beta_bar * bg.du_h2o;
       const double uu = uc_bar * u_co2 + uh_bar * u_h2o;
       double v[NQW];

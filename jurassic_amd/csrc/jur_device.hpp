@@ -264,6 +264,18 @@ __device__ __forceinline__ void st32(V *base, unsigned i, V v) {
 }
 
 __device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
+
+// 1/x in FP64 to about an ulp: v_rcp_f64 and two Newton steps (five
+// instructions against about fifteen for a correctly rounded division), for
+// the FP32 Jacobian's sweeps, which need FP64 range and accuracy but not a
+// correctly rounded quotient.  x > 0 and finite.
+__device__ __forceinline__ double rcp_d(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // The reference bisection itself (for columns that are not monotone).
