@@ -949,7 +949,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_chains_f32(ModelView M, BatchView 
     const double gz = W.add(atm, zc, lev, v, 1, p, xm, qi, kt);
     if (PATH) {
       SCQ(J.kz, ip) = gzc + gz;
-      SCQ(J.kz + 1, ip) = gdc + gu / ds;
+      SCQ(J.kz + 1, ip) = gdc + gu * rcp_d(ds);
     }
     Hn = H;
     p = pn;
@@ -1398,7 +1398,7 @@ struct PathRev {
     } else if (lb == -2) {
       // points over more than three levels: straight to the tile
       const unsigned long long los5 = (unsigned long long)PCR(PC_LO);
-      const double f = 2 * hds / PATH_HP;
+      const double f = hds * (2 / PATH_HP);
 #pragma unroll
       for (int q = 0; q < 5; q++) {
         const int lo = (int)(los5 >> (8 * q) & 255u);
@@ -1827,7 +1827,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
         // densities' part comes from the emitter chains
         double gzc = gz;
         if (cloud && atm->cldz > 0) gzc -= beta_bar * kcl * aux * (z - atm->clz) / (atm->cldz * atm->cldz);
-        const double gdc = EMF > 0 ? -H * beta + gu / ds : -H * beta;
+        const double gdc = EMF > 0 ? -H * beta + gu * rcp_d(ds) : -H * beta;
         if constexpr (PFU) {
           if (ip == N - 1) {
             gz_last = gzc + gz_sft;
