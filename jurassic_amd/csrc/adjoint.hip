@@ -1596,7 +1596,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
         nef = ip;
         break;
       }
-      const double tg = q / qp;
+      const double tg = F32 ? q * rcp_d(qp) : q / qp;  // qp > 0: q > 0 was checked
       qp = q;
       BetaGrad bg;
       double src, dsrc, rt, rp, beta;
@@ -1762,7 +1762,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_acc(ModelView M, BatchView B, LosV
         ucur[g] = unx[g];
       }
       if (ip > 0) ld_rev(ip - 1);
-      const double tg = qi_cur / qm;
+      const double tg = F32 ? qi_cur * rcp_d(qm) : qi_cur / qm;
       qi_cur = qm;
       const double z = cu.z, p = cu.p, t = cu.t, ds = cu.ds;
       const double q_h2o = fq_h2o >= 0 ? cu.qh : 0.0;
