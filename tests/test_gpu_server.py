@@ -72,6 +72,11 @@ def _run(gpu, ctl, ret, reqs, precision, lanes=0, forward_lane=0, path_split=0):
             res = srv.wait(i, 120.0)
             assert res.status == 0, res.message
             out.append((res, srv.fetch(i, res)))
+        # worker seconds by phase, then the process's allocation seconds,
+        # count, and the seconds of those that freed a smaller buffer
+        prof = srv.profile()
+        assert len(prof) == 7 and min(prof) >= 0
+        assert prof[5] >= 1 and prof[6] <= prof[4]
         return out, srv.stats()
     finally:
         srv.close()
