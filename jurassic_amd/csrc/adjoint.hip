@@ -437,7 +437,7 @@ __device__ __forceinline__ double beta_grad_f(const ModelView &M, const CtmChan 
     const double S = K.h2o_slf * (double)exp2f(KF.h2o_l2r * (float)(296 - t));
     const double dS = S * KF.h2o_dlr;
     const double bq = KF.bq;
-    const double th = (double)tanhf((float)(bq * rt));
+    const double th = (double)tanh_pos_f((float)(bq * rt));
     const double a1u = nu * th;
     const double da1u = nu * (1 - th * th) * (-bq * rt * rt);
     const double a2 = 296 * rt, da2 = -296 * rt * rt;

@@ -278,6 +278,15 @@ __device__ __forceinline__ double rcp_d(double x) {
 }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// tanh(x) for x >= 0 in float: (e - 1)/(e + 1) with e = exp(2x) from the
+// hardware exponential (relative error about 1e-7 / x below x = 1, a few
+// ulp above; 1 beyond x = 9, where tanhf rounds to 1 too).
+__device__ __forceinline__ float tanh_pos_f(float x) {
+  if (x > 9.f) return 1.f;
+  const float e = __expf(2.f * x);
+  return fdiv(e - 1.f, e + 1.f);
+}
+
 // The reference bisection itself (for columns that are not monotone).
 template <int C>
 __device__ __forceinline__ int col_bisect(const float4 *col, int n, double x, float4 &r) {

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(WAVE) void k_radiance_f32(ModelView M, BatchView B,
       const float uh = (float)uhc;
       const float qh = (float)qhc;
       const float slf = h2o_slf * exp2f(h2o_l2r * (296.f - tf));
-      const float a1 = nuf * uh * tanhf(0.7193876f * rt * nuf);
+      const float a1 = nuf * uh * tanh_pos_f(0.7193876f * rt * nuf);
       const float a3 = pf * (float)(1.0 / JUR_P0) * (qh * slf + (1.f - qh) * h2o_frn) * 1e-20f;
       beta += a1 * (296.f * rt) * a3 * rds;
     }
